@@ -1,0 +1,264 @@
+"""Benchmark: FastGB waveform + log-likelihood throughput on one MI355X.
+
+Workload (one "step" = one fused wave+logL call over the whole batch):
+10^6 binaries, N = 256, Tobs = 4 yr, dt = 15 s, f0 ~ U[1, 6] mHz, amp
+log-uniform [1e-23, 1e-21], chirp mass U[0.2, 1] Msun (fdot, fddot from it),
+isotropic sky/orientation, scored against one 4,207,681-bin complex128 A/E
+stream of analytic-PSD Gaussian noise plus 1000 injected binaries.
+Inputs are resident in HBM before the timed region.
+
+Prints ONE JSON line.  ``value`` = binaries/s of the plain FP64 HIP path;
+the mixed-precision path on the same batch is reported under ``mixed`` when
+the library supports it.  ``roofline`` is for the dominant kernel (the fused
+FP64 wave+logL kernel), from HIP events around the timed launches;
+``cpu_baseline`` times the numpy FP64 oracle on a bounded sample.
+"""
+
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+from pathlib import Path
+
+import numpy as np
+
+ROOT = Path(__file__).resolve().parent
+sys.path.insert(0, str(ROOT))
+
+YRSID_SI = 31558149.763545603
+FP64_PEAK_TFLOPS = 78.6
+HBM_PEAK_GBS = 8000.0
+
+
+def flop_per_binary(N: int) -> float:
+    """Algorithmic FLOP per binary of wave+logL: N*(343 + 15 log2 N) + 200
+    (slow part + TDI + A/E 313/sample, likelihood 30/sample, transform
+    5 N log2 N per channel; FMA = 2 FLOP; transcendentals not counted)."""
+    return N * (343 + 15 * np.log2(N)) + 200
+
+
+def bytes_per_binary(N: int) -> float:
+    """Fused wave+logL HBM bytes per binary: data A/E (32 B) + PSD A/E (16 B)
+    per sample, plus 9 params, 1 output, indices (112 B)."""
+    return 48 * N + 112
+
+
+def make_workload(B, N, T, dt, seed):
+    from tests.helpers import draw_params
+
+    rng = np.random.default_rng(seed)
+    params = draw_params(B, rng, f_lo=1e-3, f_hi=6e-3)
+    return params, rng
+
+
+def build_data(gb, params, N, T, dt, rng, n_inject=1000):
+    import torch
+
+    from gbgpu_amd.noise import A1TDISens, noise_realisation
+
+    Tadj = int(T / dt) * dt
+    df = 1.0 / Tadj
+    L = int(Tadj / dt) // 2 + 1
+    f = np.arange(L) * df
+    f[0] = df
+    S = A1TDISens.get_Sn(f)
+    dev = torch.device("cuda", 0)
+    tmpl = torch.zeros((1, 2, L), dtype=torch.complex128, device=dev)
+    gb.generate_global_template(params[:, :n_inject].T, np.zeros(n_inject, np.int32), tmpl, N=N, T=T, dt=dt)
+    A = tmpl[0, 0] + torch.from_numpy(noise_realisation(S, df, rng)).to(dev)
+    E = tmpl[0, 1] + torch.from_numpy(noise_realisation(S, df, rng)).to(dev)
+    St = torch.from_numpy(S).to(dev)
+    d_d = float((4 * df * (torch.sum(A.abs() ** 2 / St) + torch.sum(E.abs() ** 2 / St))).item())
+    return A.contiguous(), E.contiguous(), St, St.clone(), d_d, df, L
+
+
+def time_calls(fn, steps, warmup):
+    import torch
+
+    for _ in range(warmup):
+        fn()
+    torch.cuda.synchronize()
+    ev0 = torch.cuda.Event(enable_timing=True)
+    ev1 = torch.cuda.Event(enable_timing=True)
+    t0 = time.perf_counter()
+    ev0.record()
+    for _ in range(steps):
+        fn()
+    ev1.record()
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    return ev0.elapsed_time(ev1) / 1e3 / steps, wall / steps
+
+
+def cpu_baseline(N, T, dt, budget_s=12.0, chunk=500):
+    """numpy FP64 oracle (run_wave + brute-force get_ll) on the host, 1 thread."""
+    try:
+        from threadpoolctl import threadpool_limits
+
+        limiter = threadpool_limits(1)
+    except Exception:  # pragma: no cover
+        limiter = None
+    from oracle import fastgb as F
+    from tests.helpers import draw_params
+
+    rng = np.random.default_rng(1234)
+    Tadj = int(T / dt) * dt
+    df = 1 / Tadj
+    L = int(Tadj / dt) // 2 + 1
+    f = np.arange(L) * df
+    f[0] = df
+    from gbgpu_amd.noise import A1TDISens
+
+    S = A1TDISens.get_Sn(f)
+    dA = (rng.standard_normal(L) + 1j * rng.standard_normal(L)) * np.sqrt(S / (8 * df))
+    dE = (rng.standard_normal(L) + 1j * rng.standard_normal(L)) * np.sqrt(S / (8 * df))
+    done = 0
+    t0 = time.perf_counter()
+    while time.perf_counter() - t0 < budget_s:
+        p = draw_params(chunk, rng, f_lo=1e-3, f_hi=6e-3)
+        o = F.run_wave(*p, N=N, T=T, dt=dt)
+        d_h, h_h = F.inner_terms(o["AETf"][:, 0], o["AETf"][:, 1], o["start_inds"], dA, dE, S, S, df)
+        F.loglike(1.0, d_h, h_h)
+        done += chunk
+    el = time.perf_counter() - t0
+    if limiter is not None:
+        limiter.unregister() if hasattr(limiter, "unregister") else None
+    return dict(
+        value=done / el,
+        unit="binaries/s",
+        cores=1,
+        kind="port",
+        sample=f"{done} binaries of the same workload (N={N}, 4 yr) through the numpy FP64 oracle "
+        f"run_wave + get_ll in chunks of {chunk}, {el:.1f} s on 1 host thread",
+    )
+
+
+def parity_sample(gb, params, A, E, SA, SE, d_d, df, N, T, dt, precision, n=64):
+    from oracle import fastgb as F
+
+    p = params[:, :n]
+    ll = gb.get_ll(p, [A, E], [SA, SE], N=N, T=T, dt=dt, precision=precision)
+    o = F.run_wave(*p, N=N, T=T, dt=dt)
+    d_h, h_h = F.inner_terms(o["AETf"][:, 0], o["AETf"][:, 1], o["start_inds"], A.cpu().numpy(), E.cpu().numpy(),
+                             SA.cpu().numpy(), SE.cpu().numpy(), df)
+    ll_o = F.loglike(d_d, d_h, h_h)
+    gb.run_wave(*p, N=N, T=T, dt=dt, precision=precision)
+    aet = gb.AETf.cpu().numpy()[:, :2]
+    werr = np.sqrt(np.sum(np.abs(aet - o["AETf"][:, :2]) ** 2, axis=(1, 2)) / np.sum(np.abs(o["AETf"][:, :2]) ** 2,
+                                                                                     axis=(1, 2)))
+    return dict(n=n, max_abs_logL_err=float(np.max(np.abs(ll - ll_o))), max_rel_wave_err=float(werr.max()))
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--num-bin", type=int, default=1_000_000)
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
+    ap.add_argument("--cpu-budget", type=float, default=12.0)
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(json.dumps({"error": "this build measures one GPU only; --gpus must be 1"}))
+        sys.exit(2)
+
+    import torch
+
+    from gbgpu_amd import GBGPU, _capi
+
+    B, N, dt = args.num_bin, 256, 15.0
+    T = 4 * YRSID_SI
+    print(f"[bench] workload B={B} N={N} T=4yr", file=sys.stderr, flush=True)
+    params, rng = make_workload(B, N, T, dt, seed=2024)
+    gb = GBGPU()
+    A, E, SA, SE, d_d, df, L = build_data(gb, params, N, T, dt, rng)
+    gb.d_d = d_d
+    dev = A.device
+    p_dev = torch.from_numpy(np.ascontiguousarray(params)).to(dev)
+    ll = torch.empty(B, dtype=torch.float64, device=dev)
+    lib = _capi.load()
+    Tadj = int(T / dt) * dt
+
+    def runner(precision):
+        cfg, _tab = gb._cfg(N, Tadj, False, precision)
+        dat = _capi.Data(A=A.data_ptr(), E=E.data_ptr(), psdA=SA.data_ptr(), psdE=SE.data_ptr(), data_index=None,
+                         noise_index=None, data_length=L, start_freq_ind=0, df=df)
+        st = _capi.stream_handle()
+
+        def call():
+            rc = lib.gbgpu_wave_ll(ll.data_ptr(), None, None, p_dev.data_ptr(), 9, B, cfg, dat, d_d, 0, st)
+            _capi.check(rc, "gbgpu_wave_ll")
+
+        return call, _tab
+
+    call64, _t64 = runner("fp64")
+    print("[bench] timing fp64", file=sys.stderr, flush=True)
+    sec64, wall64 = time_calls(call64, args.steps, args.warmup)
+    par64 = parity_sample(gb, params, A, E, SA, SE, d_d, df, N, T, dt, "fp64")
+    value = B / sec64
+    flops = flop_per_binary(N) * B
+    achieved_tflops = flops / sec64 / 1e12
+
+    mixed = None
+    try:
+        callm, _tm = runner("mixed")
+        callm()
+        torch.cuda.synchronize()
+        print("[bench] timing mixed", file=sys.stderr, flush=True)
+        secm, _ = time_calls(callm, args.steps, args.warmup)
+        parm = parity_sample(gb, params, A, E, SA, SE, d_d, df, N, T, dt, "mixed")
+        mixed = dict(value=B / secm, ms_per_step=secm * 1e3, speedup_vs_fp64=sec64 / secm, parity=parm)
+    except RuntimeError as e:
+        mixed = dict(error=str(e)[:200])
+
+    cpu = None
+    if not args.no_cpu:
+        print("[bench] cpu baseline", file=sys.stderr, flush=True)
+        cpu = cpu_baseline(N, T, dt, budget_s=args.cpu_budget)
+
+    out = {
+        "metric": "binaries/sec (wave+logL), plain FP64 vs batched mixed, at <=1e-6 rel err",
+        "value": value,
+        "unit": "binaries/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": sec64 * 1e3,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic (seeded parameters; analytic-PSD Gaussian noise plus 1000 injected binaries)",
+        "config": {
+            "workload": "plain FP64 fused wave+logL, uniform batch: 1e6 binaries, N=256, Tobs=4 yr, dt=15 s, "
+            "f0 U[1,6] mHz, one 4,207,681-bin A/E stream",
+            "num_bin": B,
+            "N": N,
+            "Tobs_s": Tadj,
+            "data_length": L,
+        },
+        "roofline": {
+            "bound": "mfma",
+            "compute": "FP64 VALU (MI355X FP64 vector and matrix peaks are both 78.6 TFLOP/s)",
+            "achieved": achieved_tflops,
+            "peak": FP64_PEAK_TFLOPS,
+            "unit": "TFLOP/s",
+            "frac": achieved_tflops / FP64_PEAK_TFLOPS,
+            "traffic": None,
+            "kernel": "k_wave_f64<256, MODE_LL>",
+            "flop_per_binary": flop_per_binary(N),
+            "algorithmic_hbm_GBs": bytes_per_binary(N) * B / sec64 / 1e9,
+        },
+        "cpu_baseline": cpu,
+        "parity": par64,
+        "mixed": mixed,
+        "wall_ms_per_step": wall64 * 1e3,
+    }
+    print(json.dumps(out))
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,124 @@
+"""ctypes binding of the C ABI in include/gbgpu_hip.h.
+
+The library ``gbgpu_amd/libgbgpu_hip.so`` is built in-tree by
+``gbgpu_amd/build.py``.  Loading fails loudly (``RuntimeError``) when it is
+missing: there is no CPU fallback behind the HIP backend.  Every wrapper
+raises ``RuntimeError`` carrying ``gbgpu_last_error()`` when the library
+returns a non-zero status.
+"""
+
+from __future__ import annotations
+
+import ctypes
+import os
+from pathlib import Path
+
+LIB_PATH = Path(__file__).resolve().parent / "libgbgpu_hip.so"
+
+GBGPU_FP64 = 0
+GBGPU_MIXED = 1
+ORBIT_ROWS = 19
+
+c_void_p = ctypes.c_void_p
+c_int = ctypes.c_int
+c_double = ctypes.c_double
+c_longlong = ctypes.c_longlong
+
+
+class WaveCfg(ctypes.Structure):
+    """``gbgpu_wave_cfg``."""
+
+    _fields_ = [
+        ("N", c_int),
+        ("T", c_double),
+        ("armlength", c_double),
+        ("orbit_table", c_void_p),
+        ("tdi2", c_int),
+        ("precision", c_int),
+    ]
+
+
+class Data(ctypes.Structure):
+    """``gbgpu_data``."""
+
+    _fields_ = [
+        ("A", c_void_p),
+        ("E", c_void_p),
+        ("psdA", c_void_p),
+        ("psdE", c_void_p),
+        ("data_index", c_void_p),
+        ("noise_index", c_void_p),
+        ("data_length", c_int),
+        ("start_freq_ind", c_int),
+        ("df", c_double),
+    ]
+
+
+# name -> (restype, argtypes) for every symbol declared in include/gbgpu_hip.h
+SIGNATURES = {
+    "gbgpu_get_ll": (
+        c_int,
+        [c_void_p] * 8 + [c_double, c_void_p, c_int, c_int, c_void_p, c_void_p, c_int, c_void_p],
+    ),
+    "gbgpu_fill_global": (
+        c_int,
+        [c_void_p] * 5 + [c_int, c_int, c_void_p, c_int, c_int, c_void_p],
+    ),
+    "gbgpu_fill_global_rows": (
+        c_int,
+        [c_void_p] * 4 + [c_longlong, c_void_p, c_longlong, c_int, c_int, c_void_p, c_int, c_int, c_void_p],
+    ),
+    "gbgpu_run_wave": (
+        c_int,
+        [c_void_p, c_void_p, c_void_p, c_void_p, c_int, c_int, ctypes.POINTER(WaveCfg), c_void_p],
+    ),
+    "gbgpu_wave_ll": (
+        c_int,
+        [c_void_p, c_void_p, c_void_p, c_void_p, c_int, c_int, ctypes.POINTER(WaveCfg), ctypes.POINTER(Data),
+         c_double, c_int, c_void_p],
+    ),
+    "gbgpu_last_error": (ctypes.c_char_p, []),
+    "gbgpu_abi_version": (c_int, []),
+}
+
+_lib = None
+
+
+def load(path: os.PathLike | None = None):
+    """Load (once) and return the ctypes library handle."""
+    global _lib
+    if _lib is not None and path is None:
+        return _lib
+    p = Path(path) if path is not None else LIB_PATH
+    if not p.exists():
+        raise RuntimeError(
+            f"HIP library {p} is missing: build it with `python -m gbgpu_amd.build` "
+            "(the hip backend has no CPU fallback)"
+        )
+    lib = ctypes.CDLL(str(p))
+    for name, (res, args) in SIGNATURES.items():
+        fn = getattr(lib, name)
+        fn.restype = res
+        fn.argtypes = args
+    if path is None:
+        _lib = lib
+    return lib
+
+
+def check(rc: int, what: str) -> None:
+    if rc != 0:
+        msg = load().gbgpu_last_error()
+        raise RuntimeError(f"{what} failed ({rc}): {msg.decode() if msg else ''}")
+
+
+def ptr(t) -> int | None:
+    """Device pointer of a torch tensor (None passes NULL)."""
+    if t is None:
+        return None
+    return t.data_ptr()
+
+
+def stream_handle(device=None) -> int:
+    import torch
+
+    return torch.cuda.current_stream(device).cuda_stream

@@ -1,0 +1,291 @@
+// Drop-in likelihood and global-template kernels (gfx950).
+//
+// get_ll: templates arrive (M, num_bin)-major as in the reference
+// (src/gbgpu/cutils/gbgpu_utils.cu:125-185).  A wavefront owns 8 binaries;
+// lane -> (binary = lane & 7, row = lane >> 3), so every load instruction
+// touches 8 template rows x 8 adjacent binaries (8 full 128-B lines) and
+// 8 binaries x 8 consecutive data/PSD bins.  Each binary's partial sums are
+// reduced with xor shuffles over the 8 lanes that share it.
+//
+// fill_global: deterministic replacement of the reference's atomic scatter
+// (gbgpu_utils.cu:55-101).  Binaries are stably radix-sorted by
+// key = group * (L + M) + (start + M); a wavefront then owns a 64-bin output
+// tile, finds the contiguous key range of binaries overlapping it by binary
+// search, and sums their samples in sorted order -- one thread per output
+// bin, fixed summation order, so results are bit-identical run to run.
+#include <hipcub/hipcub.hpp>
+
+#include "common.hpp"
+
+namespace gbgpu {
+
+constexpr unsigned long long KEY_NONE = ~0ULL;
+
+__global__ void __launch_bounds__(256) k_get_ll(double2 *__restrict__ d_h, double2 *__restrict__ h_h,
+                                                const double2 *__restrict__ At, const double2 *__restrict__ Et,
+                                                const double2 *__restrict__ Ad, const double2 *__restrict__ Ed,
+                                                const double *__restrict__ SA, const double *__restrict__ SE,
+                                                double df4, const int *__restrict__ start, int M, long long B,
+                                                const int *__restrict__ di, const int *__restrict__ ni,
+                                                long long L) {
+    const int lane = threadIdx.x & 63;
+    const int wave = threadIdx.x >> 6;
+    const int bsub = lane & 7, row = lane >> 3;
+    const long long b = ((long long)blockIdx.x * 4 + wave) * 8 + bsub;
+    const bool valid = b < B;
+    double dhr = 0.0, dhi = 0.0, hh = 0.0;
+    if (valid) {
+        const long long s = start[b];
+        const long long doff = (di ? (long long)di[b] : 0LL) * L;
+        const long long noff = (ni ? (long long)ni[b] : 0LL) * L;
+        for (int i = row; i < M; i += 8) {
+            const long long j = s + i;
+            if (j < 0 || j >= L) continue;
+            const long long t = (long long)i * B + b;
+            const double2 hA = At[t], hE = Et[t];
+            const double2 dA = Ad[doff + j], dE = Ed[doff + j];
+            const double iA = 1.0 / SA[noff + j], iE = 1.0 / SE[noff + j];
+            dhr += (dA.x * hA.x + dA.y * hA.y) * iA + (dE.x * hE.x + dE.y * hE.y) * iE;
+            dhi += (dA.x * hA.y - dA.y * hA.x) * iA + (dE.x * hE.y - dE.y * hE.x) * iE;
+            hh += (hA.x * hA.x + hA.y * hA.y) * iA + (hE.x * hE.x + hE.y * hE.y) * iE;
+        }
+    }
+#pragma unroll
+    for (int off = 8; off < 64; off <<= 1) {
+        dhr += __shfl_xor(dhr, off, 64);
+        dhi += __shfl_xor(dhi, off, 64);
+        hh += __shfl_xor(hh, off, 64);
+    }
+    if (valid && row == 0) {
+        d_h[b] = make_double2(df4 * dhr, df4 * dhi);
+        h_h[b] = make_double2(df4 * hh, 0.0);
+    }
+}
+
+// ------------------------------------------------------------ fill_global ---
+__global__ void k_fill_keys(unsigned long long *__restrict__ keys, int *__restrict__ vals,
+                            const int *__restrict__ start, const long long *__restrict__ start64,
+                            const int *__restrict__ group, long long B, int M, long long L, int G,
+                            long long start_shift) {
+    const long long b = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (b >= B) return;
+    const long long s = (start ? (long long)start[b] : start64[b]) - start_shift;
+    const long long g = group ? group[b] : 0;
+    unsigned long long key = KEY_NONE;
+    if (g >= 0 && g < G && s > -(long long)M && s < L)
+        key = (unsigned long long)g * (unsigned long long)(L + M) + (unsigned long long)(s + M);
+    keys[b] = key;
+    vals[b] = (int)b;
+}
+
+// (M, B)-major templates -> binary-major rows (B, M), both channels; 64x64
+// tiles through LDS so both the reads and the writes are coalesced.
+__global__ void __launch_bounds__(256) k_transpose(double2 *__restrict__ rowsA, double2 *__restrict__ rowsE,
+                                                   const double2 *__restrict__ At,
+                                                   const double2 *__restrict__ Et, int M, long long B) {
+    __shared__ double2 tA[64][65], tE[64][65];
+    const long long b0 = (long long)blockIdx.x * 64;
+    const int i0 = blockIdx.y * 64;
+    const int tx = threadIdx.x & 63, ty = threadIdx.x >> 6;  // 64 x 4
+    for (int r = ty; r < 64; r += 4) {
+        const int i = i0 + r;
+        const long long b = b0 + tx;
+        if (i < M && b < B) {
+            tA[r][tx] = At[(long long)i * B + b];
+            tE[r][tx] = Et[(long long)i * B + b];
+        }
+    }
+    __syncthreads();
+    for (int r = ty; r < 64; r += 4) {
+        const long long b = b0 + r;
+        const int i = i0 + tx;
+        if (i < M && b < B) {
+            rowsA[b * M + i] = tA[tx][r];
+            rowsE[b * M + i] = tE[tx][r];
+        }
+    }
+}
+
+__device__ __forceinline__ long long lower_bound_u64(const unsigned long long *__restrict__ a, long long n,
+                                                     unsigned long long x) {
+    long long lo = 0, hi = n;
+    while (lo < hi) {
+        const long long mid = (lo + hi) >> 1;
+        if (a[mid] < x)
+            lo = mid + 1;
+        else
+            hi = mid;
+    }
+    return lo;
+}
+
+// One wavefront per (group, 64-bin tile).  rows: binary-major, row b starts
+// at rows + b * row_stride; sample i of binary b is rows[b * row_stride + i].
+__global__ void __launch_bounds__(256) k_gather_sum(double2 *__restrict__ Aglob, double2 *__restrict__ Eglob,
+                                                    const double2 *__restrict__ rowsA,
+                                                    const double2 *__restrict__ rowsE, long long row_stride,
+                                                    const unsigned long long *__restrict__ keys,
+                                                    const int *__restrict__ perm, long long B, int M, long long L,
+                                                    long long tiles_per_group, long long num_tiles) {
+    const int lane = threadIdx.x & 63;
+    const long long tile = (long long)blockIdx.x * 4 + (threadIdx.x >> 6);
+    if (tile >= num_tiles) return;
+    const long long g = tile / tiles_per_group;
+    const long long j0 = (tile - g * tiles_per_group) * 64;
+    const unsigned long long KS = (unsigned long long)(L + M);
+    const unsigned long long base = (unsigned long long)g * KS;
+    // binaries with start in [j0 - M + 1, j0 + 63], i.e. key in [base + j0 + 1, base + j0 + 63 + M]
+    const long long lo = lower_bound_u64(keys, B, base + (unsigned long long)j0 + 1ULL);
+    const long long hi = lower_bound_u64(keys, B, base + (unsigned long long)(j0 + 64 + M));
+    const long long j = j0 + lane;
+    double ar = 0.0, ai = 0.0, er = 0.0, ei = 0.0;
+    for (long long r = lo; r < hi; r++) {
+        const long long s = (long long)(keys[r] - base) - M;
+        const long long i = j - s;
+        if (i >= 0 && i < M) {
+            const long long off = (long long)perm[r] * row_stride + i;
+            const double2 a = rowsA[off], e = rowsE[off];
+            ar += a.x;
+            ai += a.y;
+            er += e.x;
+            ei += e.y;
+        }
+    }
+    if (j < L && hi > lo) {
+        const long long o = g * L + j;
+        double2 A = Aglob[o], E = Eglob[o];
+        A.x += ar;
+        A.y += ai;
+        E.x += er;
+        E.y += ei;
+        Aglob[o] = A;
+        Eglob[o] = E;
+    }
+}
+
+// Sort binaries by (group, start) and run the gather-sum over binary-major rows.
+static int fill_rows_impl(double2 *Aglob, double2 *Eglob, const double2 *rowsA, const double2 *rowsE,
+                          long long row_stride, const int *start, const long long *start64, long long start_shift,
+                          int M, long long B, const int *group, int G, long long L, hipStream_t st) {
+    if (B == 0 || G == 0) return GBGPU_OK;
+    unsigned long long *keys = nullptr, *keys_sorted = nullptr;
+    int *vals = nullptr, *perm = nullptr;
+    void *tmp = nullptr;
+    size_t tmp_bytes = 0;
+    hipError_t e;
+    // end_bit: keys are < G * (L + M) <= 2^62 or KEY_NONE; sorting the full
+    // 64 bits keeps KEY_NONE last.
+    e = hipcub::DeviceRadixSort::SortPairs(nullptr, tmp_bytes, keys, keys_sorted, vals, perm, (int)B, 0, 64, st);
+    if (e != hipSuccess) return hip_fail(e, "SortPairs(size)");
+    const size_t kb = sizeof(unsigned long long) * (size_t)B, vb = sizeof(int) * (size_t)B;
+    char *ws = nullptr;
+    const size_t total = 2 * kb + 2 * vb + tmp_bytes + 256 * 4;
+    e = hipMallocAsync((void **)&ws, total, st);
+    if (e != hipSuccess) return hip_fail(e, "hipMallocAsync(fill workspace)");
+    size_t off = 0;
+    auto carve = [&](size_t n) {
+        char *p = ws + off;
+        off += (n + 255) & ~(size_t)255;
+        return p;
+    };
+    keys = (unsigned long long *)carve(kb);
+    keys_sorted = (unsigned long long *)carve(kb);
+    vals = (int *)carve(vb);
+    perm = (int *)carve(vb);
+    tmp = carve(tmp_bytes);
+    hipLaunchKernelGGL(k_fill_keys, dim3((unsigned)((B + 255) / 256)), dim3(256), 0, st, keys, vals, start, start64,
+                       group, B, M, L, G, start_shift);
+    e = hipcub::DeviceRadixSort::SortPairs(tmp, tmp_bytes, keys, keys_sorted, vals, perm, (int)B, 0, 64, st);
+    if (e != hipSuccess) {
+        (void)hipFreeAsync(ws, st);
+        return hip_fail(e, "SortPairs");
+    }
+    const long long tpg = (L + 63) / 64;
+    const long long nt = tpg * G;
+    hipLaunchKernelGGL(k_gather_sum, dim3((unsigned)((nt + 3) / 4)), dim3(256), 0, st, Aglob, Eglob, rowsA, rowsE,
+                       row_stride, keys_sorted, perm, B, M, L, tpg, nt);
+    e = hipGetLastError();
+    hipError_t e2 = hipFreeAsync(ws, st);
+    if (e != hipSuccess) return hip_fail(e, "k_gather_sum");
+    if (e2 != hipSuccess) return hip_fail(e2, "hipFreeAsync");
+    return GBGPU_OK;
+}
+
+}  // namespace gbgpu
+
+using namespace gbgpu;
+
+extern "C" int gbgpu_get_ll(gbgpu_cmplx *d_h, gbgpu_cmplx *h_h, const gbgpu_cmplx *A_template,
+                            const gbgpu_cmplx *E_template, const gbgpu_cmplx *A_data, const gbgpu_cmplx *E_data,
+                            const double *A_psd, const double *E_psd, double df, const int *start_ind, int M,
+                            int num_bin, const int *data_index, const int *noise_index, int data_length,
+                            void *stream) {
+    if (num_bin < 0 || M < 0 || data_length <= 0) {
+        set_error("gbgpu_get_ll: negative num_bin/M or data_length <= 0");
+        return GBGPU_E_ARG;
+    }
+    if (num_bin == 0) return GBGPU_OK;
+    if (!d_h || !h_h || !A_template || !E_template || !A_data || !E_data || !A_psd || !E_psd || !start_ind) {
+        set_error("gbgpu_get_ll: null pointer argument");
+        return GBGPU_E_ARG;
+    }
+    hipStream_t st = static_cast<hipStream_t>(stream);
+    const long long waves = ((long long)num_bin + 7) / 8;
+    hipLaunchKernelGGL(k_get_ll, dim3((unsigned)((waves + 3) / 4)), dim3(256), 0, st, (double2 *)d_h, (double2 *)h_h,
+                       (const double2 *)A_template, (const double2 *)E_template, (const double2 *)A_data,
+                       (const double2 *)E_data, A_psd, E_psd, 4.0 * df, start_ind, M, (long long)num_bin,
+                       data_index, noise_index, (long long)data_length);
+    GBGPU_CHECK_LAUNCH("k_get_ll");
+    return GBGPU_OK;
+}
+
+extern "C" int gbgpu_fill_global(gbgpu_cmplx *A_glob, gbgpu_cmplx *E_glob, const gbgpu_cmplx *A_template,
+                                 const gbgpu_cmplx *E_template, const int *start_ind, int M, int num_bin,
+                                 const int *group_index, int data_length, int num_groups, void *stream) {
+    if (num_bin < 0 || M < 0 || data_length <= 0 || num_groups < 0) {
+        set_error("gbgpu_fill_global: negative size argument");
+        return GBGPU_E_ARG;
+    }
+    if (num_bin == 0 || M == 0 || num_groups == 0) return GBGPU_OK;
+    if (!A_glob || !E_glob || !A_template || !E_template || !start_ind || !group_index) {
+        set_error("gbgpu_fill_global: null pointer argument");
+        return GBGPU_E_ARG;
+    }
+    hipStream_t st = static_cast<hipStream_t>(stream);
+    const size_t rows_bytes = sizeof(double2) * (size_t)num_bin * (size_t)M;
+    double2 *rows = nullptr;
+    hipError_t e = hipMallocAsync((void **)&rows, 2 * rows_bytes, st);
+    if (e != hipSuccess) return hip_fail(e, "hipMallocAsync(rows)");
+    dim3 grid((unsigned)(((long long)num_bin + 63) / 64), (unsigned)((M + 63) / 64));
+    hipLaunchKernelGGL(k_transpose, grid, dim3(256), 0, st, rows, rows + (size_t)num_bin * M,
+                       (const double2 *)A_template, (const double2 *)E_template, M, (long long)num_bin);
+    e = hipGetLastError();
+    if (e != hipSuccess) {
+        (void)hipFreeAsync(rows, st);
+        return hip_fail(e, "k_transpose");
+    }
+    int rc = fill_rows_impl((double2 *)A_glob, (double2 *)E_glob, rows, rows + (size_t)num_bin * M, M, start_ind,
+                            nullptr, 0, M, num_bin, group_index, num_groups, data_length, st);
+    e = hipFreeAsync(rows, st);
+    if (rc) return rc;
+    if (e != hipSuccess) return hip_fail(e, "hipFreeAsync(rows)");
+    return GBGPU_OK;
+}
+
+extern "C" int gbgpu_fill_global_rows(gbgpu_cmplx *A_glob, gbgpu_cmplx *E_glob, const gbgpu_cmplx *A_rows,
+                                      const gbgpu_cmplx *E_rows, long long row_stride, const long long *start_inds,
+                                      long long start_freq_ind, int M, int num_bin, const int *group_index,
+                                      int data_length, int num_groups, void *stream) {
+    if (num_bin < 0 || M < 0 || data_length <= 0 || num_groups < 0 || row_stride < M) {
+        set_error("gbgpu_fill_global_rows: bad size argument");
+        return GBGPU_E_ARG;
+    }
+    if (num_bin == 0 || M == 0 || num_groups == 0) return GBGPU_OK;
+    if (!A_glob || !E_glob || !A_rows || !E_rows || !start_inds) {
+        set_error("gbgpu_fill_global_rows: null pointer argument");
+        return GBGPU_E_ARG;
+    }
+    return fill_rows_impl((double2 *)A_glob, (double2 *)E_glob, (const double2 *)A_rows, (const double2 *)E_rows,
+                          row_stride, nullptr, start_inds, start_freq_ind, M, num_bin, group_index, num_groups,
+                          data_length, static_cast<hipStream_t>(stream));
+}

@@ -1,0 +1,60 @@
+// Host-side runtime of the library: per-thread error messages and the
+// per-(device, N) twiddle caches.  No device synchronisation happens here
+// except the one-time blocking upload of a new twiddle table.
+#include <cmath>
+#include <map>
+#include <mutex>
+#include <string>
+#include <vector>
+
+#include "common.hpp"
+
+namespace gbgpu {
+
+static thread_local std::string g_last_error;
+
+void set_error(const std::string &msg) { g_last_error = msg; }
+
+int hip_fail(hipError_t e, const char *where) {
+    set_error(std::string(where) + ": " + hipGetErrorString(e));
+    return GBGPU_E_HIP;
+}
+
+namespace {
+std::mutex g_tw_mutex;
+std::map<std::pair<int, int>, void *> g_tw64, g_tw32;
+
+template <class C>
+const C *get_twiddles(std::map<std::pair<int, int>, void *> &cache, int N) {
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess) return nullptr;
+    std::lock_guard<std::mutex> lock(g_tw_mutex);
+    auto key = std::make_pair(dev, N);
+    auto it = cache.find(key);
+    if (it != cache.end()) return static_cast<const C *>(it->second);
+    std::vector<C> host(N);
+    for (int k = 0; k < N; k++) {
+        // long double keeps the table correctly rounded in float64
+        long double ang = -2.0L * 3.141592653589793238462643383279502884L * (long double)k / (long double)N;
+        host[k].x = (decltype(host[k].x))std::cos(ang);
+        host[k].y = (decltype(host[k].y))std::sin(ang);
+    }
+    void *dptr = nullptr;
+    if (hipMalloc(&dptr, sizeof(C) * N) != hipSuccess) return nullptr;
+    if (hipMemcpy(dptr, host.data(), sizeof(C) * N, hipMemcpyHostToDevice) != hipSuccess) {
+        (void)hipFree(dptr);
+        return nullptr;
+    }
+    cache[key] = dptr;
+    return static_cast<const C *>(dptr);
+}
+}  // namespace
+
+const double2 *twiddles_f64(int N) { return get_twiddles<double2>(g_tw64, N); }
+const float2 *twiddles_f32(int N) { return get_twiddles<float2>(g_tw32, N); }
+
+}  // namespace gbgpu
+
+extern "C" const char *gbgpu_last_error(void) { return gbgpu::g_last_error.c_str(); }
+
+extern "C" int gbgpu_abi_version(void) { return 1; }

@@ -1,0 +1,242 @@
+// Per-binary constants and the per-sample FastGB slow part (device code).
+//
+// Equations restate src/gbgpu/gbgpu.py:243-316 (per-binary set-up),
+// :399-540 (slow part) and :348-388 (TDI X, Y, Z), with the third-body hooks
+// of src/gbgpu/thirdbody.py:155-226 and :228-428.
+#pragma once
+
+#include "common.hpp"
+
+namespace gbgpu {
+
+// Per-binary constants, computed once per binary and broadcast through LDS.
+struct Src {
+    double amp, f0, fdot, fddot, phi0;
+    double om;   // 2 pi f0                          (gbgpu.py:464)
+    double omq;  // om - 2 pi q / T                  (gbgpu.py:462-463, :497)
+    double k[3], u[3], v[3];                      // (gbgpu.py:254-258)
+    double DPr, DPi, DCr, DCi;                    // (gbgpu.py:250-251)
+    double A2, varpi, e2, n2, T2;                 // third body (thirdbody.py:39-80)
+    double tdr, tdi;                              // TDI2 factor, (1, 0) for TDI1
+    long long start;                              // rint(f0 T) - N/2 (gbgpu.py:296, :387, :462)
+};
+
+__device__ inline Src make_src(const double *__restrict__ params, int num_params, long long B, long long b,
+                               int N, double T, int tdi2, double armlength) {
+    Src s;
+    const double amp = params[0 * B + b];
+    const double f0 = params[1 * B + b];
+    const double fdot = params[2 * B + b];
+    const double fddot = params[3 * B + b];
+    const double phi0 = params[4 * B + b];
+    const double iota = params[5 * B + b];
+    const double psi = params[6 * B + b];
+    const double lam = params[7 * B + b];
+    const double beta = params[8 * B + b];
+    s.amp = amp;
+    s.f0 = f0;
+    s.fdot = fdot;
+    s.fddot = fddot;
+    s.phi0 = phi0;
+    const double theta = PI / 2 - beta;
+    const double cosiota = cos(iota);
+    double sinps, cosps;
+    sincos(2.0 * psi, &sinps, &cosps);
+    const double Aplus = amp * (1.0 + cosiota * cosiota);
+    const double Across = -2.0 * amp * cosiota;
+    // DP = Aplus cos2psi - i Across sin2psi ; DC = -Aplus sin2psi - i Across cos2psi
+    s.DPr = Aplus * cosps;
+    s.DPi = -Across * sinps;
+    s.DCr = -Aplus * sinps;
+    s.DCi = -Across * cosps;
+    double sinth, costh, sinph, cosph;
+    sincos(theta, &sinth, &costh);
+    sincos(lam, &sinph, &cosph);
+    s.u[0] = costh * cosph;
+    s.u[1] = costh * sinph;
+    s.u[2] = -sinth;
+    s.v[0] = sinph;
+    s.v[1] = -cosph;
+    s.v[2] = 0.0;
+    s.k[0] = -sinth * cosph;
+    s.k[1] = -sinth * sinph;
+    s.k[2] = -costh;
+    const double q = rint(f0 * T);  // round half to even, as numpy.rint
+    s.om = 2.0 * PI * f0;
+    s.omq = s.om - 2.0 * PI * (q / T);
+    s.start = (long long)q - (long long)(N / 2);
+    if (num_params >= GBGPU_NPARAMS_THIRD) {
+        s.A2 = params[9 * B + b];
+        s.varpi = params[10 * B + b];
+        s.e2 = params[11 * B + b];
+        s.n2 = params[12 * B + b];
+        s.T2 = params[13 * B + b];
+    } else {
+        s.A2 = s.varpi = s.e2 = s.n2 = s.T2 = 0.0;
+    }
+    if (tdi2) {
+        // 2i sin(2 wL) e^{-2 i wL}, wL = 2 pi f0 L / c   (gbgpu.py:301-304)
+        const double omegaL = 2 * PI * f0 * (armlength / C_SI);
+        double s2, c2;
+        sincos(2 * omegaL, &s2, &c2);
+        const double sn = sin(2 * omegaL);
+        // 2i * sn * (c2 - i s2) = 2 sn s2 + 2i sn c2
+        s.tdr = 2.0 * sn * s2;
+        s.tdi = 2.0 * sn * c2;
+    } else {
+        s.tdr = 1.0;
+        s.tdi = 0.0;
+    }
+    return s;
+}
+
+// ------------------------------------------------------------ third body ---
+// Mikkola inverse of Kepler's equation, restating src/gbgpu/thirdbody.py:228-320
+// element by element (including its branch structure).
+__device__ inline double kepler_u(double l, double e) {
+    const bool neg = l < 0;
+    l = neg ? -l : l;
+    const bool over2pi = l > 2 * PI;
+    double mult = 0.0;
+    if (over2pi) {
+        mult = floor(l / (2 * PI));
+        l -= mult * 2.0 * PI;
+    }
+    const bool overpi = l > PI;
+    if (overpi) l = 2.0 * PI - l;
+    const double alpha = (1.0 - e) / (4.0 * e + 0.5);
+    const double beta = 0.5 * l / (4.0 * e + 0.5);
+    double z = sqrt(beta * beta + alpha * alpha * alpha);
+    z = neg ? beta - z : beta + z;
+    z = (z < 0) ? -pow(-z, 1.0 / 3.0) : pow(z, 1.0 / 3.0);
+    const double s = z - alpha / z;
+    const double w = s - 0.078 * s * s * s * s * s / (1.0 + e);
+    const double u0 = l + e * (3.0 * w - 4.0 * w * w * w);
+    double su0, cu0;
+    sincos(u0, &su0, &cu0);
+    const double f = u0 - e * su0 - l;
+    const double f1 = 1.0 - e * cu0;
+    double f2 = u0 - f - l;
+    double f3 = 1.0 - f1;
+    double f4 = -f2;
+    f2 *= 0.5;
+    f3 *= 0.166666666666667;
+    f4 *= 0.0416666666666667;
+    const double u1 = -f / f1;
+    const double u2 = -f / (f1 + f2 * u1);
+    const double u3 = -f / (f1 + f2 * u2 + f3 * u2 * u2);
+    const double u4 = -f / (f1 + f2 * u3 + f3 * u3 * u3 + f4 * u3 * u3 * u3);
+    double u = u0 + u4;
+    if (overpi) u = 2.0 * PI - u;
+    if (over2pi) u = 2.0 * PI * mult + u;
+    if (neg) u = -u;
+    return u;
+}
+
+// Line-of-sight velocity (m/s), src/gbgpu/thirdbody.py:322-380.
+__device__ inline double third_vlos(const Src &s, double t) {
+    double u = kepler_u(s.n2 * (t - s.T2), s.e2);
+    if (s.e2 > 1e-6) {
+        const double bb = (1.0 - sqrt(1.0 - s.e2 * s.e2)) / s.e2;
+        double su, cu;
+        sincos(u, &su, &cu);
+        u += 2.0 * atan2(bb * su, 1.0 - bb * cu);
+    }
+    return s.A2 * (sin(u + s.varpi) + s.e2 * sin(s.varpi));
+}
+
+// ------------------------------------------------------ FP64 slow part ---
+struct SampleGeom {
+    double t;
+    double kP[3];   // k . P_s / c
+    double kr12, kr13, kr23;
+    double ur12, vr12, ur13, vr13, ur23, vr23;
+};
+
+__device__ __forceinline__ SampleGeom load_geom(const Src &s, const double *__restrict__ tab, int N, int n) {
+    SampleGeom g;
+    g.t = tab[n];
+    double P[9], r[9];
+#pragma unroll
+    for (int i = 0; i < 9; i++) P[i] = tab[(1 + i) * N + n];
+#pragma unroll
+    for (int i = 0; i < 9; i++) r[i] = tab[(10 + i) * N + n];
+    constexpr double INV_C = 1.0 / C_SI;
+#pragma unroll
+    for (int sc = 0; sc < 3; sc++)
+        g.kP[sc] = (s.k[0] * P[3 * sc] + s.k[1] * P[3 * sc + 1] + s.k[2] * P[3 * sc + 2]) * INV_C;
+    g.kr12 = s.k[0] * r[0] + s.k[1] * r[1] + s.k[2] * r[2];
+    g.kr13 = s.k[0] * r[3] + s.k[1] * r[4] + s.k[2] * r[5];
+    g.kr23 = s.k[0] * r[6] + s.k[1] * r[7] + s.k[2] * r[8];
+    g.ur12 = s.u[0] * r[0] + s.u[1] * r[1] + s.u[2] * r[2];
+    g.vr12 = s.v[0] * r[0] + s.v[1] * r[1] + s.v[2] * r[2];
+    g.ur13 = s.u[0] * r[3] + s.u[1] * r[4] + s.u[2] * r[5];
+    g.vr13 = s.v[0] * r[3] + s.v[1] * r[4] + s.v[2] * r[5];
+    g.ur23 = s.u[0] * r[6] + s.u[1] * r[7] + s.u[2] * r[8];
+    g.vr23 = s.v[0] * r[6] + s.v[1] * r[7] + s.v[2] * r[8];
+    return g;
+}
+
+// One transfer-function link: 0.25 sin(a)/a e^{-i(a + Phi_s)} A_arm
+// (gbgpu.py:508-525), with e^{-i Phi_s} = (cphi, -sphi) precomputed.
+__device__ __forceinline__ double2 link_f64(double fonfs, double kr, double cphi, double sphi, double2 Aarm) {
+    const double a = 0.5 * fonfs * (1.0 + kr);
+    double sa, ca;
+    sincos(a, &sa, &ca);
+    const double sinc = (a != 0.0) ? 0.25 * sa / a : 0.25;
+    // e^{-ia} e^{-iPhi} = (ca - i sa)(cphi - i sphi)
+    const double er = ca * cphi - sa * sphi;
+    const double ei = -(sa * cphi + ca * sphi);
+    const double2 e = make_double2(sinc * er, sinc * ei);
+    return cmul(e, Aarm);
+}
+
+// Time-domain X, Y, Z slow part at sample n, already multiplied by
+// fctr2 * amp * 0.5 T / N (gbgpu.py:359-375, :298; amp / amp cancels).
+// v3[sc] (third-body v_LOS at xi_sc) and term3[sc] (its phase integral) are
+// used only when THIRD.
+template <bool THIRD>
+__device__ __forceinline__ void slow_xyz_f64(const Src &s, const SampleGeom &g, double fstar_inv, double scale,
+                                             const double *v3, const double *term3, double2 &X, double2 &Y,
+                                             double2 &Z) {
+    double fonfs[3], cphi[3], sphi[3];
+#pragma unroll
+    for (int sc = 0; sc < 3; sc++) {
+        const double xi = g.t - g.kP[sc];
+        double fi = s.f0 + s.fdot * xi + 0.5 * s.fddot * xi * xi;
+        if (THIRD) fi *= 1.0 + v3[sc] / C_SI;
+        fonfs[sc] = fi * fstar_inv;
+        double argS = s.phi0 + s.omq * g.t + PI * s.fdot * (xi * xi) + (1.0 / 3.0) * PI * s.fddot * (xi * xi * xi);
+        if (THIRD) argS += term3[sc];
+        const double Phi = s.om * g.kP[sc] - argS;
+        sincos(Phi, &sphi[sc], &cphi[sc]);
+    }
+    // arm antenna factors A_ij = (r e+ r) DP + (r ex r) DC  (gbgpu.py:467-473)
+    // with r e+ r = (v.r)^2 - (u.r)^2 and r ex r = 2 (u.r)(v.r); r31 = -r13.
+    const double ep12 = g.vr12 * g.vr12 - g.ur12 * g.ur12, ex12 = 2.0 * g.ur12 * g.vr12;
+    const double ep23 = g.vr23 * g.vr23 - g.ur23 * g.ur23, ex23 = 2.0 * g.ur23 * g.vr23;
+    const double ep31 = g.vr13 * g.vr13 - g.ur13 * g.ur13, ex31 = 2.0 * g.ur13 * g.vr13;
+    const double2 A12 = make_double2(ep12 * s.DPr + ex12 * s.DCr, ep12 * s.DPi + ex12 * s.DCi);
+    const double2 A23 = make_double2(ep23 * s.DPr + ex23 * s.DCr, ep23 * s.DPi + ex23 * s.DCi);
+    const double2 A31 = make_double2(ep31 * s.DPr + ex31 * s.DCr, ep31 * s.DPi + ex31 * s.DCi);
+    // links (ij, arm, spacecraft of the delay, k.r_ij)  (gbgpu.py:509-516)
+    const double2 G12 = link_f64(fonfs[0], g.kr12, cphi[0], sphi[0], A12);
+    const double2 G23 = link_f64(fonfs[1], g.kr23, cphi[1], sphi[1], A23);
+    const double2 G31 = link_f64(fonfs[2], -g.kr13, cphi[2], sphi[2], A31);
+    const double2 G21 = link_f64(fonfs[1], -g.kr12, cphi[1], sphi[1], A12);
+    const double2 G32 = link_f64(fonfs[2], -g.kr23, cphi[2], sphi[2], A23);
+    const double2 G13 = link_f64(fonfs[0], g.kr13, cphi[0], sphi[0], A31);
+    // TDI transfer (gbgpu.py:352-375)
+    const double f = s.f0 + s.fdot * g.t + 0.5 * s.fddot * g.t * g.t;
+    const double omL = f * fstar_inv;
+    double sL, cL;
+    sincos(omL, &sL, &cL);
+    const double2 fctr = make_double2(cL, -sL);
+    const double m = 4.0 * omL * sL * scale;
+    const double2 fctr2 = make_double2(m * cL, -m * sL);
+    X = cmul(fctr2, cadd(csub(G21, G31), cmul(csub(G12, G13), fctr)));
+    Y = cmul(fctr2, cadd(csub(G32, G12), cmul(csub(G23, G21), fctr)));
+    Z = cmul(fctr2, cadd(csub(G13, G23), cmul(csub(G31, G32), fctr)));
+}
+
+}  // namespace gbgpu

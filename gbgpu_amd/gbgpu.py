@@ -1,0 +1,568 @@
+"""GBGPU user API on the gfx950 backend.
+
+Mirrors the reference class (src/gbgpu/gbgpu.py:29-1327): same class, method
+and attribute names, argument order, defaults, layouts and error behaviour.
+The waveform array passes that the reference runs as numpy/cupy code
+(src/gbgpu/gbgpu.py:108-540) are hand-written HIP kernels here, reached
+through the C ABI (include/gbgpu_hip.h); torch supplies device memory and the
+stream.  Arrays returned on the device are torch tensors (the reference
+returns cupy arrays on its GPU backend).
+"""
+
+from __future__ import annotations
+
+import abc
+import warnings
+
+import numpy as np
+
+from . import _capi
+from .cutils import get_backend
+from .orbits import EqualArmlengthOrbits, Orbits, orbit_table
+from .utils.constants import C_SI, YRSID_SI
+from .utils.utility import get_N
+
+_PRECISIONS = {"fp64": _capi.GBGPU_FP64, "mixed": _capi.GBGPU_MIXED}
+
+
+def _torch():
+    import torch
+
+    return torch
+
+
+class GBGPUBase(abc.ABC):
+    """Generate Galactic-binary waveforms and likelihoods in batches.
+
+    Args:
+        orbits: object with ``armlength``, ``t_base`` and ``get_pos(t, sc)``
+            (default: analytic ``EqualArmlengthOrbits``).
+        force_backend: ``None``, ``"hip"`` or ``"gpu"`` (all select the
+            gfx950 backend).
+        precision: ``"fp64"`` (plain) or ``"mixed"`` (FP64 phases, start
+            bins and likelihood accumulation; FP32 slow part and transform).
+    """
+
+    # kernels implement only these hook sets; subclasses with other hooks
+    # cannot run on the GPU path
+    _kernel_variant = "circular"
+
+    def __init__(self, orbits: Orbits = None, force_backend=None, precision: str = "fp64"):
+        self.backend = get_backend(force_backend)
+        self.d_d = None
+        self.orbits = orbits
+        self.precision = precision
+        self._tables = {}
+        self._wave_state = None  # (params tensor, N, T, tdi2, precision) for lazy templates
+        self._XYZf = None
+        self._AETf = None
+
+    # ----------------------------------------------------------- backend ---
+    @property
+    def xp(self):
+        return self.backend.xp
+
+    @property
+    def get_ll_func(self):
+        """get_ll native func (src/gbgpu/gbgpu.py:76-79)."""
+        return self.backend.get_ll
+
+    @property
+    def fill_global_func(self):
+        """fill_global native func (src/gbgpu/gbgpu.py:81-84)."""
+        return self.backend.fill_global
+
+    @property
+    def orbits(self):
+        return self._orbits
+
+    @orbits.setter
+    def orbits(self, orbits):
+        if orbits is None:
+            self._orbits = EqualArmlengthOrbits()
+        elif not (hasattr(orbits, "get_pos") and hasattr(orbits, "armlength")):
+            raise ValueError("Input orbits must provide armlength, t_base and get_pos(t, sc)")
+        else:
+            self._orbits = orbits
+        if hasattr(self._orbits, "configure"):
+            self._orbits.configure(linear_interp_setup=True)
+        self._tables = {}
+
+    @property
+    def precision(self):
+        return self._precision
+
+    @precision.setter
+    def precision(self, p):
+        if p not in _PRECISIONS:
+            raise ValueError(f"precision must be one of {list(_PRECISIONS)}")
+        self._precision = p
+
+    @property
+    def citation(self):
+        return "Cornish & Littenberg 2007 (arXiv:0704.1808); Katz et al. 2022 (arXiv:2205.03461)"
+
+    def _device(self):
+        return self.backend.device()
+
+    def _table(self, T, N):
+        key = (float(T), int(N))
+        t = self._tables.get(key)
+        if t is None:
+            torch = _torch()
+            t = torch.from_numpy(orbit_table(self.orbits, T, N)).to(self._device())
+            self._tables[key] = t
+        return t
+
+    def _cfg(self, N, T, tdi2, precision):
+        if N & (N - 1) or not (32 <= N <= 2048):
+            raise ValueError(f"N={N}: the HIP kernels support powers of two in [32, 2048]")
+        tab = self._table(T, N)
+        cfg = _capi.WaveCfg(
+            N=int(N),
+            T=float(T),
+            armlength=float(self.orbits.armlength),
+            orbit_table=tab.data_ptr(),
+            tdi2=int(bool(tdi2)),
+            precision=_PRECISIONS[precision or self.precision],
+        )
+        return cfg, tab
+
+    # ---------------------------------------------------------- waveform ---
+    def _prepare(self, amp, f0, fdot, fddot, phi0, iota, psi, lam, beta, args, N, T, dt, oversample):
+        """Shared front half of run_wave (src/gbgpu/gbgpu.py:177-229)."""
+        N_obs = int(T / dt)
+        T = N_obs * dt
+        if T > np.max(self.orbits.t_base):
+            raise ValueError(
+                f"Observation time ({T}) is larger than length of time in orbital information "
+                f"({np.max(self.orbits.t_base)})"
+            )
+        base = [np.atleast_1d(np.asarray(x, dtype=np.float64)) for x in (amp, f0, fdot, fddot, phi0, iota, psi, lam, beta)]
+        B = len(base[0])
+        if any(len(x) != B for x in base):
+            raise ValueError("all parameter arrays must have the same length")
+        if len(args) == 0:
+            add_args = ()
+        else:
+            add_args = self.prepare_additional_args(*args)
+        if N is None:
+            N_temp = self.special_get_N(base[0], base[1], T, *args, oversample=oversample)
+            N = int(np.max(N_temp))
+        N = int(N)
+        rows = base + [np.broadcast_to(np.asarray(a, dtype=np.float64), (B,)) for a in add_args]
+        torch = _torch()
+        params = torch.from_numpy(np.ascontiguousarray(np.stack(rows, axis=0))).to(self._device())
+        return params, N, T
+
+    def run_wave(self, amp, f0, fdot, fddot, phi0, iota, psi, lam, beta, *args, N=None, T=4 * YRSID_SI, dt=10.0,
+                 oversample=1, tdi2=False, precision=None):
+        """Create A/E/T and X/Y/Z templates for a batch (src/gbgpu/gbgpu.py:108-316).
+
+        Outputs (torch, on the GPU): ``XYZf``/``AETf`` (num_bin, 3, N)
+        complex128, ``start_inds`` (num_bin,) int64, plus ``df``, ``N``,
+        ``num_bin`` -- as the reference sets them.
+        """
+        self._check_hooks()
+        params, N, T = self._prepare(amp, f0, fdot, fddot, phi0, iota, psi, lam, beta, args, N, T, dt, oversample)
+        self._run_wave_params(params, N, T, tdi2, precision)
+
+    def _run_wave_params(self, params, N, T, tdi2, precision):
+        torch = _torch()
+        B = params.shape[1]
+        self.num_bin = B
+        self.N = N
+        self.df = 1 / T
+        cfg, _tab = self._cfg(N, T, tdi2, precision)
+        dev = params.device
+        XYZf = torch.empty((B, 3, N), dtype=torch.complex128, device=dev)
+        AETf = torch.empty((B, 3, N), dtype=torch.complex128, device=dev)
+        start = torch.empty((B,), dtype=torch.int64, device=dev)
+        lib = _capi.load()
+        rc = lib.gbgpu_run_wave(XYZf.data_ptr(), AETf.data_ptr(), start.data_ptr(), params.data_ptr(),
+                                params.shape[0], B, cfg, _capi.stream_handle())
+        _capi.check(rc, "gbgpu_run_wave")
+        self._XYZf, self._AETf = XYZf, AETf
+        self.start_inds = self.kmin = start
+        self._wave_state = None
+
+    def _materialize(self):
+        if self._wave_state is not None:
+            params, N, T, tdi2, precision = self._wave_state
+            self._run_wave_params(params, N, T, tdi2, precision)
+
+    @property
+    def XYZf(self):
+        self._materialize()
+        return self._XYZf
+
+    @property
+    def AETf(self):
+        self._materialize()
+        return self._AETf
+
+    # output views (src/gbgpu/gbgpu.py:318-346, :542-578)
+    @property
+    def X_out(self):
+        return self.XYZf[:, 0].T.flatten()
+
+    @property
+    def Y_out(self):
+        return self.XYZf[:, 1].T.flatten()
+
+    @property
+    def Z_out(self):
+        return self.XYZf[:, 2].T.flatten()
+
+    @property
+    def A_out(self):
+        return self.AETf[:, 0].T.flatten()
+
+    @property
+    def E_out(self):
+        return self.AETf[:, 1].T.flatten()
+
+    @property
+    def T_out(self):
+        return self.AETf[:, 2].T.flatten()
+
+    @property
+    def A(self):
+        return self.AETf[:, 0]
+
+    @property
+    def E(self):
+        return self.AETf[:, 1]
+
+    @property
+    def T(self):
+        return self.AETf[:, 2]
+
+    @property
+    def X(self):
+        return self.XYZf[:, 0]
+
+    @property
+    def Y(self):
+        return self.XYZf[:, 1]
+
+    @property
+    def Z(self):
+        return self.XYZf[:, 2]
+
+    @property
+    def freqs(self):
+        torch = _torch()
+        return (torch.arange(self.N, device=self.start_inds.device)[None, :] + self.start_inds[:, None]) * self.df
+
+    # -------------------------------------------------------- likelihood ---
+    def _to_dev(self, x, dtype):
+        torch = _torch()
+        if isinstance(x, np.ndarray):
+            return torch.from_numpy(np.ascontiguousarray(x.astype(dtype, copy=False))).to(self._device())
+        if torch.is_tensor(x):
+            tdt = {np.complex128: torch.complex128, np.float64: torch.float64, np.int32: torch.int32}[dtype]
+            if x.dtype != tdt:
+                raise TypeError(f"expected {tdt}, got {x.dtype}")
+            return x.to(self._device()).contiguous()
+        raise TypeError("Make sure the data arrays are numpy arrays or torch tensors.")
+
+    @staticmethod
+    def _check_data(data, data_index, noise_index):
+        """Host-only checks of get_ll's data (src/gbgpu/gbgpu.py:650-665)."""
+        torch = _torch()
+        if not isinstance(data[0], np.ndarray) and not torch.is_tensor(data[0]):
+            raise TypeError("Make sure the data arrays are the same type as template arrays (cupy vs numpy).")
+        if data[0].ndim == 1:
+            if data_index is not None:
+                raise ValueError("If inputing 1D data, cannot use data_index kwarg.")
+            if noise_index is not None:
+                raise ValueError("If inputing 1D data, cannot use noise_index kwarg.")
+            return data[0].shape[0]
+        elif data[0].ndim == 2:
+            return data[0].shape[1]
+        raise ValueError("data must be 1D or 2D")
+
+    def _prep_data(self, data, psd, data_index, noise_index, num_bin):
+        """Validation and flattening of get_ll's data/psd (src/gbgpu/gbgpu.py:650-686)."""
+        torch = _torch()
+        data_length = self._check_data(data, data_index, noise_index)
+        dA = self._to_dev(data[0], np.complex128).reshape(-1)
+        dE = self._to_dev(data[1], np.complex128).reshape(-1)
+        SA = self._to_dev(psd[0], np.float64).reshape(-1)
+        SE = self._to_dev(psd[1], np.float64).reshape(-1)
+        if data_index is None:
+            data_index = torch.zeros(num_bin, dtype=torch.int32, device=dA.device)
+        if noise_index is None:
+            noise_index = torch.zeros(num_bin, dtype=torch.int32, device=dA.device)
+        if isinstance(data_index, np.ndarray) and data_index.dtype != np.int32:
+            raise AssertionError("data_index must be int32")
+        if isinstance(noise_index, np.ndarray) and noise_index.dtype != np.int32:
+            raise AssertionError("noise_index must be int32")
+        data_index = self._to_dev(data_index, np.int32)
+        noise_index = self._to_dev(noise_index, np.int32)
+        assert len(data_index) == num_bin
+        assert len(data_index) == len(noise_index)
+        assert int(data_index.max()) * data_length <= len(dA)
+        assert int(noise_index.max()) * data_length <= len(dA)
+        if int(data_index.max()) * data_length >= len(dA) or int(noise_index.max()) * data_length >= len(SA):
+            raise ValueError("data_index/noise_index point past the provided streams")
+        return dA, dE, SA, SE, data_index, noise_index, data_length
+
+    def get_ll(self, params, data, psd, phase_marginalize=False, start_freq_ind=0, data_index=None,
+               noise_index=None, **kwargs):
+        """Batched log-likelihood (src/gbgpu/gbgpu.py:580-735).
+
+        Runs the fused waveform + likelihood kernel: templates stay in LDS.
+        ``gb.A``/``gb.E``/... remain available afterwards (regenerated on
+        first access).  Returns a numpy (num_bin,) float64 array.
+        """
+        if self.d_d is None:
+            raise ValueError(
+                "self.d_d attribute must be set before computing log-Likelihood. This attribute is the data "
+                "with data inner product (<d|d>)."
+            )
+        self._check_hooks()
+        torch = _torch()
+        tdi2 = kwargs.pop("tdi2", False)
+        precision = kwargs.pop("precision", None)
+        N = kwargs.pop("N", None)
+        T = kwargs.pop("T", 4 * YRSID_SI)
+        dt = kwargs.pop("dt", 10.0)
+        oversample = kwargs.pop("oversample", 1)
+        if kwargs:
+            raise TypeError(f"unexpected keyword arguments {list(kwargs)}")
+        params = np.asarray(params) if not torch.is_tensor(params) else params.cpu().numpy()
+        self._check_data(data, data_index, noise_index)
+        p, N, T = self._prepare(*params[:9], params[9:], N, T, dt, oversample)
+        B = p.shape[1]
+        self.num_bin, self.N, self.df = B, N, 1 / T
+        dA, dE, SA, SE, di, ni, L = self._prep_data(data, psd, data_index, noise_index, B)
+        cfg, _tab = self._cfg(N, T, tdi2, precision)
+        dev = p.device
+        ll = torch.empty(B, dtype=torch.float64, device=dev)
+        d_h = torch.empty(B, dtype=torch.complex128, device=dev)
+        h_h = torch.empty(B, dtype=torch.complex128, device=dev)
+        dat = _capi.Data(A=dA.data_ptr(), E=dE.data_ptr(), psdA=SA.data_ptr(), psdE=SE.data_ptr(),
+                         data_index=di.data_ptr(), noise_index=ni.data_ptr(), data_length=int(L),
+                         start_freq_ind=int(start_freq_ind), df=float(1 / T))
+        lib = _capi.load()
+        rc = lib.gbgpu_wave_ll(ll.data_ptr(), d_h.data_ptr(), h_h.data_ptr(), p.data_ptr(), p.shape[0], B, cfg,
+                               dat, float(np.real(self.d_d)), int(bool(phase_marginalize)), _capi.stream_handle())
+        _capi.check(rc, "gbgpu_wave_ll")
+        # start bins without materialising templates: rint(f0 T) - N/2
+        f0 = p[1]
+        self.start_inds = self.kmin = (torch.round(f0 * T).to(torch.int64) - N // 2)
+        self._wave_state = (p, N, T, tdi2, precision)
+        if phase_marginalize:
+            self.non_marg_d_h = d_h.cpu().numpy()
+            d_h = torch.abs(d_h).to(torch.complex128)
+        self.h_h = h_h
+        self.d_h = d_h
+        return ll.cpu().numpy()
+
+    # -------------------------------------------------- global templates ---
+    def fill_global_template(self, group_index, templates, A, E, start_inds, N=None, start_freq_ind=0):
+        """Sum binaries into per-group A/E arrays (src/gbgpu/gbgpu.py:737-837).
+
+        Deterministic (bit-reproducible) summation.  As in the reference,
+        ``templates`` is overwritten with the sum of the given binaries.
+        """
+        torch = _torch()
+        total_groups, nchannels, data_length = templates.shape
+        if nchannels < 2:
+            raise ValueError("Calculates for A and E channels.")
+        elif nchannels > 2:
+            warnings.warn("Only calculating A and E channels here currently.")
+        if not torch.is_tensor(templates) or templates.device.type != "cuda":
+            raise TypeError("Make sure the data arrays are the same type as template arrays (cupy vs numpy).")
+        dev = templates.device
+        group_index = torch.as_tensor(np.asarray(group_index) if not torch.is_tensor(group_index) else group_index,
+                                      dtype=torch.int32).to(dev).contiguous()
+        num_bin = len(group_index)
+        tA = torch.zeros((total_groups, data_length), dtype=torch.complex128, device=dev)
+        tE = torch.zeros((total_groups, data_length), dtype=torch.complex128, device=dev)
+        A = A if torch.is_tensor(A) else torch.from_numpy(np.asarray(A)).to(dev)
+        E = E if torch.is_tensor(E) else torch.from_numpy(np.asarray(E)).to(dev)
+        if A.ndim > 2 or E.ndim > 2:
+            raise ValueError("A_in, E_in have maximum allowable dimension of 2.")
+        lib = _capi.load()
+        if A.ndim == 2:
+            assert E.ndim == 2
+            N = A.shape[1]
+            if A.stride(1) != 1:
+                A = A.contiguous()
+            if E.stride(1) != 1 or E.stride(0) != A.stride(0):
+                E = E.contiguous()
+                A = A.contiguous()
+            st = torch.as_tensor(start_inds if torch.is_tensor(start_inds) else np.asarray(start_inds),
+                                 dtype=torch.int64).to(dev).contiguous()
+            rc = lib.gbgpu_fill_global_rows(tA.data_ptr(), tE.data_ptr(), A.data_ptr(), E.data_ptr(), A.stride(0),
+                                            st.data_ptr(), int(start_freq_ind), int(N), num_bin,
+                                            group_index.data_ptr(), int(data_length), int(total_groups),
+                                            _capi.stream_handle())
+            _capi.check(rc, "gbgpu_fill_global_rows")
+        else:
+            if N is None:
+                raise ValueError(
+                    "If providing a 1D flattened array for A and E, the N kwarg also needs to be provided."
+                )
+            st = (torch.as_tensor(start_inds if torch.is_tensor(start_inds) else np.asarray(start_inds)).to(dev)
+                  - start_freq_ind).to(torch.int32).contiguous()
+            self.fill_global_func(tA.reshape(-1), tE.reshape(-1), A.contiguous(), E.contiguous(), st, N, num_bin,
+                                  group_index, data_length, total_groups)
+        templates[:, 0] = tA
+        templates[:, 1] = tE
+
+    def generate_global_template(self, params, group_index, templates, start_freq_ind=0, **kwargs):
+        """run_wave then fill_global_template (src/gbgpu/gbgpu.py:839-878)."""
+        params = np.asarray(params)
+        self.run_wave(*params.T, **kwargs)
+        self.fill_global_template(group_index, templates, self.AETf[:, 0], self.AETf[:, 1], self.start_inds,
+                                  self.N, start_freq_ind=start_freq_ind)
+
+    def inject_signal(self, *args, fmax=None, T=4.0 * YRSID_SI, dt=10.0, **kwargs):
+        """Place binary 0 into zero A/E arrays on [0, fmax] (src/gbgpu/gbgpu.py:880-946)."""
+        if fmax is None:
+            fmax = 1 / (2 * dt)
+        if T > np.max(self.orbits.t_base):
+            raise ValueError(
+                f"Observation time ({T}) is larger than length of time in orbital information "
+                f"({np.max(self.orbits.t_base)})"
+            )
+        N_obs = int(T / dt)
+        T = N_obs * dt
+        kwargs["T"] = T
+        kwargs["dt"] = dt
+        self.df = df = 1 / T
+        f = np.arange(0.0, fmax + df, df)
+        num = len(f)
+        A_out = np.zeros(num, dtype=np.complex128)
+        E_out = np.zeros(num, dtype=np.complex128)
+        self.run_wave(*args, **kwargs)
+        start = int(self.start_inds[0].item())
+        A_temp = self.A[0].cpu().numpy()
+        E_temp = self.E[0].cpu().numpy()
+        A_out[start : start + self.N] = A_temp
+        E_out[start : start + self.N] = E_temp
+        return A_out, E_out
+
+    # ---------------------------------------------------- Fisher (FD) ---
+    def _apply_parameter_transforms(self, params, parameter_transforms):
+        """src/gbgpu/gbgpu.py:948-955."""
+        for ind_trans, trans in parameter_transforms.items():
+            if isinstance(ind_trans, int):
+                params[ind_trans] = trans(params[ind_trans])
+            else:
+                params[np.asarray(ind_trans)] = trans(*params[np.asarray(ind_trans)])
+        return params
+
+    def information_matrix(self, params, eps=1e-9, parameter_transforms={}, inds=None, N=1024, psd_func=None,
+                           psd_kwargs={}, easy_central_difference=False, return_gpu=False, **kwargs):
+        """Finite-difference information matrix (src/gbgpu/gbgpu.py:957-1138).
+
+        Waveforms come from the HIP run_wave; the 5-point (or central)
+        multiplicative-step stencil and the inner products follow the
+        reference.  ``psd_func`` defaults to the analytic A-channel PSD.
+        """
+        torch = _torch()
+        if psd_func is None:
+            from .noise import A1TDISens
+
+            psd_func = A1TDISens.get_Sn
+        if N is None:
+            raise ValueError("N must be provided up front for Infromation Matrix.")
+        kwargs["N"] = N
+        params = np.atleast_2d(np.asarray(params, dtype=np.float64))
+        num_params, num_bins = params.shape
+        if inds is None:
+            inds = np.arange(num_params)
+        nd = len(inds)
+        dev = self._device()
+        dh = torch.zeros((num_bins, nd, 2, N), dtype=torch.complex128, device=dev)
+
+        def wave(p):
+            self.run_wave(*p, **kwargs)
+            return torch.stack([self.A, self.E], dim=1).clone()
+
+        for i, ind in enumerate(inds):
+            def step(m):
+                p = np.ones_like(params)
+                p[ind] += m * eps
+                p *= params
+                return self._apply_parameter_transforms(p, parameter_transforms)
+
+            up1 = wave(step(1))
+            dn1 = wave(step(-1))
+            if easy_central_difference:
+                dh[:, i] = (up1 - dn1) / (2 * eps)
+            else:
+                up2 = wave(step(2))
+                if not bool(torch.all(self.start_inds == self.start_inds[0])):
+                    raise ValueError(
+                        "The user should decrease steps size (eps) because the frequency bins are changing during "
+                        "derivative calculation, which is not allowed."
+                    )
+                dn2 = wave(step(-2))
+                dh[:, i] = (-up2 + dn2 + 8 * (up1 - dn1)) / (12 * eps)
+        fr = self.freqs.cpu().numpy()
+        psd = torch.from_numpy(np.asarray(psd_func(fr, **psd_kwargs), dtype=np.float64)).to(dev)
+        info = torch.zeros((num_bins, nd, nd), dtype=torch.float64, device=dev)
+        for i in range(nd):
+            for j in range(i, nd):
+                ip = 4 * self.df * torch.sum((dh[:, i].conj() * dh[:, j]).real / psd[:, None, :], dim=(1, 2))
+                info[:, i, j] = ip
+                info[:, j, i] = ip
+        if return_gpu:
+            return info
+        return info.cpu().numpy()
+
+    # ------------------------------------------------------------- hooks ---
+    def _check_hooks(self):
+        variant = type(self)._kernel_variant
+        base_hooks = {"add_to_argS", "prepare_additional_args", "special_get_N", "shift_frequency"}
+        for cls in type(self).__mro__:
+            if cls in (GBGPU, GBGPUBase, object) or getattr(cls, "_kernel_variant", None) != variant:
+                continue
+            overridden = base_hooks & set(vars(cls))
+            if overridden and cls.__module__.split(".")[0] != __name__.split(".")[0]:
+                raise NotImplementedError(
+                    f"{cls.__name__} overrides {sorted(overridden)}: the GPU kernels implement only the circular "
+                    "and third-body hook sets"
+                )
+
+    @abc.abstractmethod
+    def add_to_argS(self, argS, f0, fdot, fddot, xi, *args):
+        """Phase hook (src/gbgpu/gbgpu.py:1140-1167); evaluated inside the kernel."""
+
+    @abc.abstractmethod
+    def prepare_additional_args(self, *args):
+        """src/gbgpu/gbgpu.py:1169-1184."""
+
+    @abc.abstractmethod
+    def special_get_N(self, amp, f0, T, *args, oversample=1):
+        """src/gbgpu/gbgpu.py:1186-1210."""
+
+    @abc.abstractmethod
+    def shift_frequency(self, fi, xi, *args):
+        """Frequency hook (src/gbgpu/gbgpu.py:1212-1229); evaluated inside the kernel."""
+
+
+class GBGPU(GBGPUBase):
+    """Circular Galactic binaries (src/gbgpu/gbgpu.py:1233-1327)."""
+
+    def prepare_additional_args(self, *args):
+        # the reference returns (does not raise) this ValueError (gbgpu.py:1255-1257)
+        raise ValueError(
+            "If providing more args than the base args, must be a class derived from GBGPUBase with an adjusted "
+            "'prepare_additional_args' method."
+        )
+
+    def special_get_N(self, amp, f0, T, *args, oversample=1):
+        return get_N(amp, f0, T, oversample=oversample)
+
+    def shift_frequency(self, fi, xi, *args):
+        return fi
+
+    def add_to_argS(self, argS, f0, fdot, fddot, xi, *args):
+        return argS

@@ -1,0 +1,118 @@
+"""Spacecraft orbits for the FastGB slow part.
+
+The reference obtains positions from ``lisatools.detector.EqualArmlengthOrbits``
+(src/gbgpu/gbgpu.py:86-101, :390-397), which interpolates an orbit file that
+is not vendored.  This module supplies the analytic rigid-rotating
+constellation of Cornish & Rubbo (Phys. Rev. D 67, 022001, 2003, eq. 56),
+parametrised by the constants in src/gbgpu/cutils/LISA.h:12-27
+(eccentricity ``ec``, ``kappa``, ``lambda``, modulation frequency ``fm``).
+
+Any object with the same small interface (``armlength``, ``t_base``,
+``get_pos(t, sc)``, ``configure(...)``) can be passed to ``GBGPU(orbits=...)``:
+the GPU kernels only read a per-(T, N) table of positions built on the host
+from ``get_pos``, so the device code does not depend on the orbit model.
+"""
+
+from __future__ import annotations
+
+import numpy as np
+
+from .utils.constants import (
+    AU_SI,
+    LARM_SI,
+    LISA_EC,
+    LISA_FM,
+    LISA_KAPPA,
+    LISA_LAMBDA,
+    YRSID_SI,
+)
+
+
+class Orbits:
+    """Minimal orbit interface used by the FastGB path."""
+
+    armlength: float
+    t_base: np.ndarray
+
+    def configure(self, linear_interp_setup: bool = False, **kwargs) -> None:
+        """Kept for interface parity with lisatools' ``Orbits.configure``."""
+        self.linear_interp_setup = linear_interp_setup
+
+    def get_pos(self, t, sc):  # pragma: no cover - interface
+        raise NotImplementedError
+
+
+class EqualArmlengthOrbits(Orbits):
+    """Analytic equal-arm (to first order in ``ec``) rigid-rotating orbits.
+
+    Args:
+        armlength: nominal arm length in metres (LISA.h ``Larm``).
+        ec: orbital eccentricity; ``None`` keeps ``armlength / (2*sqrt(3)*AU)``
+            which equals LISA.h's 0.0048241852 for 2.5e9 m.
+        kappa, lam: initial guiding-centre phase and constellation orientation.
+        fm: orbital modulation frequency in Hz.
+        t_max: last time (s) for which the orbits are declared valid; ``run_wave``
+            refuses observation times beyond it, as the reference does against
+            its orbit file (src/gbgpu/gbgpu.py:181-184).
+    """
+
+    def __init__(
+        self,
+        armlength: float = LARM_SI,
+        ec: float | None = LISA_EC,
+        kappa: float = LISA_KAPPA,
+        lam: float = LISA_LAMBDA,
+        fm: float = LISA_FM,
+        t_max: float = 10.0 * YRSID_SI,
+    ):
+        self.armlength = float(armlength)
+        self.ec = float(ec) if ec is not None else armlength / (2.0 * np.sqrt(3.0) * AU_SI)
+        self.kappa = float(kappa)
+        self.lam = float(lam)
+        self.fm = float(fm)
+        self.t_base = np.array([0.0, float(t_max)])
+        self.linear_interp_setup = False
+
+    def get_pos(self, t, sc):
+        """Position (m) of spacecraft ``sc`` (1, 2 or 3) at times ``t``.
+
+        ``sc`` may be a scalar or an array broadcastable against ``t``; the
+        result has shape ``t.shape + (3,)`` like lisatools' ``get_pos``.
+        """
+        t = np.asarray(t, dtype=np.float64)
+        sc = np.asarray(sc)
+        alpha = 2.0 * np.pi * self.fm * t + self.kappa
+        beta = 2.0 * np.pi * (sc.astype(np.float64) - 1.0) / 3.0 + self.lam
+        sa, ca = np.sin(alpha), np.cos(alpha)
+        sb, cb = np.sin(beta), np.cos(beta)
+        a = AU_SI
+        e = self.ec
+        x = a * ca + a * e * (sa * ca * sb - (1.0 + sa * sa) * cb)
+        y = a * sa + a * e * (sa * ca * cb - (1.0 + ca * ca) * sb)
+        z = -np.sqrt(3.0) * a * e * (ca * cb + sa * sb)
+        return np.stack(np.broadcast_arrays(x, y, z), axis=-1)
+
+
+def orbit_table(orbits: Orbits, T: float, N: int) -> np.ndarray:
+    """Host-side per-(T, N) table read by the GPU kernels.
+
+    Rows (structure of arrays, each of length N, float64):
+      0      t_n = n*T/N                         (src/gbgpu/gbgpu.py:269)
+      1..9   P1, P2, P3 (x, y, z each)           (src/gbgpu/gbgpu.py:390-397)
+      10..18 r12, r13, r23 = (Pj - Pi)/armlength (src/gbgpu/gbgpu.py:424-427)
+    """
+    tm = np.linspace(0.0, T, num=N, endpoint=False)
+    P = [np.asarray(orbits.get_pos(tm, np.full(N, s, dtype=np.int32)), dtype=np.float64) for s in (1, 2, 3)]
+    L = orbits.armlength
+    r12 = (P[1] - P[0]) / L
+    r13 = (P[2] - P[0]) / L
+    r23 = (P[2] - P[1]) / L
+    rows = [tm]
+    for p in P:
+        rows += [p[:, 0], p[:, 1], p[:, 2]]
+    for r in (r12, r13, r23):
+        rows += [r[:, 0], r[:, 1], r[:, 2]]
+    return np.ascontiguousarray(np.stack(rows, axis=0))
+
+
+ORBIT_TABLE_ROWS = 19

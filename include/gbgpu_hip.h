@@ -1,0 +1,129 @@
+/* gbgpu_hip.h -- C ABI of the MI355X (gfx950) FastGB waveform + likelihood library.
+ *
+ * All pointers are DEVICE pointers unless stated otherwise; complex arrays are
+ * interleaved {re, im} float64 pairs (numpy complex128 layout, the same as the
+ * reference's gcmplx::complex<double>, src/gbgpu/cutils/global.h:66).  Every
+ * entry point is asynchronous on the caller's HIP stream (``stream`` is a
+ * hipStream_t passed as void*, NULL = default stream), never synchronises the
+ * device, allocates nothing on the hot path beyond its first-use caches, and
+ * returns 0 on success or a negative GBGPU_E* code; gbgpu_last_error() then
+ * holds a message for the calling thread.
+ */
+#ifndef GBGPU_HIP_H
+#define GBGPU_HIP_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct gbgpu_cmplx {
+    double re, im;
+} gbgpu_cmplx;
+
+enum {
+    GBGPU_OK = 0,
+    GBGPU_E_ARG = -1,     /* invalid argument (size, pointer, unsupported N) */
+    GBGPU_E_HIP = -2,     /* HIP runtime error */
+    GBGPU_E_ALLOC = -3,   /* cache allocation failed */
+};
+
+enum { GBGPU_FP64 = 0, GBGPU_MIXED = 1 };
+
+/* Parameter rows of the (num_params, num_bin) float64 parameter array:
+ * amp, f0, fdot, fddot, phi0, iota, psi, lam, beta   (src/gbgpu/gbgpu.py:108-120)
+ * and, when num_params == 14, the third-body rows A2, varpi, e2, n2, T2 with
+ * n2 = 2*pi/(P2*YRSID_SI) and T2 in seconds, i.e. already passed through
+ * prepare_additional_args (src/gbgpu/thirdbody.py:39-80). */
+#define GBGPU_NPARAMS 9
+#define GBGPU_NPARAMS_THIRD 14
+#define GBGPU_ORBIT_ROWS 19
+
+/* Waveform configuration shared by every waveform entry point. */
+typedef struct gbgpu_wave_cfg {
+    int N;                     /* samples per binary: power of two, 32..2048 */
+    double T;                  /* observation time, already int(T/dt)*dt (gbgpu.py:177-179) */
+    double armlength;          /* metres; f* = c / (2 pi armlength) (gbgpu.py:243) */
+    const double *orbit_table; /* (19, N) float64: t_n, P1, P2, P3 (xyz), r12, r13, r23 (xyz) */
+    int tdi2;                  /* 1: multiply by 2i sin(2wL) e^{-2iwL} (gbgpu.py:301-304) */
+    int precision;             /* GBGPU_FP64 or GBGPU_MIXED */
+} gbgpu_wave_cfg;
+
+/* Data against which templates are scored (get_ll's data/psd arguments,
+ * src/gbgpu/gbgpu.py:580-712). */
+typedef struct gbgpu_data {
+    const gbgpu_cmplx *A, *E;  /* (num_streams, data_length) */
+    const double *psdA, *psdE; /* (num_psds, data_length) */
+    const int *data_index;     /* (num_bin,) or NULL = all zero */
+    const int *noise_index;    /* (num_bin,) or NULL = all zero */
+    int data_length;
+    int start_freq_ind;        /* global bin of element 0 of every stream */
+    double df;                 /* bin width, 1/T */
+} gbgpu_data;
+
+/* ---- drop-in replacements of the reference's native entry points ------- */
+
+/* Replaces get_ll_wrap (src/gbgpu/cutils/gbgpu_utils.hh:9-13,
+ * gbgpu_utils.cu:188-210).  Same arguments and meaning; templates are
+ * (M, num_bin) major.  Bins outside [0, data_length) contribute nothing
+ * (the reference reads out of bounds there).  64-bit flat indexing. */
+int gbgpu_get_ll(gbgpu_cmplx *d_h, gbgpu_cmplx *h_h, const gbgpu_cmplx *A_template,
+                 const gbgpu_cmplx *E_template, const gbgpu_cmplx *A_data, const gbgpu_cmplx *E_data,
+                 const double *A_psd, const double *E_psd, double df, const int *start_ind, int M,
+                 int num_bin, const int *data_index, const int *noise_index, int data_length,
+                 void *stream);
+
+/* Replaces fill_global_wrap (src/gbgpu/cutils/gbgpu_utils.hh:6-7,
+ * gbgpu_utils.cu:104-122): A_glob[g*L + start_b + i] += template[i*num_bin + b]
+ * for 0 <= start_b + i < L.  Deterministic: each output bin is summed by one
+ * thread in (start, binary) order, so results are bit-identical across runs
+ * (the reference's GPU build uses unordered atomics).  num_groups (the
+ * leading dimension of the caller's templates array, templates.shape[0] at
+ * src/gbgpu/gbgpu.py:778) is the one argument the reference does not pass;
+ * binaries whose group index lies outside [0, num_groups) are skipped.
+ * Uses stream-ordered scratch (hipMallocAsync) of 2*M*num_bin complex plus
+ * sort buffers. */
+int gbgpu_fill_global(gbgpu_cmplx *A_glob, gbgpu_cmplx *E_glob, const gbgpu_cmplx *A_template,
+                      const gbgpu_cmplx *E_template, const int *start_ind, int M, int num_bin,
+                      const int *group_index, int data_length, int num_groups, void *stream);
+
+/* fill_global over binary-major rows (no transposition): sample i of binary
+ * b is A_rows[b*row_stride + i]; start_inds are int64 as run_wave produces
+ * them and are shifted by -start_freq_ind (gbgpu.py:804).  group_index may be
+ * NULL (all zero).  Same deterministic summation as gbgpu_fill_global.  Backs
+ * fill_global_template for 2-D inputs and generate_global_template
+ * (src/gbgpu/gbgpu.py:737-878). */
+int gbgpu_fill_global_rows(gbgpu_cmplx *A_glob, gbgpu_cmplx *E_glob, const gbgpu_cmplx *A_rows,
+                           const gbgpu_cmplx *E_rows, long long row_stride, const long long *start_inds,
+                           long long start_freq_ind, int M, int num_bin, const int *group_index,
+                           int data_length, int num_groups, void *stream);
+
+/* ---- waveform-level entry points (replace the xp array code) ----------- */
+
+/* Replaces GBGPUBase.run_wave's array passes (src/gbgpu/gbgpu.py:108-316,
+ * :348-540): writes XYZf and AETf as (num_bin, 3, N) complex128 (either may be
+ * NULL), fftshifted and scaled exactly as the reference, and start_inds
+ * (num_bin,) int64 = rint(f0*T) - N/2. */
+int gbgpu_run_wave(gbgpu_cmplx *XYZf, gbgpu_cmplx *AETf, long long *start_inds, const double *params,
+                   int num_params, int num_bin, const gbgpu_wave_cfg *cfg, void *stream);
+
+/* Fused run_wave + get_ll (src/gbgpu/gbgpu.py:580-735): per binary computes
+ * d_h, h_h (complex, as get_ll_wrap) and loglike = -1/2 Re(d_d + h_h - 2 d_h),
+ * with |d_h| in place of d_h when phase_marginalize (gbgpu.py:714-728).
+ * Templates never leave LDS.  d_h / h_h may be NULL. */
+int gbgpu_wave_ll(double *loglike, gbgpu_cmplx *d_h, gbgpu_cmplx *h_h, const double *params,
+                  int num_params, int num_bin, const gbgpu_wave_cfg *cfg, const gbgpu_data *data,
+                  double d_d, int phase_marginalize, void *stream);
+
+/* ---- library -------------------------------------------------------- */
+
+/* Message describing the last failure on the calling thread. */
+const char *gbgpu_last_error(void);
+
+/* ABI version of this header (1). */
+int gbgpu_abi_version(void);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* GBGPU_HIP_H */

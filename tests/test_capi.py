@@ -1,0 +1,60 @@
+"""The C-ABI library loads and exports every symbol include/*.h declares
+(no compute calls: runs without a GPU)."""
+
+import ctypes
+import re
+from pathlib import Path
+
+import pytest
+
+ROOT = Path(__file__).resolve().parent.parent
+
+
+def declared_functions():
+    names = []
+    for h in sorted((ROOT / "include").glob("*.h")):
+        text = re.sub(r"/\*.*?\*/", "", h.read_text(), flags=re.S)
+        names += re.findall(r"\b(gbgpu_[a-z0-9_]+)\s*\(", text)
+    return sorted(set(names))
+
+
+def test_header_declares_entry_points():
+    names = declared_functions()
+    for n in ("gbgpu_get_ll", "gbgpu_fill_global", "gbgpu_run_wave", "gbgpu_wave_ll", "gbgpu_last_error"):
+        assert n in names
+
+
+def test_library_exports_every_declared_symbol():
+    from gbgpu_amd import _capi
+
+    lib = _capi.load()
+    for n in declared_functions():
+        assert hasattr(lib, n), n
+    # the ctypes table covers exactly the declared functions
+    assert set(_capi.SIGNATURES) == set(declared_functions())
+    assert lib.gbgpu_abi_version() == 1
+
+
+def test_invalid_arguments_return_errors_without_gpu_work():
+    from gbgpu_amd import _capi
+
+    lib = _capi.load()
+    cfg = _capi.WaveCfg(N=100, T=1e7, armlength=2.5e9, orbit_table=1, tdi2=0, precision=0)
+    rc = lib.gbgpu_run_wave(None, None, 1, 1, 9, 4, cfg, None)
+    assert rc == -1 and b"power of two" in lib.gbgpu_last_error()
+    cfg.N = 256
+    rc = lib.gbgpu_run_wave(None, None, 1, 1, 7, 4, cfg, None)
+    assert rc == -1 and b"num_params" in lib.gbgpu_last_error()
+    rc = lib.gbgpu_get_ll(None, None, None, None, None, None, None, None, 1.0, None, 4, -1, None, None, 10, None)
+    assert rc == -1
+    rc = lib.gbgpu_fill_global(None, None, None, None, None, 4, 3, None, 10, -2, None)
+    assert rc == -1
+    # zero-size batches are no-ops
+    assert lib.gbgpu_get_ll(None, None, None, None, None, None, None, None, 1.0, None, 4, 0, None, None, 10, None) == 0
+
+
+def test_missing_library_fails_loudly(tmp_path):
+    from gbgpu_amd import _capi
+
+    with pytest.raises(RuntimeError, match="missing"):
+        _capi.load(tmp_path / "nope.so")

@@ -1,0 +1,95 @@
+"""Constants (notebook known answers) and the analytic orbit model (closed forms)."""
+
+import numpy as np
+import pytest
+
+from gbgpu_amd.orbits import EqualArmlengthOrbits, orbit_table
+from gbgpu_amd.utils import utility as U
+from oracle import fastgb as F
+
+YR = F.YRSID_SI
+
+
+def test_notebook_get_amplitude_and_fdot():
+    # examples/GBGPU_tutorial.ipynb cells 30 and 32 (printed to 9 digits)
+    amp = U.get_amplitude(np.full(3, 0.2), np.full(3, 0.1), np.full(3, 0.1), np.full(3, 6.0))
+    fdot = U.get_fdot(np.full(3, 0.001), m1=np.full(3, 0.2), m2=np.full(3, 0.1))
+    assert np.allclose(amp, 6.37562267e-23, rtol=2e-9, atol=0)
+    assert np.allclose(fdot, 1.73157515e-19, rtol=2e-9, atol=0)
+    # oracle agrees
+    assert np.allclose(F.get_amplitude(0.2, 0.1, 0.1, 6.0), 6.37562267e-23, rtol=2e-9, atol=0)
+    mc = U.get_chirp_mass(0.2, 0.1)
+    assert np.allclose(F.get_fdot(0.001, mc), 1.73157515e-19, rtol=2e-9, atol=0)
+    # chirp-mass inversion round trip
+    assert np.allclose(U.get_chirp_mass_from_f_fdot(0.001, fdot), mc, rtol=1e-12)
+
+
+def test_get_fGW_and_eta():
+    f0 = np.array([1e-3, 2e-3])
+    t = np.array([[0.0, 10.0], [5.0, 7.0]])
+    f = U.get_fGW(f0, np.array([1e-15, 2e-15]), np.array([1e-25, 0.0]), t)
+    assert np.allclose(f[0, 1], 1e-3 + 1e-14 + 0.5e-25 * 100)
+    assert U.get_eta(1.0, 1.0) == 0.25
+
+
+@pytest.mark.parametrize("orb_cls", [EqualArmlengthOrbits, F.AnalyticOrbits])
+def test_orbit_closed_forms(orb_cls):
+    orb = orb_cls()
+    t = np.linspace(0, 3 * YR, 4001)
+    P = [orb.get_pos(t, np.full(t.shape, s)) for s in (1, 2, 3)]
+    L = orb.armlength
+    # arm lengths equal L to first order in ec: |L_ij/L - 1| <= few * ec
+    for i, j in ((0, 1), (0, 2), (1, 2)):
+        Lij = np.linalg.norm(P[j] - P[i], axis=-1)
+        assert np.abs(Lij / L - 1).max() < 5 * F.EC
+    # guiding centre on the 1 AU circle, 1/fm period
+    c = (P[0] + P[1] + P[2]) / 3
+    assert np.allclose(np.linalg.norm(c, axis=-1), F.AU_SI, rtol=3 * F.EC**2)
+    per = 1 / F.FM
+    for s in (1, 2, 3):
+        assert np.allclose(orb.get_pos(np.array([0.0]), s), orb.get_pos(np.array([per]), s), rtol=0, atol=1e-3)
+    # constellation plane inclined 60 degrees to the ecliptic
+    nrm = np.cross(P[1] - P[0], P[2] - P[0])
+    nrm /= np.linalg.norm(nrm, axis=-1, keepdims=True)
+    tilt = np.degrees(np.arccos(np.abs(nrm[:, 2])))
+    assert np.allclose(tilt, 60.0, atol=0.5)
+
+
+def test_product_and_oracle_orbits_agree():
+    t = np.linspace(0, YR, 257)
+    a, b = EqualArmlengthOrbits(), F.AnalyticOrbits()
+    for s in (1, 2, 3):
+        assert np.allclose(a.get_pos(t, s), b.get_pos(t, s), rtol=1e-14, atol=1e-3)
+
+
+def test_orbit_table_layout():
+    orb = EqualArmlengthOrbits()
+    T, N = YR, 64
+    tab = orbit_table(orb, T, N)
+    assert tab.shape == (19, N) and tab.dtype == np.float64 and tab.flags.c_contiguous
+    tm = np.linspace(0, T, N, endpoint=False)
+    assert np.array_equal(tab[0], tm)
+    P2 = orb.get_pos(tm, 2)
+    assert np.array_equal(tab[4:7].T, P2)
+    r13 = (orb.get_pos(tm, 3) - orb.get_pos(tm, 1)) / orb.armlength
+    assert np.allclose(tab[13:16].T, r13, rtol=0, atol=1e-15)
+
+
+def test_get_N_bands_without_sensitivity():
+    f0 = np.array([5e-4, 2e-3, 0.02, 0.05, 0.2])
+    amp = np.full(5, 1e-30)
+    N = U.get_N(amp, f0, YR, sensitivity=False)
+    # 1 yr -> mult 1: 32, 64, 256, 256 (overlapping band applied last), 256
+    assert list(N) == [32, 64, 256, 256, 256]
+    assert list(U.get_N(amp, f0, 4 * YR, oversample=2, sensitivity=False)) == [256, 512, 2048, 2048, 2048]
+
+
+def test_get_N_notebook_cell34():
+    # examples/GBGPU_tutorial.ipynb cell 34: weak 1 mHz source over 1 yr -> 64
+    amp = U.get_amplitude(0.2, 0.1, 0.1, 6.0)
+    assert list(U.get_N(np.full(3, amp), np.full(3, 0.001), YR)) == [64, 64, 64]
+
+
+def test_baseline_N_rule():
+    f = np.array([0.5e-3, 1e-3, 2.9e-3, 3e-3, 5.9e-3, 6e-3, 9.9e-3, 10e-3, 20e-3])
+    assert list(U.baseline_N(f)) == [128, 256, 256, 512, 512, 1024, 1024, 2048, 2048]

@@ -1,0 +1,270 @@
+"""GPU parity: the HIP path against the CPU FP64 oracle on identical seeded inputs.
+
+Bars (stated here, checked below):
+  * start bins: bit-exact;
+  * FP64 waveforms: per-binary relative L2 error <= 1e-9;
+  * FP64 log-likelihoods: |dlogL| <= 1e-8 * max(1, |logL|);
+  * get_ll / fill_global drop-ins: relative 1e-12 against the serial C
+    restatement; fill_global bit-identical across repeated runs.
+"""
+
+import ctypes
+
+import numpy as np
+import pytest
+
+from oracle import fastgb as F
+from tests.helpers import YR, draw_params, make_data, oracle_c, rel_err
+
+pytestmark = pytest.mark.gpu
+
+DT = 15.0
+
+
+def _gb(**kw):
+    from gbgpu_amd import GBGPU
+
+    return GBGPU(**kw)
+
+
+@pytest.mark.parametrize("N", [32, 64, 128, 256, 512, 1024, 2048])
+def test_run_wave_fp64_matches_oracle(N):
+    import torch
+
+    rng = np.random.default_rng(100 + N)
+    B = 37
+    p = draw_params(B, rng)
+    T = YR
+    print(f"run_wave fp64 N={N} B={B}", flush=True)
+    gb = _gb()
+    gb.run_wave(*p, N=N, T=T, dt=DT)
+    torch.cuda.synchronize()
+    o = F.run_wave(*p, N=N, T=T, dt=DT)
+    assert np.array_equal(gb.start_inds.cpu().numpy(), o["start_inds"])
+    xyz = gb.XYZf.cpu().numpy()
+    aet = gb.AETf.cpu().numpy()
+    assert rel_err(xyz, o["XYZf"], axis=(1, 2)).max() <= 1e-9
+    assert rel_err(aet[:, :2], o["AETf"][:, :2], axis=(1, 2)).max() <= 1e-9
+    # T channel is ~f^2-suppressed; compare against the A/E scale
+    assert np.abs(aet[:, 2] - o["AETf"][:, 2]).max() <= 1e-9 * np.abs(o["AETf"]).max()
+    # (N, B)-major *_out views as the reference lays them out
+    assert np.allclose(gb.A_out.cpu().numpy(), o["AETf"][:, 0].T.flatten(), rtol=0, atol=1e-9 * np.abs(o["AETf"]).max())
+    assert np.allclose(gb.freqs.cpu().numpy(), F.freqs(o["start_inds"], N, o["df"]))
+
+
+def test_run_wave_4yr_tdi2():
+    rng = np.random.default_rng(7)
+    p = draw_params(16, rng, f_lo=1e-3, f_hi=6e-3)
+    gb = _gb()
+    gb.run_wave(*p, N=256, T=4 * YR, dt=DT, tdi2=True)
+    o = F.run_wave(*p, N=256, T=4 * YR, dt=DT, tdi2=True)
+    assert np.array_equal(gb.start_inds.cpu().numpy(), o["start_inds"])
+    assert rel_err(gb.XYZf.cpu().numpy(), o["XYZf"], axis=(1, 2)).max() <= 1e-9
+
+
+def test_start_bins_half_integer_and_edges():
+    """rint(f0*T) rounds half to even; f0*T exactly k + 1/2 must match numpy."""
+    T = F.adjust_T(YR, DT)
+    ks = np.array([40000.5, 40001.5, 12345.5, 99999.0, 3.0e5 + 0.5])
+    f0 = ks / T
+    B = len(f0)
+    p = draw_params(B, np.random.default_rng(3))
+    p[1] = f0
+    p[2] = 0.0
+    p[3] = 0.0
+    gb = _gb()
+    gb.run_wave(*p, N=64, T=YR, dt=DT)
+    o = F.run_wave(*p, N=64, T=YR, dt=DT)
+    assert np.array_equal(gb.start_inds.cpu().numpy(), o["start_inds"])
+    assert np.array_equal(o["start_inds"], (np.rint(f0 * T) - 32).astype(np.int64))
+
+
+@pytest.mark.parametrize("N", [128, 256, 1024])
+def test_get_ll_fused_fp64_matches_oracle(N):
+    rng = np.random.default_rng(200 + N)
+    B = 40
+    p = draw_params(B, rng)
+    A, E, SA, SE, df, L = make_data(p[:, :10], N, YR, DT, rng)
+    # score perturbed copies so d_h != h_h
+    q = p.copy()
+    q[1] *= 1 + 1e-7 * rng.standard_normal(B)
+    q[4] += 0.1
+    print(f"wave_ll fp64 N={N}", flush=True)
+    gb = _gb()
+    d_d = float(4 * df * (np.sum(np.abs(A) ** 2 / SA) + np.sum(np.abs(E) ** 2 / SE)))
+    gb.d_d = d_d
+    ll = gb.get_ll(q, [A, E], [SA, SE], N=N, T=YR, dt=DT)
+    o = F.run_wave(*q, N=N, T=YR, dt=DT)
+    d_h, h_h = F.inner_terms(o["AETf"][:, 0], o["AETf"][:, 1], o["start_inds"], A, E, SA, SE, df)
+    ll_o = F.loglike(d_d, d_h, h_h)
+    scale = np.maximum(1.0, np.abs(ll_o))
+    assert np.all(np.abs(ll - ll_o) <= 1e-8 * scale), np.abs(ll - ll_o).max()
+    assert np.allclose(gb.d_h.cpu().numpy(), d_h, rtol=1e-9, atol=1e-9 * np.abs(d_h).max())
+    # phase marginalisation
+    llm = gb.get_ll(q, [A, E], [SA, SE], N=N, T=YR, dt=DT, phase_marginalize=True)
+    llm_o = F.loglike(d_d, d_h, h_h, phase_marginalize=True)
+    assert np.all(np.abs(llm - llm_o) <= 1e-8 * np.maximum(1, np.abs(llm_o)))
+    # templates remain reachable after get_ll, as in the reference
+    assert rel_err(gb.A.cpu().numpy(), o["AETf"][:, 0], axis=1).max() <= 1e-9
+
+
+def test_get_ll_multi_stream_indices():
+    import torch
+
+    rng = np.random.default_rng(11)
+    B, N = 24, 128
+    p = draw_params(B, rng)
+    L = 150000
+    T = YR
+    df = 1 / F.adjust_T(T, DT)
+    start_freq_ind = int(np.rint(1e-3 / df)) - 200
+    S = 3
+    dA = (rng.standard_normal((S, L)) + 1j * rng.standard_normal((S, L))) * 1e-20
+    dE = (rng.standard_normal((S, L)) + 1j * rng.standard_normal((S, L))) * 1e-20
+    f = (np.arange(L) + start_freq_ind) * df
+    from gbgpu_amd.noise import A1TDISens
+
+    SA = np.stack([A1TDISens.get_Sn(f) * (1 + 0.1 * s) for s in range(S)])
+    SE = SA * 1.3
+    di = rng.integers(0, S, B).astype(np.int32)
+    ni = rng.integers(0, S, B).astype(np.int32)
+    gb = _gb()
+    gb.d_d = 0.0
+    ll = gb.get_ll(p, [torch.from_numpy(dA).cuda(), torch.from_numpy(dE).cuda()], [SA, SE], N=N, T=T, dt=DT,
+                   start_freq_ind=start_freq_ind, data_index=di, noise_index=ni)
+    o = F.run_wave(*p, N=N, T=T, dt=DT)
+    d_h, h_h = F.inner_terms(o["AETf"][:, 0], o["AETf"][:, 1], o["start_inds"] - start_freq_ind, dA, dE, SA, SE, df,
+                             di, ni, L)
+    ll_o = F.loglike(0.0, d_h, h_h)
+    assert np.all(np.abs(ll - ll_o) <= 1e-8 * np.maximum(1, np.abs(ll_o)))
+
+
+def test_get_ll_dropin_matches_c_oracle():
+    """gbgpu_get_ll with (M, B)-major templates vs the serial C restatement."""
+    import torch
+
+    from gbgpu_amd import _capi
+
+    rng = np.random.default_rng(5)
+    B, M, L, S = 301, 200, 5000, 2
+    At = (rng.standard_normal((M, B)) + 1j * rng.standard_normal((M, B)))
+    Et = (rng.standard_normal((M, B)) + 1j * rng.standard_normal((M, B)))
+    dA = rng.standard_normal((S, L)) + 1j * rng.standard_normal((S, L))
+    dE = rng.standard_normal((S, L)) + 1j * rng.standard_normal((S, L))
+    SA = rng.uniform(0.5, 2.0, (S, L))
+    SE = rng.uniform(0.5, 2.0, (S, L))
+    start = rng.integers(-50, L - M + 50, B).astype(np.int32)  # includes out-of-range windows
+    di = rng.integers(0, S, B).astype(np.int32)
+    ni = rng.integers(0, S, B).astype(np.int32)
+    df = 1e-8
+    c = oracle_c()
+    dh_c = np.zeros(B, np.complex128)
+    hh_c = np.zeros(B, np.complex128)
+    ptr = lambda a: a.ctypes.data_as(ctypes.c_void_p)
+    c.oracle_get_ll(ptr(dh_c), ptr(hh_c), ptr(At), ptr(Et), ptr(dA), ptr(dE), ptr(SA), ptr(SE), df, ptr(start), M, B,
+                    ptr(di), ptr(ni), L)
+    g = lambda a: torch.from_numpy(np.ascontiguousarray(a)).cuda()
+    dh = torch.zeros(B, dtype=torch.complex128, device="cuda")
+    hh = torch.zeros(B, dtype=torch.complex128, device="cuda")
+    lib = _capi.load()
+    tA, tE, gA, gE, gSA, gSE, gs, gdi, gni = map(g, (At, Et, dA, dE, SA, SE, start, di, ni))
+    rc = lib.gbgpu_get_ll(dh.data_ptr(), hh.data_ptr(), tA.data_ptr(), tE.data_ptr(), gA.data_ptr(), gE.data_ptr(),
+                          gSA.data_ptr(), gSE.data_ptr(), df, gs.data_ptr(), M, B, gdi.data_ptr(), gni.data_ptr(), L,
+                          _capi.stream_handle())
+    _capi.check(rc, "gbgpu_get_ll")
+    assert np.allclose(dh.cpu().numpy(), dh_c, rtol=1e-12, atol=1e-12 * np.abs(dh_c).max())
+    assert np.allclose(hh.cpu().numpy(), hh_c, rtol=1e-12, atol=0)
+    assert np.all(hh.cpu().numpy().imag == 0)
+
+
+def test_fill_global_dropin_deterministic():
+    import torch
+
+    from gbgpu_amd.cutils import get_backend
+
+    rng = np.random.default_rng(9)
+    B, M, L, G = 1500, 128, 20000, 3
+    At = rng.standard_normal((M, B)) + 1j * rng.standard_normal((M, B))
+    Et = rng.standard_normal((M, B)) + 1j * rng.standard_normal((M, B))
+    start = rng.integers(-M, L + 10, B).astype(np.int32)
+    start[:400] = rng.integers(1000, 1100, 400)  # dense overlap region
+    grp = rng.integers(0, G, B).astype(np.int32)
+    c = oracle_c()
+    ref_A = np.zeros((G, L), np.complex128)
+    ref_E = np.zeros((G, L), np.complex128)
+    ptr = lambda a: a.ctypes.data_as(ctypes.c_void_p)
+    c.oracle_fill_global(ptr(ref_A), ptr(ref_E), ptr(At), ptr(Et), ptr(start), M, B, ptr(grp), L)
+    be = get_backend("hip")
+    g = lambda a: torch.from_numpy(np.ascontiguousarray(a)).cuda()
+    outs = []
+    for _ in range(2):
+        oA = torch.zeros(G * L, dtype=torch.complex128, device="cuda")
+        oE = torch.zeros(G * L, dtype=torch.complex128, device="cuda")
+        be.fill_global(oA, oE, g(At.reshape(-1)), g(Et.reshape(-1)), g(start), M, B, g(grp), L)
+        torch.cuda.synchronize()
+        outs.append((oA.cpu().numpy().reshape(G, L), oE.cpu().numpy().reshape(G, L)))
+    scale = np.abs(ref_A).max()
+    assert np.abs(outs[0][0] - ref_A).max() <= 1e-12 * scale
+    assert np.abs(outs[0][1] - ref_E).max() <= 1e-12 * scale
+    assert np.array_equal(outs[0][0].view(np.uint64), outs[1][0].view(np.uint64))
+    assert np.array_equal(outs[0][1].view(np.uint64), outs[1][1].view(np.uint64))
+
+
+def test_fill_global_template_api_matches_oracle():
+    import torch
+
+    rng = np.random.default_rng(21)
+    B, N = 300, 256
+    p = draw_params(B, rng, f_lo=1e-3, f_hi=1.2e-3)  # dense band -> many overlaps
+    T = YR
+    L = 60000
+    start_freq_ind = int(1e-3 * F.adjust_T(T, DT)) - 500
+    grp = rng.integers(0, 2, B).astype(np.int32)
+    gb = _gb()
+    gb.run_wave(*p, N=N, T=T, dt=DT)
+    o = F.run_wave(*p, N=N, T=T, dt=DT)
+    ref = F.fill_global(2, L, o["AETf"][:, 0], o["AETf"][:, 1], o["start_inds"] - start_freq_ind, grp)
+    scale = np.abs(ref).max()
+    # 2-D (B, N) inputs -> row path
+    templates = torch.zeros((2, 2, L), dtype=torch.complex128, device="cuda")
+    gb.fill_global_template(grp, templates, gb.A, gb.E, gb.start_inds, start_freq_ind=start_freq_ind)
+    assert np.abs(templates.cpu().numpy() - ref).max() <= 1e-9 * scale
+    # 1-D (N, B)-flattened inputs -> drop-in path
+    templates2 = torch.zeros((2, 2, L), dtype=torch.complex128, device="cuda")
+    gb.fill_global_template(grp, templates2, gb.A_out, gb.E_out, gb.start_inds, N=N, start_freq_ind=start_freq_ind)
+    assert np.abs(templates2.cpu().numpy() - ref).max() <= 1e-9 * scale
+    # generate_global_template (params as (B, 9))
+    templates3 = torch.zeros((2, 2, L), dtype=torch.complex128, device="cuda")
+    gb.generate_global_template(p.T, grp, templates3, start_freq_ind=start_freq_ind, N=N, T=T, dt=DT)
+    assert np.abs(templates3.cpu().numpy() - ref).max() <= 1e-9 * scale
+
+
+def test_inject_signal_matches_oracle():
+    rng = np.random.default_rng(4)
+    p = draw_params(1, rng)[:, 0]
+    gb = _gb()
+    A, E = gb.inject_signal(*p, N=128, T=YR, dt=DT)
+    o = F.run_wave(*p, N=128, T=YR, dt=DT)
+    s = int(o["start_inds"][0])
+    assert A.shape == E.shape == (len(np.arange(0.0, 1 / (2 * DT) + o["df"], o["df"])),)
+    assert rel_err(A[s : s + 128], o["AETf"][0, 0]) <= 1e-9
+    assert np.all(A[:s] == 0) and np.all(A[s + 128 :] == 0)
+
+
+@pytest.mark.parametrize("e2", [0.0, 0.3])
+def test_third_body_matches_oracle(e2):
+    from gbgpu_amd import GBGPUThirdBody
+
+    rng = np.random.default_rng(31)
+    B, N = 12, 256
+    p = draw_params(B, rng)
+    A2 = rng.uniform(0, 100, B)
+    varpi = rng.uniform(0, 2 * np.pi, B)
+    P2 = 10 ** rng.uniform(0, 1, B)
+    T2 = rng.uniform(0, 1, B)
+    e = np.full(B, e2)
+    gb = GBGPUThirdBody()
+    gb.run_wave(*p, A2, varpi, e, P2, T2, N=N, T=YR, dt=DT)
+    o = F.run_wave(*p, A2, varpi, e, P2, T2, N=N, T=YR, dt=DT)
+    assert np.array_equal(gb.start_inds.cpu().numpy(), o["start_inds"])
+    assert rel_err(gb.XYZf.cpu().numpy(), o["XYZf"], axis=(1, 2)).max() <= 1e-8

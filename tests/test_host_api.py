@@ -1,0 +1,78 @@
+"""Host-side behaviour of the GBGPU mirror that needs no GPU: validation
+errors raised before any device work, as the reference raises them."""
+
+import numpy as np
+import pytest
+
+from gbgpu_amd import GBGPU, GBGPUThirdBody
+from gbgpu_amd.cutils import BackendUnavailableException, get_backend
+from tests.helpers import YR
+
+
+def _params(B=3):
+    return np.array([np.full(B, v) for v in (1e-22, 2e-3, 1e-18, 0.0, 0.1, 0.2, 0.3, 0.4, 0.5)])
+
+
+def test_get_ll_requires_d_d():
+    gb = GBGPU()
+    with pytest.raises(ValueError, match="d_d"):
+        gb.get_ll(_params(), [np.zeros(10, complex)] * 2, [np.ones(10)] * 2)
+
+
+def test_observation_time_beyond_orbits():
+    gb = GBGPU()
+    with pytest.raises(ValueError, match="Observation time"):
+        gb.run_wave(*_params(), N=128, T=20 * YR, dt=15.0)
+    with pytest.raises(ValueError, match="Observation time"):
+        gb.inject_signal(*_params(1)[:, 0], N=128, T=20 * YR, dt=15.0)
+
+
+def test_data_index_with_1d_data_rejected():
+    gb = GBGPU()
+    gb.d_d = 0.0
+    with pytest.raises(ValueError, match="data_index"):
+        gb.get_ll(_params(), [np.zeros(10, complex)] * 2, [np.ones(10)] * 2, data_index=np.zeros(3, np.int32))
+    with pytest.raises(ValueError, match="noise_index"):
+        gb.get_ll(_params(), [np.zeros(10, complex)] * 2, [np.ones(10)] * 2, noise_index=np.zeros(3, np.int32))
+
+
+def test_wrong_data_type_rejected():
+    gb = GBGPU()
+    gb.d_d = 0.0
+    with pytest.raises(TypeError):
+        gb.get_ll(_params(), [[0j] * 10] * 2, [np.ones(10)] * 2)
+
+
+def test_extra_args_on_circular_class_rejected():
+    gb = GBGPU()
+    with pytest.raises(ValueError, match="prepare_additional_args"):
+        gb.run_wave(*_params(), *np.ones((5, 3)), N=128, T=YR, dt=15.0)
+
+
+def test_unknown_backend_and_precision():
+    with pytest.raises(BackendUnavailableException):
+        get_backend("cuda12x")
+    with pytest.raises(ValueError):
+        GBGPU(precision="fp16")
+    assert get_backend("gpu").name == "hip"
+    assert get_backend(("gbgpu", "hip")).name == "hip"
+
+
+def test_third_body_prepare_and_get_N():
+    gb = GBGPUThirdBody()
+    A2, varpi, e2, n2, T2 = gb.prepare_additional_args(19.5, 0.0, 0.3, 0.6, 0.5)
+    assert np.isclose(n2[0], 2 * np.pi / (0.6 * 31558149.763545603))
+    assert np.isclose(T2[0], 0.5 * 31558149.763545603)
+    # a 0.1-yr outer orbit over 4 yr needs >= 2/P2 samples per yr -> N doubles from the base
+    N = gb.special_get_N(np.array([1e-30]), np.array([2e-3]), 4 * YR, 0.0, 0.0, 0.0, np.array([0.1]), 0.0)
+    assert N[0] >= 4 * 2 / 0.1
+
+
+def test_fill_global_template_validation():
+    gb = GBGPU()
+    with pytest.raises(ValueError, match="A and E"):
+        gb.fill_global_template(np.zeros(2, np.int32), np.zeros((1, 1, 10), complex), np.zeros((2, 4)),
+                                np.zeros((2, 4)), np.zeros(2, int))
+    with pytest.raises(TypeError):
+        gb.fill_global_template(np.zeros(2, np.int32), np.zeros((1, 2, 10), complex), np.zeros((2, 4)),
+                                np.zeros((2, 4)), np.zeros(2, int))
