@@ -1,12 +1,12 @@
 // Length-N forward DFT (numpy sign convention, unnormalised) in LDS.
 //
 // Stockham autosort radix-4 stages followed, when log2 N is odd, by one
-// in-place radix-2 stage.  A group of TPB = N/4 threads owns one binary: each
-// thread performs exactly one radix-4 butterfly per channel per stage, holding
-// its four inputs in registers across the barrier that separates the reads
-// from the writes, so a single LDS buffer per channel suffices.  Output is in
-// natural order.  Every thread of the workgroup must call this (it contains
-// __syncthreads); all binaries of a workgroup share N.
+// in-place radix-2 stage.  A group of TPB threads (TPB <= N/4) owns one
+// binary: each thread performs N/(4 TPB) radix-4 butterflies per channel per
+// stage, holding their inputs in registers across the barrier that separates
+// the reads from the writes, so a single LDS buffer per channel suffices.
+// Output is in natural order.  Every thread of the workgroup must call this
+// (it contains __syncthreads); all binaries of a workgroup share N.
 #pragma once
 
 #include "common.hpp"
@@ -25,34 +25,42 @@ __device__ __forceinline__ void bfly4(CT &v0, CT &v1, CT &v2, CT &v3) {
 
 // buf: NCH channels of N contiguous elements for this thread's binary.
 // tw:  w[k] = exp(-2 pi i k / N), k < N.
-template <int N, int NCH, class CT>
+template <int N, int NCH, int TPB, class CT>
 __device__ __forceinline__ void fft_lds(CT *__restrict__ buf, int lane, const CT *__restrict__ tw) {
     constexpr int Q = N / 4;
+    constexpr int R = Q / TPB;  // butterflies per thread per stage
+    static_assert(R >= 1 && R * TPB == Q, "TPB must divide N/4");
 #pragma unroll
     for (int Ns = 1; Ns * 4 <= N; Ns *= 4) {
-        const int k = lane & (Ns - 1);
         const int step = N / (4 * Ns);
-        const CT w1 = tw[k * step];
-        const CT w2 = tw[2 * k * step];
-        const CT w3 = tw[3 * k * step];
-        CT v[NCH][4];
+        CT v[R][NCH][4];
 #pragma unroll
-        for (int c = 0; c < NCH; c++)
+        for (int rr = 0; rr < R; rr++)
 #pragma unroll
-            for (int r = 0; r < 4; r++) v[c][r] = buf[c * N + lane + r * Q];
+            for (int c = 0; c < NCH; c++)
+#pragma unroll
+                for (int r = 0; r < 4; r++) v[rr][c][r] = buf[c * N + lane + rr * TPB + r * Q];
         __syncthreads();
-        const int base = ((lane - k) << 2) + k;
 #pragma unroll
-        for (int c = 0; c < NCH; c++) {
-            CT x0 = v[c][0];
-            CT x1 = cmul(v[c][1], w1);
-            CT x2 = cmul(v[c][2], w2);
-            CT x3 = cmul(v[c][3], w3);
-            bfly4(x0, x1, x2, x3);
-            buf[c * N + base] = x0;
-            buf[c * N + base + Ns] = x1;
-            buf[c * N + base + 2 * Ns] = x2;
-            buf[c * N + base + 3 * Ns] = x3;
+        for (int rr = 0; rr < R; rr++) {
+            const int j = lane + rr * TPB;
+            const int k = j & (Ns - 1);
+            const CT w1 = tw[k * step];
+            const CT w2 = tw[2 * k * step];
+            const CT w3 = tw[3 * k * step];
+            const int base = ((j - k) << 2) + k;
+#pragma unroll
+            for (int c = 0; c < NCH; c++) {
+                CT x0 = v[rr][c][0];
+                CT x1 = cmul(v[rr][c][1], w1);
+                CT x2 = cmul(v[rr][c][2], w2);
+                CT x3 = cmul(v[rr][c][3], w3);
+                bfly4(x0, x1, x2, x3);
+                buf[c * N + base] = x0;
+                buf[c * N + base + Ns] = x1;
+                buf[c * N + base + 2 * Ns] = x2;
+                buf[c * N + base + 3 * Ns] = x3;
+            }
         }
         __syncthreads();
     }
@@ -60,8 +68,8 @@ __device__ __forceinline__ void fft_lds(CT *__restrict__ buf, int lane, const CT
     constexpr bool odd = (N & 0x55555555) == 0;  // N is 2 * 4^m
     if (odd) {
 #pragma unroll
-        for (int h = 0; h < 2; h++) {
-            const int j = lane + h * Q;
+        for (int h = 0; h < 2 * R; h++) {
+            const int j = lane + h * TPB;
             const CT w = tw[j];
 #pragma unroll
             for (int c = 0; c < NCH; c++) {

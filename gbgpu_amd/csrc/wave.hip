@@ -1,147 +1,76 @@
-// FastGB waveform kernels for gfx950: run_wave (XYZ/AET output) and the
-// fused waveform + likelihood, FP64 ("plain") precision.
+// FastGB waveform kernels for gfx950: run_wave (XYZ/AET spectra to HBM) and
+// the fused waveform + likelihood, for both precision policies (prec.hpp).
 //
-// Work decomposition: a group of TPB = N/4 threads owns one binary; a
-// workgroup holds BPW = max(256, TPB)/TPB binaries.  Each thread evaluates the
-// slow part at samples n = lane + q*TPB (q < 4) straight into LDS, the group
-// runs the length-N transform in LDS (fft_lds.hpp), and the epilogue either
-// streams the fftshifted, scaled spectra to HBM (run_wave) or reads the
-// binary's data/PSD window with coalesced loads and reduces <d|h>, <h|h>
-// over the group (wave_ll).
+// Work decomposition: a group of TPB = min(N/4, 256) threads owns one binary
+// and a 256-thread workgroup holds BPW = 256/TPB binaries.  Each thread
+// evaluates the slow part at SPT = N/TPB samples straight into LDS, the group
+// runs the length-N transform in LDS, and the epilogue either streams the
+// fftshifted, scaled spectra to HBM (run_wave) or reads the binary's
+// data/PSD window with coalesced loads and reduces <d|h>, <h|h> over the
+// group (wave_ll).
 #include "common.hpp"
-#include "fft_lds.hpp"
-#include "source.hpp"
 #include "wave_common.hpp"
 
 namespace gbgpu {
 
-template <int N, int MODE, bool THIRD>
-__global__ void __launch_bounds__(Geo<N>::WG) k_wave_f64(WaveArgs a) {
+// LDS budget (160 KB per workgroup on gfx950): the FP64 third-body run_wave
+// variant at N = 2048 would need 168 KB.
+template <int N, int MODE, bool THIRD, class P>
+constexpr bool fits_lds() {
+    return !(THIRD && MODE == MODE_WAVE && N > 1024 && sizeof(typename P::CT) == 16);
+}
+
+template <int N, int MODE, bool THIRD, class P>
+__global__ void __launch_bounds__(WG_SIZE) k_wave(WaveArgs a) {
     using G = Geo<N>;
-    constexpr int TPB = G::TPB, BPW = G::BPW, WPB = G::WPB;
+    using CT = typename P::CT;
+    constexpr int TPB = G::TPB, BPW = G::BPW, WPB = G::WPB, SPT = G::SPT;
     constexpr int NCH = (MODE == MODE_WAVE) ? 3 : 2;
     constexpr int NT = THIRD ? N : 1;
-    __shared__ double2 sbuf[BPW * NCH * N];
+    // xi shares the transform buffer when that is large enough
+    constexpr bool XI_IN_BUF = NCH * sizeof(CT) * N >= 3 * sizeof(double) * N;
+    constexpr int NXI = (THIRD && !XI_IN_BUF) ? N : 1;
+    __shared__ CT sbuf[BPW * NCH * N];
     __shared__ Src ssrc[BPW];
+    __shared__ typename P::Aux saux[BPW];
     __shared__ double stot[BPW * WPB];
-    // third body: v_LOS kept in FP32 (it enters only as 1 + v/c), the phase
-    // integral in FP64; xi lives in the transform buffer until the slow part.
     __shared__ float s_v[BPW * 3 * NT];
     __shared__ double s_g[BPW * 3 * NT];
+    __shared__ double s_xi[BPW * 3 * NXI];
 
     const int tid = threadIdx.x;
     const int slot = tid / TPB;
     const int lane = tid % TPB;
     const long long b = (long long)blockIdx.x * BPW + slot;
     const bool valid = b < a.B;
-    const double2 *tw = static_cast<const double2 *>(a.tw);
 
-    if (lane == 0 && valid) ssrc[slot] = make_src(a.params, a.num_params, a.B, b, N, a.T, a.tdi2, a.armlength);
+    if (lane == 0 && valid) {
+        const Src s0 = make_src(a.params, a.num_params, a.B, b, N, a.T, a.tdi2, a.armlength);
+        ssrc[slot] = s0;
+        saux[slot] = P::make_aux(s0);
+    }
     __syncthreads();
     const Src s = ssrc[slot];
-    double2 *buf = sbuf + slot * NCH * N;
+    CT *buf = sbuf + slot * NCH * N;
     double *tot = stot + slot * WPB;
+    ThirdScratch ts{XI_IN_BUF ? reinterpret_cast<double *>(buf) : s_xi + slot * 3 * NXI, s_v + slot * 3 * NT,
+                    s_g + slot * 3 * NT};
 
-    if constexpr (THIRD) {
-        // Third-body pre-pass (thirdbody.py:155-226, :382-428): v_LOS and
-        // g = v_LOS * f_GW at xi_sc(t_n), then the cumulative trapezoid
-        // term_n = sum_{m=1..n} (xi_m - xi_{m-1}) (g_{m-1} + g_m)/2 * 2 pi / c.
-        double *xi3 = reinterpret_cast<double *>(buf);  // 3N doubles <= NCH*N double2
-        float *v3 = s_v + slot * 3 * N;
-        double *g3 = s_g + slot * 3 * N;
-#pragma unroll
-        for (int q = 0; q < 4; q++) {
-            const int n = lane + q * TPB;
-#pragma unroll
-            for (int sc = 0; sc < 3; sc++) {
-                double xi = 0.0, gg = 0.0;
-                float v = 0.0f;
-                if (valid) {
-                    const double t = a.tab[n];
-                    const double P0 = a.tab[(1 + 3 * sc) * N + n], P1 = a.tab[(2 + 3 * sc) * N + n],
-                                 P2 = a.tab[(3 + 3 * sc) * N + n];
-                    xi = t - (s.k[0] * P0 + s.k[1] * P1 + s.k[2] * P2) * (1.0 / C_SI);
-                    const double vd = third_vlos(s, xi);
-                    v = (float)vd;
-                    gg = vd * (s.f0 + s.fdot * xi + 0.5 * s.fddot * xi * xi);
-                }
-                xi3[sc * N + n] = xi;
-                v3[sc * N + n] = v;
-                g3[sc * N + n] = gg;
-            }
-        }
-        __syncthreads();
-        double loc[3][4];
-#pragma unroll
-        for (int sc = 0; sc < 3; sc++) {
-            double acc = 0.0;
-#pragma unroll
-            for (int i = 0; i < 4; i++) {
-                const int n = 4 * lane + i;
-                if (n > 0) {
-                    const double dtt = xi3[sc * N + n] - xi3[sc * N + n - 1];
-                    acc += dtt * (g3[sc * N + n - 1] + g3[sc * N + n]) / 2.0 * PI * 2 / C_SI;
-                }
-                loc[sc][i] = acc;
-            }
-        }
-        __syncthreads();
-#pragma unroll
-        for (int sc = 0; sc < 3; sc++) {
-            const double pre = group_excl_scan<TPB>(loc[sc][3], lane, tot);
-#pragma unroll
-            for (int i = 0; i < 4; i++) g3[sc * N + 4 * lane + i] = pre + loc[sc][i];
-        }
-        __syncthreads();
-    }
+    binary_spectrum<N, NCH, THIRD, P>(s, saux[slot], valid, lane, a.tab, a.fstar_inv, a.scale, buf,
+                                      static_cast<const CT *>(a.tw), ts, tot);
 
-    // slow part straight into LDS
-#pragma unroll
-    for (int q = 0; q < 4; q++) {
-        const int n = lane + q * TPB;
-        double2 X = make_double2(0.0, 0.0), Y = X, Z = X;
-        if (valid) {
-            const SampleGeom g = load_geom(s, a.tab, N, n);
-            double v3[3] = {0.0, 0.0, 0.0}, t3[3] = {0.0, 0.0, 0.0};
-            if constexpr (THIRD) {
-#pragma unroll
-                for (int sc = 0; sc < 3; sc++) {
-                    v3[sc] = (double)s_v[slot * 3 * N + sc * N + n];
-                    t3[sc] = s_g[slot * 3 * N + sc * N + n];
-                }
-            }
-            slow_xyz_f64<THIRD>(s, g, a.fstar_inv, a.scale, v3, t3, X, Y, Z);
-        }
-        if constexpr (MODE == MODE_WAVE) {
-            buf[n] = X;
-            buf[N + n] = Y;
-            buf[2 * N + n] = Z;
-        } else {
-            // A = (Z - X)/sqrt2, E = (X - 2Y + Z)/sqrt6 (utility.py:34-38); linear,
-            // so it commutes with the transform.
-            buf[n] = cscale(csub(Z, X), INV_SQRT2);
-            buf[N + n] = cscale(cadd(csub(X, cscale(Y, 2.0)), Z), INV_SQRT6);
-        }
-    }
-    __syncthreads();
-
-    fft_lds<N, NCH>(buf, lane, tw);
-
-    const double2 tdf = make_double2(s.tdr, s.tdi);
+    const double2 fac = P::fac(s);
     if constexpr (MODE == MODE_WAVE) {
         if (!valid) return;
         double2 *xyz = a.XYZf ? a.XYZf + b * 3 * N : nullptr;
         double2 *aet = a.AETf ? a.AETf + b * 3 * N : nullptr;
 #pragma unroll
-        for (int q = 0; q < 4; q++) {
+        for (int q = 0; q < SPT; q++) {
             const int m = lane + q * TPB;
             const int idx = (m + N / 2) & (N - 1);  // fftshift
-            double2 X = buf[idx], Y = buf[N + idx], Z = buf[2 * N + idx];
-            if (a.tdi2) {
-                X = cmul(X, tdf);
-                Y = cmul(Y, tdf);
-                Z = cmul(Z, tdf);
-            }
+            const double2 X = cmul(P::up(buf[idx]), fac);
+            const double2 Y = cmul(P::up(buf[N + idx]), fac);
+            const double2 Z = cmul(P::up(buf[2 * N + idx]), fac);
             if (xyz) {
                 xyz[m] = X;
                 xyz[N + m] = Y;
@@ -159,25 +88,7 @@ __global__ void __launch_bounds__(Geo<N>::WG) k_wave_f64(WaveArgs a) {
         if (valid) {
             const long long doff = (a.di ? (long long)a.di[b] : 0LL) * a.L;
             const long long noff = (a.ni ? (long long)a.ni[b] : 0LL) * a.L;
-            const long long j0 = s.start - a.start_freq_ind;
-#pragma unroll
-            for (int q = 0; q < 4; q++) {
-                const int m = lane + q * TPB;
-                const int idx = (m + N / 2) & (N - 1);
-                const long long j = j0 + m;
-                if (j < 0 || j >= a.L) continue;
-                double2 hA = buf[idx], hE = buf[N + idx];
-                if (a.tdi2) {
-                    hA = cmul(hA, tdf);
-                    hE = cmul(hE, tdf);
-                }
-                const double2 dA = a.dA[doff + j], dE = a.dE[doff + j];
-                const double iA = 1.0 / a.SA[noff + j], iE = 1.0 / a.SE[noff + j];
-                // conj(d) h / S
-                dhr += (dA.x * hA.x + dA.y * hA.y) * iA + (dE.x * hE.x + dE.y * hE.y) * iE;
-                dhi += (dA.x * hA.y - dA.y * hA.x) * iA + (dE.x * hE.y - dE.y * hE.x) * iE;
-                hh += (hA.x * hA.x + hA.y * hA.y) * iA + (hE.x * hE.x + hE.y * hE.y) * iE;
-            }
+            ll_partial<N, P>(buf, fac, lane, s.start - a.start_freq_ind, doff, noff, a, dhr, dhi, hh);
         }
         dhr = group_total<TPB>(dhr, lane, tot);
         dhi = group_total<TPB>(dhi, lane, tot);
@@ -195,25 +106,19 @@ __global__ void __launch_bounds__(Geo<N>::WG) k_wave_f64(WaveArgs a) {
 }
 
 // --------------------------------------------------------------- dispatch ---
-// The third-body run_wave variant at N = 2048 would need 168 KB of LDS.
-template <int N, int MODE, bool THIRD>
-constexpr bool fits_lds() {
-    return !(THIRD && MODE == MODE_WAVE && N > 1024);
-}
-
-template <int MODE, bool THIRD>
-static int launch_f64(int N, const WaveArgs &a, hipStream_t st) {
-#define GBGPU_CASE(NN)                                                                      \
-    case NN: {                                                                              \
-        if constexpr (fits_lds<NN, MODE, THIRD>()) {                                        \
-            const long long grid = (a.B + Geo<NN>::BPW - 1) / Geo<NN>::BPW;                 \
-            hipLaunchKernelGGL((k_wave_f64<NN, MODE, THIRD>), dim3((unsigned)grid),       \
-                               dim3(Geo<NN>::WG), 0, st, a);                                \
-        } else {                                                                            \
-            set_error("third-body run_wave supports N <= 1024 (LDS limit)");                \
-            return GBGPU_E_ARG;                                                             \
-        }                                                                                   \
-        break;                                                                              \
+template <int MODE, bool THIRD, class P>
+static int launch_wave(int N, const WaveArgs &a, hipStream_t st) {
+#define GBGPU_CASE(NN)                                                                              \
+    case NN: {                                                                                      \
+        if constexpr (fits_lds<NN, MODE, THIRD, P>()) {                                             \
+            const long long grid = (a.B + Geo<NN>::BPW - 1) / Geo<NN>::BPW;                         \
+            hipLaunchKernelGGL((k_wave<NN, MODE, THIRD, P>), dim3((unsigned)grid), dim3(WG_SIZE), 0, \
+                               st, a);                                                              \
+        } else {                                                                                    \
+            set_error("FP64 third-body run_wave supports N <= 1024 (LDS limit)");                   \
+            return GBGPU_E_ARG;                                                                     \
+        }                                                                                           \
+        break;                                                                                      \
     }
     switch (N) {
         GBGPU_CASE(32)
@@ -228,11 +133,18 @@ static int launch_f64(int N, const WaveArgs &a, hipStream_t st) {
             return GBGPU_E_ARG;
     }
 #undef GBGPU_CASE
-    GBGPU_CHECK_LAUNCH("k_wave_f64");
+    GBGPU_CHECK_LAUNCH("k_wave");
     return GBGPU_OK;
 }
 
-static int check_cfg(const double *params, int num_params, int num_bin, const gbgpu_wave_cfg *cfg) {
+template <int MODE>
+static int launch_any(int N, bool third, int precision, const WaveArgs &a, hipStream_t st) {
+    if (precision == GBGPU_MIXED)
+        return third ? launch_wave<MODE, true, Mixed>(N, a, st) : launch_wave<MODE, false, Mixed>(N, a, st);
+    return third ? launch_wave<MODE, true, F64>(N, a, st) : launch_wave<MODE, false, F64>(N, a, st);
+}
+
+int check_cfg(const double *params, int num_params, int num_bin, const gbgpu_wave_cfg *cfg) {
     if (!cfg || !params || !cfg->orbit_table) {
         set_error("null params, cfg or orbit_table");
         return GBGPU_E_ARG;
@@ -256,7 +168,7 @@ static int check_cfg(const double *params, int num_params, int num_bin, const gb
     return GBGPU_OK;
 }
 
-static WaveArgs base_args(const double *params, int num_params, int num_bin, const gbgpu_wave_cfg *cfg) {
+WaveArgs base_args(const double *params, int num_params, long long num_bin, const gbgpu_wave_cfg *cfg) {
     WaveArgs a = {};
     a.params = params;
     a.num_params = num_params;
@@ -270,8 +182,29 @@ static WaveArgs base_args(const double *params, int num_params, int num_bin, con
     return a;
 }
 
-int mixed_run_wave(const WaveArgs &a, int N, bool third, hipStream_t st);
-int mixed_wave_ll(const WaveArgs &a, int N, bool third, hipStream_t st);
+void set_data(WaveArgs &a, const gbgpu_data *data) {
+    a.dA = reinterpret_cast<const double2 *>(data->A);
+    a.dE = reinterpret_cast<const double2 *>(data->E);
+    a.SA = data->psdA;
+    a.SE = data->psdE;
+    a.di = data->data_index;
+    a.ni = data->noise_index;
+    a.L = data->data_length;
+    a.start_freq_ind = data->start_freq_ind;
+    a.df4 = 4.0 * data->df;
+}
+
+int check_data(const gbgpu_data *data) {
+    if (!data || !data->A || !data->E || !data->psdA || !data->psdE || data->data_length <= 0) {
+        set_error("null data pointers or data_length <= 0");
+        return GBGPU_E_ARG;
+    }
+    return GBGPU_OK;
+}
+
+const void *twiddles_for(int N, int precision) {
+    return precision == GBGPU_MIXED ? (const void *)twiddles_f32(N) : (const void *)twiddles_f64(N);
+}
 
 }  // namespace gbgpu
 
@@ -290,16 +223,10 @@ extern "C" int gbgpu_run_wave(gbgpu_cmplx *XYZf, gbgpu_cmplx *AETf, long long *s
     a.XYZf = reinterpret_cast<double2 *>(XYZf);
     a.AETf = reinterpret_cast<double2 *>(AETf);
     a.start_inds = start_inds;
-    const bool third = num_params == GBGPU_NPARAMS_THIRD;
-    hipStream_t st = static_cast<hipStream_t>(stream);
-    if (cfg->precision == GBGPU_MIXED) {
-        a.tw = twiddles_f32(cfg->N);
-        if (!a.tw) return hip_fail(hipErrorOutOfMemory, "twiddles");
-        return mixed_run_wave(a, cfg->N, third, st);
-    }
-    a.tw = twiddles_f64(cfg->N);
+    a.tw = twiddles_for(cfg->N, cfg->precision);
     if (!a.tw) return hip_fail(hipErrorOutOfMemory, "twiddles");
-    return third ? launch_f64<MODE_WAVE, true>(cfg->N, a, st) : launch_f64<MODE_WAVE, false>(cfg->N, a, st);
+    return launch_any<MODE_WAVE>(cfg->N, num_params == GBGPU_NPARAMS_THIRD, cfg->precision, a,
+                                 static_cast<hipStream_t>(stream));
 }
 
 extern "C" int gbgpu_wave_ll(double *loglike, gbgpu_cmplx *d_h, gbgpu_cmplx *h_h, const double *params,
@@ -307,34 +234,21 @@ extern "C" int gbgpu_wave_ll(double *loglike, gbgpu_cmplx *d_h, gbgpu_cmplx *h_h
                              double d_d, int phase_marginalize, void *stream) {
     int rc = check_cfg(params, num_params, num_bin, cfg);
     if (rc) return rc;
-    if (!loglike || !data || !data->A || !data->E || !data->psdA || !data->psdE || data->data_length <= 0) {
-        set_error("null loglike/data pointers or data_length <= 0");
+    if ((rc = check_data(data))) return rc;
+    if (!loglike) {
+        set_error("loglike must not be null");
         return GBGPU_E_ARG;
     }
     if (num_bin == 0) return GBGPU_OK;
     WaveArgs a = base_args(params, num_params, num_bin, cfg);
-    a.dA = reinterpret_cast<const double2 *>(data->A);
-    a.dE = reinterpret_cast<const double2 *>(data->E);
-    a.SA = data->psdA;
-    a.SE = data->psdE;
-    a.di = data->data_index;
-    a.ni = data->noise_index;
-    a.L = data->data_length;
-    a.start_freq_ind = data->start_freq_ind;
-    a.df4 = 4.0 * data->df;
+    set_data(a, data);
     a.d_d = d_d;
     a.phase_marg = phase_marginalize;
     a.loglike = loglike;
     a.d_h = reinterpret_cast<double2 *>(d_h);
     a.h_h = reinterpret_cast<double2 *>(h_h);
-    const bool third = num_params == GBGPU_NPARAMS_THIRD;
-    hipStream_t st = static_cast<hipStream_t>(stream);
-    if (cfg->precision == GBGPU_MIXED) {
-        a.tw = twiddles_f32(cfg->N);
-        if (!a.tw) return hip_fail(hipErrorOutOfMemory, "twiddles");
-        return mixed_wave_ll(a, cfg->N, third, st);
-    }
-    a.tw = twiddles_f64(cfg->N);
+    a.tw = twiddles_for(cfg->N, cfg->precision);
     if (!a.tw) return hip_fail(hipErrorOutOfMemory, "twiddles");
-    return third ? launch_f64<MODE_LL, true>(cfg->N, a, st) : launch_f64<MODE_LL, false>(cfg->N, a, st);
+    return launch_any<MODE_LL>(cfg->N, num_params == GBGPU_NPARAMS_THIRD, cfg->precision, a,
+                               static_cast<hipStream_t>(stream));
 }

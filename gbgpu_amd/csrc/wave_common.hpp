@@ -1,18 +1,28 @@
-// Scaffolding shared by the FP64 and mixed-precision waveform kernels:
-// launch geometry, kernel arguments and group-level scan/reduction.
+// Scaffolding shared by the waveform kernels: launch geometry, kernel
+// arguments, group-level scan/reduction, and the per-binary spectrum routine
+// (third-body pre-pass -> slow part -> LDS transform).
 #pragma once
 
 #include "common.hpp"
+#include "fft_lds.hpp"
+#include "prec.hpp"
+#include "source.hpp"
 
 namespace gbgpu {
 
 enum { MODE_WAVE = 0, MODE_LL = 1 };
 
+constexpr int WG_SIZE = 256;
+
+// A group of TPB = min(N/4, 256) threads owns one binary; each thread holds
+// SPT = N/TPB samples (4 up to N = 1024, 8 at 2048).  Workgroups are 256
+// threads holding BPW binaries.
 template <int N>
 struct Geo {
-    static constexpr int TPB = N / 4;
-    static constexpr int WG = TPB >= 256 ? TPB : 256;
+    static constexpr int TPB = (N / 4 < WG_SIZE) ? N / 4 : WG_SIZE;
+    static constexpr int WG = WG_SIZE;
     static constexpr int BPW = WG / TPB;
+    static constexpr int SPT = N / TPB;
     static constexpr int WPB = (TPB + 63) / 64;  // wavefronts per binary
 };
 
@@ -75,6 +85,130 @@ __device__ __forceinline__ double group_total(double x, int lane, double *tot) {
         __syncthreads();
     }
     return x;
+}
+
+// LDS scratch of one binary for the third-body pre-pass (3N each).
+struct ThirdScratch {
+    double *xi;
+    float *v;
+    double *g;  // g = v f_GW, then the cumulative phase term
+};
+
+// Computes the NCH spectra of this group's binary into `buf` (NCH x N,
+// natural FFT order, before fftshift and before P::fac()).  NCH == 3 gives
+// X, Y, Z; NCH == 2 gives A, E.  Invalid slots (beyond the batch) write
+// zeros and still take part in every barrier.
+template <int N, int NCH, bool THIRD, class P>
+__device__ __forceinline__ void binary_spectrum(const Src &s, const typename P::Aux &aux, bool valid, int lane,
+                                                const double *__restrict__ tab, double fstar_inv, double scale,
+                                                typename P::CT *buf, const typename P::CT *__restrict__ tw,
+                                                ThirdScratch ts, double *tot) {
+    using G = Geo<N>;
+    using CT = typename P::CT;
+    constexpr int TPB = G::TPB, SPT = G::SPT;
+    if constexpr (THIRD) {
+        // Third-body pre-pass (thirdbody.py:155-226, :382-428): v_LOS and
+        // g = v_LOS f_GW at xi_sc(t_n), then the cumulative trapezoid
+        // term_n = sum_{m=1..n} (xi_m - xi_{m-1}) (g_{m-1} + g_m)/2 * 2 pi / c.
+#pragma unroll
+        for (int q = 0; q < SPT; q++) {
+            const int n = lane + q * TPB;
+#pragma unroll
+            for (int sc = 0; sc < 3; sc++) {
+                double xi = 0.0, gg = 0.0;
+                float v = 0.0f;
+                if (valid) {
+                    const double t = tab[n];
+                    const double P0 = tab[(1 + 3 * sc) * N + n], P1 = tab[(2 + 3 * sc) * N + n],
+                                 P2 = tab[(3 + 3 * sc) * N + n];
+                    xi = t - (s.k[0] * P0 + s.k[1] * P1 + s.k[2] * P2) * (1.0 / C_SI);
+                    const double vd = third_vlos(s, xi);
+                    v = (float)vd;
+                    gg = vd * (s.f0 + s.fdot * xi + 0.5 * s.fddot * xi * xi);
+                }
+                ts.xi[sc * N + n] = xi;
+                ts.v[sc * N + n] = v;
+                ts.g[sc * N + n] = gg;
+            }
+        }
+        __syncthreads();
+        double loc[3][SPT];
+#pragma unroll
+        for (int sc = 0; sc < 3; sc++) {
+            double acc = 0.0;
+#pragma unroll
+            for (int i = 0; i < SPT; i++) {
+                const int n = SPT * lane + i;
+                if (n > 0) {
+                    const double dtt = ts.xi[sc * N + n] - ts.xi[sc * N + n - 1];
+                    acc += dtt * (ts.g[sc * N + n - 1] + ts.g[sc * N + n]) / 2.0 * PI * 2 / C_SI;
+                }
+                loc[sc][i] = acc;
+            }
+        }
+        __syncthreads();
+#pragma unroll
+        for (int sc = 0; sc < 3; sc++) {
+            const double pre = group_excl_scan<TPB>(loc[sc][SPT - 1], lane, tot);
+#pragma unroll
+            for (int i = 0; i < SPT; i++) ts.g[sc * N + SPT * lane + i] = pre + loc[sc][i];
+        }
+        __syncthreads();
+    }
+    using R = decltype(CT{}.x);
+#pragma unroll
+    for (int q = 0; q < SPT; q++) {
+        const int n = lane + q * TPB;
+        CT X{}, Y{}, Z{};
+        if (valid) {
+            float v3[3] = {0.f, 0.f, 0.f};
+            double t3[3] = {0.0, 0.0, 0.0};
+            if constexpr (THIRD) {
+#pragma unroll
+                for (int sc = 0; sc < 3; sc++) {
+                    v3[sc] = ts.v[sc * N + n];
+                    t3[sc] = ts.g[sc * N + n];
+                }
+            }
+            P::template sample<THIRD>(s, aux, tab, N, n, fstar_inv, scale, v3, t3, X, Y, Z);
+        }
+        if constexpr (NCH == 3) {
+            buf[n] = X;
+            buf[N + n] = Y;
+            buf[2 * N + n] = Z;
+        } else {
+            // A = (Z - X)/sqrt2, E = (X - 2Y + Z)/sqrt6 (utility.py:34-38); linear,
+            // so it commutes with the transform.
+            buf[n] = cscale(csub(Z, X), (R)INV_SQRT2);
+            buf[N + n] = cscale(cadd(csub(X, cscale(Y, (R)2)), Z), (R)INV_SQRT6);
+        }
+    }
+    __syncthreads();
+    fft_lds<N, NCH, TPB>(buf, lane, tw);
+}
+
+// Accumulates this thread's share of <d|h> (complex) and <h|h> for a binary
+// whose A/E spectra sit in buf (natural order), over output bins
+// m = lane + q TPB (fftshifted), data bin j = j0 + m.
+template <int N, class P>
+__device__ __forceinline__ void ll_partial(const typename P::CT *buf, double2 fac, int lane, long long j0,
+                                           long long doff, long long noff, const WaveArgs &a, double &dhr,
+                                           double &dhi, double &hh) {
+    using G = Geo<N>;
+#pragma unroll
+    for (int q = 0; q < G::SPT; q++) {
+        const int m = lane + q * G::TPB;
+        const int idx = (m + N / 2) & (N - 1);
+        const long long j = j0 + m;
+        if (j < 0 || j >= a.L) continue;
+        const double2 hA = cmul(P::up(buf[idx]), fac), hE = cmul(P::up(buf[N + idx]), fac);
+        const double2 dA = a.dA[doff + j], dE = a.dE[doff + j];
+        const double iA = 1.0 / a.SA[noff + j], iE = 1.0 / a.SE[noff + j];
+        // conj(d) h / S
+        dhr += (dA.x * hA.x + dA.y * hA.y) * iA + (dE.x * hE.x + dE.y * hE.y) * iE;
+        dhi += (dA.x * hA.y - dA.y * hA.x) * iA + (dE.x * hE.y - dE.y * hE.x) * iE;
+        hh += (hA.x * hA.x + hA.y * hA.y) * iA + (hE.x * hE.x + hE.y * hE.y) * iE;
+    }
 }
 
 }  // namespace gbgpu
