@@ -17,7 +17,7 @@ LIB_PATH = Path(__file__).resolve().parent / "libgbgpu_hip.so"
 
 GBGPU_FP64 = 0
 GBGPU_MIXED = 1
-ORBIT_ROWS = 19
+ORBIT_ROWS = 22
 
 c_void_p = ctypes.c_void_p
 c_int = ctypes.c_int

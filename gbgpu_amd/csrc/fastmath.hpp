@@ -1,20 +1,72 @@
-// FP32 sine/cosine for the mixed-precision slow part.
+// Sine/cosine and reciprocal building blocks for the waveform kernels.
 //
-// Both routines reduce to y in [-pi/4, pi/4] plus a quadrant and evaluate
-// the Taylor series of sin (through y^9) and cos (through y^10); the first
-// omitted terms are |y|^11/11! <= 1.8e-9 and |y|^12/12! <= 1.2e-10, below
-// FP32 rounding.
-//   sincos_f32(x):         x in radians, |x| < ~1e4 (Cody-Waite pi/2 split);
-//   sincos_rev(r):         r in revolutions held in FP64 (any magnitude): the
-//                          quadrant split r - k/4 is done in FP64 (exact for
-//                          |r| < 2^50), so a large FP64 phase loses nothing
-//                          before the FP32 polynomial.
+// All sine/cosine routines reduce x = y + q*pi/2 with y in [-pi/4, pi/4] and
+// evaluate Taylor series of sin and cos in y (coefficients 1/n!).  The first
+// omitted terms bound the truncation error:
+//   FP64: sin through y^17, cos through y^18 -> |y|^19/19! < 1e-19;
+//   FP32: sin through y^9,  cos through y^10 -> |y|^11/11! < 1.8e-9.
+// Reductions:
+//   sincos_f64(x): k = rint(2x/pi), y = x - k*P1 - k*P2 with two FMAs, where
+//     P1 = fl(pi/2) and P2 = fl(pi/2 - P1); absolute error ~1e-16 for
+//     |k| < 2^31 (|x| < 3e9 rad), far beyond the phases met here.
+//   sincos_f32(x): the same in FP32 for |x| < ~1e4;
+//   sincos_rev(r): r in revolutions held in FP64: the quarter-turn split
+//     r - k/4 is exact in FP64, the remainder goes to the FP32 polynomial.
 #pragma once
 
 #include "common.hpp"
 
 namespace gbgpu {
 
+// ------------------------------------------------------------------ FP64 ---
+__device__ __forceinline__ void sincos_poly_f64(double y, int q, double &s, double &c) {
+    const double y2 = y * y;
+    double ps = 2.8114572543455206e-15;        // 1/17!
+    ps = fma(ps, y2, -7.647163731819816e-13);  // -1/15!
+    ps = fma(ps, y2, 1.6059043836821613e-10);  // 1/13!
+    ps = fma(ps, y2, -2.505210838544172e-08);  // -1/11!
+    ps = fma(ps, y2, 2.7557319223985893e-06);  // 1/9!
+    ps = fma(ps, y2, -1.984126984126984e-04);  // -1/7!
+    ps = fma(ps, y2, 8.333333333333333e-03);   // 1/5!
+    ps = fma(ps, y2, -0.16666666666666666);    // -1/3!
+    const double sy = fma(y * y2, ps, y);
+    double pc = 1.5619206968586225e-16;        // 1/18!
+    pc = fma(pc, y2, -4.779477332387385e-14);  // -1/16!
+    pc = fma(pc, y2, 1.1470745597729725e-11);  // 1/14!
+    pc = fma(pc, y2, -2.08767569878681e-09);   // -1/12!
+    pc = fma(pc, y2, 2.755731922398589e-07);   // 1/10!
+    pc = fma(pc, y2, -2.48015873015873e-05);   // -1/8!
+    pc = fma(pc, y2, 1.388888888888889e-03);   // 1/6!
+    pc = fma(pc, y2, -0.041666666666666664);   // -1/4!
+    pc = fma(pc, y2, 0.5);                     // 1/2!
+    const double cy = fma(-y2, pc, 1.0);
+    const bool swap = q & 1;
+    const double s0 = swap ? cy : sy;
+    const double c0 = swap ? sy : cy;
+    s = (q & 2) ? -s0 : s0;
+    c = ((q + 1) & 2) ? -c0 : c0;
+}
+
+__device__ __forceinline__ void sincos_f64(double x, double &s, double &c) {
+    constexpr double TWO_OVER_PI = 0.63661977236758134308;
+    constexpr double P1 = 1.5707963267948966;      // fl(pi/2)
+    constexpr double P2 = 6.123233995736766e-17;   // fl(pi/2 - P1)
+    const double k = rint(x * TWO_OVER_PI);
+    double y = fma(-k, P1, x);
+    y = fma(-k, P2, y);
+    sincos_poly_f64(y, (int)(long long)k & 3, s, c);
+}
+
+// 1/x to ~1 ulp: hardware estimate refined by two Newton steps.
+__device__ __forceinline__ double rcp_f64(double x) {
+    double r = __builtin_amdgcn_rcp(x);
+    double e = fma(-x, r, 1.0);
+    r = fma(r, e, r);
+    e = fma(-x, r, 1.0);
+    return fma(r, e, r);
+}
+
+// ------------------------------------------------------------------ FP32 ---
 __device__ __forceinline__ void sincos_poly(float y, int q, float &s, float &c) {
     const float y2 = y * y;
     float ps = 1.0f / 362880.0f;
@@ -38,8 +90,8 @@ __device__ __forceinline__ void sincos_poly(float y, int q, float &s, float &c) 
 
 __device__ __forceinline__ void sincos_f32(float x, float &s, float &c) {
     constexpr float TWO_OVER_PI = 0.636619772367581343f;
-    constexpr float PIO2_HI = 1.57079637050628662109375f;     // float(pi/2)
-    constexpr float PIO2_LO = -4.37113900018624283e-8f;       // pi/2 - PIO2_HI
+    constexpr float PIO2_HI = 1.57079637050628662109375f;  // fl32(pi/2)
+    constexpr float PIO2_LO = -4.37113900018624283e-8f;    // pi/2 - PIO2_HI
     const float k = rintf(x * TWO_OVER_PI);
     const float y = fmaf(-k, PIO2_LO, fmaf(-k, PIO2_HI, x));
     sincos_poly(y, ((int)k) & 3, s, c);
@@ -50,5 +102,93 @@ __device__ __forceinline__ void sincos_rev(double r, float &s, float &c) {
     const float y = (float)(r - 0.25 * k) * 6.28318530717958647692f;
     sincos_poly(y, ((long long)k) & 3, s, c);
 }
+
+// ------------------------------------------------- reduction-free series ---
+// Coefficients of the two even series
+//   Rs(v) = sum_k (-1)^k v^k / (2k+1)!   (sin x = x Rs(x^2), sin u / u = Rs(u^2))
+//   Is(v) = sum_k (-1)^k v^k / (2k+2)!   (cos x = 1 - x^2 Is(x^2),
+//                                         (1 - cos u)/u = u Is(u^2))
+struct SeriesCoef {
+    static constexpr double R[13] = {1.0,
+                                     -0.16666666666666666,
+                                     0.008333333333333333,
+                                     -0.0001984126984126984,
+                                     2.7557319223985893e-06,
+                                     -2.505210838544172e-08,
+                                     1.6059043836821613e-10,
+                                     -7.647163731819816e-13,
+                                     2.8114572543455206e-15,
+                                     -8.22063524662433e-18,
+                                     1.9572941063391263e-20,
+                                     -3.868170170630684e-23,
+                                     6.446950284384474e-26};
+    static constexpr double I[13] = {0.5,
+                                     -0.041666666666666664,
+                                     0.001388888888888889,
+                                     -2.48015873015873e-05,
+                                     2.755731922398589e-07,
+                                     -2.08767569878681e-09,
+                                     1.1470745597729725e-11,
+                                     -4.779477332387385e-14,
+                                     1.5619206968586225e-16,
+                                     -4.110317623312165e-19,
+                                     8.896791392450574e-22,
+                                     -1.6117375710961184e-24,
+                                     2.4795962632247976e-27};
+};
+
+template <int K, class R>
+__device__ __forceinline__ R horner_R(R v) {
+    R p = (R)SeriesCoef::R[K - 1];
+#pragma unroll
+    for (int k = K - 2; k >= 0; k--) p = p * v + (R)SeriesCoef::R[k];
+    return p;
+}
+
+template <int K, class R>
+__device__ __forceinline__ R horner_I(R v) {
+    R p = (R)SeriesCoef::I[K - 1];
+#pragma unroll
+    for (int k = K - 2; k >= 0; k--) p = p * v + (R)SeriesCoef::I[k];
+    return p;
+}
+
+// sin/cos of a small argument with K terms of each series.
+template <int K, class R>
+__device__ __forceinline__ void sincos_series(R x, R &s, R &c) {
+    const R x2 = x * x;
+    s = x * horner_R<K>(x2);
+    c = (R)1 - x2 * horner_I<K>(x2);
+}
+
+// Link transfer factor sinc(u/2) e^{-iu/2} = sin(u)/u - i (1 - cos u)/u
+// (gbgpu.py:518-525 with u = 2a), KR and KI terms.
+template <int KR, int KI, class R>
+__device__ __forceinline__ void transfer_series(R u, R &re, R &im) {
+    const R u2 = u * u;
+    re = horner_R<KR>(u2);
+    im = -u * horner_I<KI>(u2);
+}
+
+// Series lengths per frequency tier.  Tier t applies when
+// u_max = 2 max|f| / f* (light-time and Doppler margin included) is at most
+// U_MAX[t]; the first omitted term is then below 1e-17 (FP64) or 3e-9
+// (FP32) for the transfer factor (|u| <= U_MAX), omL = f/f* (<= U_MAX/2) and
+// the mixed path's spacecraft phase offsets (<= 0.29 U_MAX rad).  Tier 3
+// falls back to range-reduced sine/cosine.
+constexpr double TIER_UMAX[3] = {0.75, 1.5, 2.25};
+template <int TIER>
+struct TierF64 {
+    static constexpr int KR = TIER == 0 ? 9 : TIER == 1 ? 11 : 13;
+    static constexpr int KI = TIER == 0 ? 8 : TIER == 1 ? 10 : 12;
+    static constexpr int KS = TIER == 0 ? 7 : TIER == 1 ? 9 : 10;  // sincos of omL
+};
+template <int TIER>
+struct TierF32 {
+    static constexpr int KR = TIER == 0 ? 5 : TIER == 1 ? 7 : 8;
+    static constexpr int KI = TIER == 0 ? 5 : TIER == 1 ? 6 : 8;
+    static constexpr int KS = TIER == 0 ? 4 : TIER == 1 ? 5 : 6;  // sincos of omL
+    static constexpr int KD = TIER == 0 ? 4 : 5;                   // sincos of phase offsets
+};
 
 }  // namespace gbgpu

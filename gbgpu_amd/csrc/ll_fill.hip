@@ -16,6 +16,7 @@
 #include <hipcub/hipcub.hpp>
 
 #include "common.hpp"
+#include "fastmath.hpp"
 
 namespace gbgpu {
 
@@ -44,7 +45,7 @@ __global__ void __launch_bounds__(256) k_get_ll(double2 *__restrict__ d_h, doubl
             const long long t = (long long)i * B + b;
             const double2 hA = At[t], hE = Et[t];
             const double2 dA = Ad[doff + j], dE = Ed[doff + j];
-            const double iA = 1.0 / SA[noff + j], iE = 1.0 / SE[noff + j];
+            const double iA = rcp_f64(SA[noff + j]), iE = rcp_f64(SE[noff + j]);
             dhr += (dA.x * hA.x + dA.y * hA.y) * iA + (dE.x * hE.x + dE.y * hE.y) * iE;
             dhi += (dA.x * hA.y - dA.y * hA.x) * iA + (dE.x * hE.y - dE.y * hE.x) * iE;
             hh += (hA.x * hA.x + hA.y * hA.y) * iA + (hE.x * hE.x + hE.y * hE.y) * iE;

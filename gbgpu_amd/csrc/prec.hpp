@@ -38,7 +38,12 @@ struct F64 {
             v3d[1] = v3[1];
             v3d[2] = v3[2];
         }
-        slow_xyz_f64<THIRD>(s, g, fstar_inv, scale, v3d, t3, X, Y, Z);
+        switch (s.tier) {
+            case 0: slow_xyz_f64<THIRD, 0>(s, g, fstar_inv, scale, v3d, t3, X, Y, Z); break;
+            case 1: slow_xyz_f64<THIRD, 1>(s, g, fstar_inv, scale, v3d, t3, X, Y, Z); break;
+            case 2: slow_xyz_f64<THIRD, 2>(s, g, fstar_inv, scale, v3d, t3, X, Y, Z); break;
+            default: slow_xyz_f64<THIRD, 3>(s, g, fstar_inv, scale, v3d, t3, X, Y, Z); break;
+        }
     }
 
     static __device__ __forceinline__ double2 fac(const Src &s) { return make_double2(s.tdr, s.tdi); }
@@ -49,43 +54,85 @@ struct F64 {
 // FP32 copies of the per-binary constants used by the FP32 part.
 struct SrcF {
     float k[3], u[3], v[3];
-    float2 DP, DC;  // relative to amp
+    float2 DP, DC;  // 0.25 * (DP, DC) / amp
     float f0, fdot, fddot;
 };
 
-__device__ __forceinline__ float2 link_f32(float fonfs, float kr, float cphi, float sphi, float2 Aarm) {
-    const float a = 0.5f * fonfs * (1.0f + kr);
+// The same factor for any u, through sine/cosine with range reduction.
+__device__ __forceinline__ float2 transfer_general(float u) {
+    const float a = 0.5f * u;
     float sa, ca;
     sincos_f32(a, sa, ca);
-    const float sinc = (a != 0.0f) ? 0.25f * sa * __builtin_amdgcn_rcpf(a) : 0.25f;
-    const float er = ca * cphi - sa * sphi;
-    const float ei = -(sa * cphi + ca * sphi);
-    return cmul(make_float2(sinc * er, sinc * ei), Aarm);
+    const float sinc = (a != 0.0f) ? sa * __builtin_amdgcn_rcpf(a) : 1.0f;
+    return make_float2(sinc * ca, -sinc * sa);
 }
 
-template <bool THIRD>
+// Mixed-precision slow part.  The light-time delay of spacecraft s is split
+// as k.P_s/c = k.Pc/c + d_s: the guiding-centre delay k.Pc/c (~500 s) and the
+// centre phase Phi_c = om k.Pc/c - argS(t - k.Pc/c) are formed and reduced to
+// a quarter turn in FP64; the offsets d_s = k.dP_s/c (|d_s| < 5 s) and
+//   Phi_s - Phi_c = 2 pi f(xi_c) d_s - pi (fdot + fddot xi_c) d_s^2
+//                   + (pi/3) fddot d_s^3          (exact for the cubic argS)
+// are FP32.  The third-body variant keeps one FP64 phase per spacecraft
+// because its phase integral differs per spacecraft.
+template <bool THIRD, int TIER>
 __device__ __forceinline__ void slow_xyz_mixed(const Src &s, const SrcF &sf, const double *__restrict__ tab, int N,
-                                               int n, float fstar_inv, float scale, const float *v3,
+                                               int n, double fstar_inv_d, float scale, const float *v3,
                                                const double *term3, float2 &X, float2 &Y, float2 &Z) {
-    const double t = tab[n];
-    float r[9];
-#pragma unroll
-    for (int i = 0; i < 9; i++) r[i] = (float)tab[(10 + i) * N + n];
-    float fonfs[3], cphi[3], sphi[3];
+    const float fstar_inv = (float)fstar_inv_d;
     constexpr double INV_C = 1.0 / C_SI;
+    constexpr float INV_CF = (float)(1.0 / C_SI);
+    constexpr float PIF = 3.14159265358979323846f;
+    const double t = tab[ROW_T * N + n];
+    float r[9], dP[9];
 #pragma unroll
-    for (int sc = 0; sc < 3; sc++) {
-        const double P0 = tab[(1 + 3 * sc) * N + n], P1 = tab[(2 + 3 * sc) * N + n], P2 = tab[(3 + 3 * sc) * N + n];
-        const double kP = (s.k[0] * P0 + s.k[1] * P1 + s.k[2] * P2) * INV_C;
-        const double xi = t - kP;
-        const float xf = (float)xi;
-        float fi = sf.f0 + sf.fdot * xf + 0.5f * sf.fddot * xf * xf;
-        if (THIRD) fi *= 1.0f + v3[sc] * (float)(1.0 / C_SI);
-        fonfs[sc] = fi * fstar_inv;
-        double argS = s.phi0 + s.omq * t + PI * s.fdot * (xi * xi) + (1.0 / 3.0) * PI * s.fddot * (xi * xi * xi);
-        if (THIRD) argS += term3[sc];
-        const double Phi = s.om * kP - argS;
-        sincos_rev(Phi * INV_TWO_PI, sphi[sc], cphi[sc]);
+    for (int i = 0; i < 9; i++) r[i] = (float)tab[(ROW_R + i) * N + n];
+#pragma unroll
+    for (int i = 0; i < 9; i++) dP[i] = (float)tab[(ROW_DP + i) * N + n];
+    const double kPc =
+        (s.k[0] * tab[ROW_PC * N + n] + s.k[1] * tab[(ROW_PC + 1) * N + n] + s.k[2] * tab[(ROW_PC + 2) * N + n]) *
+        INV_C;
+    float d[3];
+#pragma unroll
+    for (int sc = 0; sc < 3; sc++)
+        d[sc] = (sf.k[0] * dP[3 * sc] + sf.k[1] * dP[3 * sc + 1] + sf.k[2] * dP[3 * sc + 2]) * INV_CF;
+    const double xic = t - kPc;
+    const float xcf = (float)xic;
+    const float fdc = sf.fdot + sf.fddot * xcf;               // f'(xi_c)
+    const float fc = sf.f0 + sf.fdot * xcf + 0.5f * sf.fddot * xcf * xcf;
+    float fonfs[3], cphi[3], sphi[3];
+    if constexpr (!THIRD) {
+        const double argSc = s.phi0 + s.omq * t + PI * s.fdot * (xic * xic) + (1.0 / 3.0) * PI * s.fddot * (xic * xic * xic);
+        const double Phic = s.om * kPc - argSc;
+        float sC, cC;
+        sincos_rev(Phic * INV_TWO_PI, sC, cC);
+        const float w1 = 2.0f * PIF * fc, w2 = PIF * fdc, w3 = (PIF / 3.0f) * sf.fddot;
+#pragma unroll
+        for (int sc = 0; sc < 3; sc++) {
+            const float dd = d[sc];
+            const float dphi = dd * (w1 - dd * (w2 - dd * w3));
+            float sd, cd;
+            if constexpr (TIER < 3)
+                sincos_series<TierF32<TIER>::KD>(dphi, sd, cd);
+            else
+                sincos_f32(dphi, sd, cd);
+            sphi[sc] = sC * cd + cC * sd;
+            cphi[sc] = cC * cd - sC * sd;
+            fonfs[sc] = (fc - dd * fdc) * fstar_inv;  // f(xi_c - d), dropping fddot d^2/2 (< 1e-22 Hz)
+        }
+    } else {
+#pragma unroll
+        for (int sc = 0; sc < 3; sc++) {
+            const double xi = xic - (double)d[sc];
+            const float xf = (float)xi;
+            float fi = sf.f0 + sf.fdot * xf + 0.5f * sf.fddot * xf * xf;
+            fi *= 1.0f + v3[sc] * INV_CF;
+            fonfs[sc] = fi * fstar_inv;
+            double argS = s.phi0 + s.omq * t + PI * s.fdot * (xi * xi) + (1.0 / 3.0) * PI * s.fddot * (xi * xi * xi);
+            argS += term3[sc];
+            const double Phi = s.om * (kPc + (double)d[sc]) - argS;
+            sincos_rev(Phi * INV_TWO_PI, sphi[sc], cphi[sc]);
+        }
     }
     const float kr12 = sf.k[0] * r[0] + sf.k[1] * r[1] + sf.k[2] * r[2];
     const float kr13 = sf.k[0] * r[3] + sf.k[1] * r[4] + sf.k[2] * r[5];
@@ -102,17 +149,39 @@ __device__ __forceinline__ void slow_xyz_mixed(const Src &s, const SrcF &sf, con
     const float2 A12 = make_float2(ep12 * sf.DP.x + ex12 * sf.DC.x, ep12 * sf.DP.y + ex12 * sf.DC.y);
     const float2 A23 = make_float2(ep23 * sf.DP.x + ex23 * sf.DC.x, ep23 * sf.DP.y + ex23 * sf.DC.y);
     const float2 A31 = make_float2(ep31 * sf.DP.x + ex31 * sf.DC.x, ep31 * sf.DP.y + ex31 * sf.DC.y);
-    const float2 G12 = link_f32(fonfs[0], kr12, cphi[0], sphi[0], A12);
-    const float2 G23 = link_f32(fonfs[1], kr23, cphi[1], sphi[1], A23);
-    const float2 G31 = link_f32(fonfs[2], -kr13, cphi[2], sphi[2], A31);
-    const float2 G21 = link_f32(fonfs[1], -kr12, cphi[1], sphi[1], A12);
-    const float2 G32 = link_f32(fonfs[2], -kr23, cphi[2], sphi[2], A23);
-    const float2 G13 = link_f32(fonfs[0], kr13, cphi[0], sphi[0], A31);
-    const float tf = (float)t;
-    const float f = sf.f0 + sf.fdot * tf + 0.5f * sf.fddot * tf * tf;
-    const float omL = f * fstar_inv;
-    float sL, cL;
-    sincos_f32(omL, sL, cL);
+    // e^{-i Phi_s} = (cphi, -sphi)
+    const float2 E0 = make_float2(cphi[0], -sphi[0]), E1 = make_float2(cphi[1], -sphi[1]),
+                 E2 = make_float2(cphi[2], -sphi[2]);
+    auto Tf = [](float u) {
+        if constexpr (TIER < 3) {
+            float re, im;
+            transfer_series<TierF32<TIER>::KR, TierF32<TIER>::KI>(u, re, im);
+            return make_float2(re, im);
+        } else {
+            return transfer_general(u);
+        }
+    };
+    // links (ij, arm, delayed spacecraft, k.r_ij)  (gbgpu.py:509-516)
+    const float2 G12 = cmul(Tf(fonfs[0] * (1.0f + kr12)), cmul(E0, A12));
+    const float2 G23 = cmul(Tf(fonfs[1] * (1.0f + kr23)), cmul(E1, A23));
+    const float2 G31 = cmul(Tf(fonfs[2] * (1.0f - kr13)), cmul(E2, A31));
+    const float2 G21 = cmul(Tf(fonfs[1] * (1.0f - kr12)), cmul(E1, A12));
+    const float2 G32 = cmul(Tf(fonfs[2] * (1.0f - kr23)), cmul(E2, A23));
+    const float2 G13 = cmul(Tf(fonfs[0] * (1.0f + kr13)), cmul(E0, A31));
+    float omL, sL, cL;
+    if constexpr (TIER < 3) {
+        const float tf = (float)t;
+        const float f = sf.f0 + sf.fdot * tf + 0.5f * sf.fddot * tf * tf;
+        omL = f * fstar_inv;
+        sincos_series<TierF32<TIER>::KS>(omL, sL, cL);
+    } else {
+        // omL can sit near a null of sin(omL) (f ~ pi f*), where an FP32
+        // argument would cost relative accuracy: reduce it in FP64.
+        const double fd = s.f0 + s.fdot * t + 0.5 * s.fddot * t * t;
+        const double omLd = fd * fstar_inv_d;
+        omL = (float)omLd;
+        sincos_rev(omLd * INV_TWO_PI, sL, cL);
+    }
     const float2 fctr = make_float2(cL, -sL);
     const float m = 4.0f * omL * sL * scale;
     const float2 fctr2 = make_float2(m * cL, -m * sL);
@@ -132,7 +201,7 @@ struct Mixed {
             f.u[i] = (float)s.u[i];
             f.v[i] = (float)s.v[i];
         }
-        const double ia = (s.amp != 0.0) ? 1.0 / s.amp : 0.0;
+        const double ia = (s.amp != 0.0) ? 0.25 / s.amp : 0.0;  // link prefactor 1/4 folded in
         f.DP = make_float2((float)(s.DPr * ia), (float)(s.DPi * ia));
         f.DC = make_float2((float)(s.DCr * ia), (float)(s.DCi * ia));
         f.f0 = (float)s.f0;
@@ -145,7 +214,12 @@ struct Mixed {
     static __device__ __forceinline__ void sample(const Src &s, const Aux &sf, const double *__restrict__ tab, int N,
                                                   int n, double fstar_inv, double scale, const float *v3,
                                                   const double *t3, CT &X, CT &Y, CT &Z) {
-        slow_xyz_mixed<THIRD>(s, sf, tab, N, n, (float)fstar_inv, (float)scale, v3, t3, X, Y, Z);
+        switch (s.tier) {
+            case 0: slow_xyz_mixed<THIRD, 0>(s, sf, tab, N, n, fstar_inv, (float)scale, v3, t3, X, Y, Z); break;
+            case 1: slow_xyz_mixed<THIRD, 1>(s, sf, tab, N, n, fstar_inv, (float)scale, v3, t3, X, Y, Z); break;
+            case 2: slow_xyz_mixed<THIRD, 2>(s, sf, tab, N, n, fstar_inv, (float)scale, v3, t3, X, Y, Z); break;
+            default: slow_xyz_mixed<THIRD, 3>(s, sf, tab, N, n, fstar_inv, (float)scale, v3, t3, X, Y, Z); break;
+        }
     }
 
     static __device__ __forceinline__ double2 fac(const Src &s) { return make_double2(s.tdr * s.amp, s.tdi * s.amp); }

@@ -6,6 +6,7 @@
 #pragma once
 
 #include "common.hpp"
+#include "fastmath.hpp"
 
 namespace gbgpu {
 
@@ -19,7 +20,10 @@ struct Src {
     double A2, varpi, e2, n2, T2;                 // third body (thirdbody.py:39-80)
     double tdr, tdi;                              // TDI2 factor, (1, 0) for TDI1
     long long start;                              // rint(f0 T) - N/2 (gbgpu.py:296, :387, :462)
+    int tier;   // series tier (fastmath.hpp TIER_UMAX) from u_max = 2 max_t |f(t)| / f*; 3 = none
 };
+
+__device__ __forceinline__ double fmax_(double a, double b) { return a > b ? a : b; }
 
 __device__ inline Src make_src(const double *__restrict__ params, int num_params, long long B, long long b,
                                int N, double T, int tdi2, double armlength) {
@@ -73,6 +77,20 @@ __device__ inline Src make_src(const double *__restrict__ params, int num_params
         s.T2 = params[13 * B + b];
     } else {
         s.A2 = s.varpi = s.e2 = s.n2 = s.T2 = 0.0;
+    }
+    {
+        // largest |f(t)| over [0, T]: ends and the stationary point of the quadratic
+        const double fT = f0 + fdot * T + 0.5 * fddot * T * T;
+        double fmax = fmax_(fabs(f0), fabs(fT));
+        if (fddot != 0.0) {
+            const double ts = -fdot / fddot;
+            if (ts > 0.0 && ts < T) fmax = fmax_(fmax, fabs(f0 + fdot * ts + 0.5 * fddot * ts * ts));
+        }
+        const double fstar = C_SI / (armlength * 2 * PI);
+        // 1e-3 margin covers the light-time offsets (|xi - t| < 600 s) and any
+        // third-body Doppler factor (|v|/c < 1e-3)
+        const double umax = 2.0 * fmax * 1.001 / fstar;
+        s.tier = umax <= TIER_UMAX[0] ? 0 : umax <= TIER_UMAX[1] ? 1 : umax <= TIER_UMAX[2] ? 2 : 3;
     }
     if (tdi2) {
         // 2i sin(2 wL) e^{-2 i wL}, wL = 2 pi f0 L / c   (gbgpu.py:301-304)
@@ -153,18 +171,30 @@ struct SampleGeom {
     double ur12, vr12, ur13, vr13, ur23, vr23;
 };
 
+// Orbit-table rows (see orbits.py: orbit_table): t, guiding centre Pc,
+// offsets dP_s = P_s - Pc, arm vectors r12, r13, r23.
+enum { ROW_T = 0, ROW_PC = 1, ROW_DP = 4, ROW_R = 13 };
+
+// k . P_s / c for the three spacecraft, as (k.Pc + k.dP_s)/c.
+__device__ __forceinline__ void load_kP(const Src &s, const double *__restrict__ tab, int N, int n, double kP[3]) {
+    constexpr double INV_C = 1.0 / C_SI;
+    const double kc =
+        s.k[0] * tab[ROW_PC * N + n] + s.k[1] * tab[(ROW_PC + 1) * N + n] + s.k[2] * tab[(ROW_PC + 2) * N + n];
+#pragma unroll
+    for (int sc = 0; sc < 3; sc++) {
+        const int r0 = ROW_DP + 3 * sc;
+        const double kd = s.k[0] * tab[r0 * N + n] + s.k[1] * tab[(r0 + 1) * N + n] + s.k[2] * tab[(r0 + 2) * N + n];
+        kP[sc] = (kc + kd) * INV_C;
+    }
+}
+
 __device__ __forceinline__ SampleGeom load_geom(const Src &s, const double *__restrict__ tab, int N, int n) {
     SampleGeom g;
-    g.t = tab[n];
-    double P[9], r[9];
+    g.t = tab[ROW_T * N + n];
+    double r[9];
 #pragma unroll
-    for (int i = 0; i < 9; i++) P[i] = tab[(1 + i) * N + n];
-#pragma unroll
-    for (int i = 0; i < 9; i++) r[i] = tab[(10 + i) * N + n];
-    constexpr double INV_C = 1.0 / C_SI;
-#pragma unroll
-    for (int sc = 0; sc < 3; sc++)
-        g.kP[sc] = (s.k[0] * P[3 * sc] + s.k[1] * P[3 * sc + 1] + s.k[2] * P[3 * sc + 2]) * INV_C;
+    for (int i = 0; i < 9; i++) r[i] = tab[(ROW_R + i) * N + n];
+    load_kP(s, tab, N, n, g.kP);
     g.kr12 = s.k[0] * r[0] + s.k[1] * r[1] + s.k[2] * r[2];
     g.kr13 = s.k[0] * r[3] + s.k[1] * r[4] + s.k[2] * r[5];
     g.kr23 = s.k[0] * r[6] + s.k[1] * r[7] + s.k[2] * r[8];
@@ -179,15 +209,23 @@ __device__ __forceinline__ SampleGeom load_geom(const Src &s, const double *__re
 
 // One transfer-function link: 0.25 sin(a)/a e^{-i(a + Phi_s)} A_arm
 // (gbgpu.py:508-525), with e^{-i Phi_s} = (cphi, -sphi) precomputed.
+// sinc(a) e^{-ia} comes from the reduction-free series in u = 2a when the
+// binary's tier allows, else from a reduced sine/cosine.
+template <int TIER>
 __device__ __forceinline__ double2 link_f64(double fonfs, double kr, double cphi, double sphi, double2 Aarm) {
-    const double a = 0.5 * fonfs * (1.0 + kr);
-    double sa, ca;
-    sincos(a, &sa, &ca);
-    const double sinc = (a != 0.0) ? 0.25 * sa / a : 0.25;
-    // e^{-ia} e^{-iPhi} = (ca - i sa)(cphi - i sphi)
-    const double er = ca * cphi - sa * sphi;
-    const double ei = -(sa * cphi + ca * sphi);
-    const double2 e = make_double2(sinc * er, sinc * ei);
+    double tr, ti;
+    if constexpr (TIER < 3) {
+        transfer_series<TierF64<TIER>::KR, TierF64<TIER>::KI>(fonfs * (1.0 + kr), tr, ti);
+    } else {
+        const double a = 0.5 * fonfs * (1.0 + kr);
+        double sa, ca;
+        sincos_f64(a, sa, ca);
+        const double sinc = (a != 0.0) ? sa * rcp_f64(a) : 1.0;
+        tr = sinc * ca;
+        ti = -sinc * sa;
+    }
+    // 0.25 (tr + i ti)(cphi - i sphi)
+    const double2 e = make_double2(0.25 * (tr * cphi + ti * sphi), 0.25 * (ti * cphi - tr * sphi));
     return cmul(e, Aarm);
 }
 
@@ -195,7 +233,7 @@ __device__ __forceinline__ double2 link_f64(double fonfs, double kr, double cphi
 // fctr2 * amp * 0.5 T / N (gbgpu.py:359-375, :298; amp / amp cancels).
 // v3[sc] (third-body v_LOS at xi_sc) and term3[sc] (its phase integral) are
 // used only when THIRD.
-template <bool THIRD>
+template <bool THIRD, int TIER>
 __device__ __forceinline__ void slow_xyz_f64(const Src &s, const SampleGeom &g, double fstar_inv, double scale,
                                              const double *v3, const double *term3, double2 &X, double2 &Y,
                                              double2 &Z) {
@@ -209,7 +247,7 @@ __device__ __forceinline__ void slow_xyz_f64(const Src &s, const SampleGeom &g, 
         double argS = s.phi0 + s.omq * g.t + PI * s.fdot * (xi * xi) + (1.0 / 3.0) * PI * s.fddot * (xi * xi * xi);
         if (THIRD) argS += term3[sc];
         const double Phi = s.om * g.kP[sc] - argS;
-        sincos(Phi, &sphi[sc], &cphi[sc]);
+        sincos_f64(Phi, sphi[sc], cphi[sc]);
     }
     // arm antenna factors A_ij = (r e+ r) DP + (r ex r) DC  (gbgpu.py:467-473)
     // with r e+ r = (v.r)^2 - (u.r)^2 and r ex r = 2 (u.r)(v.r); r31 = -r13.
@@ -220,17 +258,20 @@ __device__ __forceinline__ void slow_xyz_f64(const Src &s, const SampleGeom &g, 
     const double2 A23 = make_double2(ep23 * s.DPr + ex23 * s.DCr, ep23 * s.DPi + ex23 * s.DCi);
     const double2 A31 = make_double2(ep31 * s.DPr + ex31 * s.DCr, ep31 * s.DPi + ex31 * s.DCi);
     // links (ij, arm, spacecraft of the delay, k.r_ij)  (gbgpu.py:509-516)
-    const double2 G12 = link_f64(fonfs[0], g.kr12, cphi[0], sphi[0], A12);
-    const double2 G23 = link_f64(fonfs[1], g.kr23, cphi[1], sphi[1], A23);
-    const double2 G31 = link_f64(fonfs[2], -g.kr13, cphi[2], sphi[2], A31);
-    const double2 G21 = link_f64(fonfs[1], -g.kr12, cphi[1], sphi[1], A12);
-    const double2 G32 = link_f64(fonfs[2], -g.kr23, cphi[2], sphi[2], A23);
-    const double2 G13 = link_f64(fonfs[0], g.kr13, cphi[0], sphi[0], A31);
+    const double2 G12 = link_f64<TIER>(fonfs[0], g.kr12, cphi[0], sphi[0], A12);
+    const double2 G23 = link_f64<TIER>(fonfs[1], g.kr23, cphi[1], sphi[1], A23);
+    const double2 G31 = link_f64<TIER>(fonfs[2], -g.kr13, cphi[2], sphi[2], A31);
+    const double2 G21 = link_f64<TIER>(fonfs[1], -g.kr12, cphi[1], sphi[1], A12);
+    const double2 G32 = link_f64<TIER>(fonfs[2], -g.kr23, cphi[2], sphi[2], A23);
+    const double2 G13 = link_f64<TIER>(fonfs[0], g.kr13, cphi[0], sphi[0], A31);
     // TDI transfer (gbgpu.py:352-375)
     const double f = s.f0 + s.fdot * g.t + 0.5 * s.fddot * g.t * g.t;
     const double omL = f * fstar_inv;
     double sL, cL;
-    sincos(omL, &sL, &cL);
+    if constexpr (TIER < 3)
+        sincos_series<TierF64<TIER>::KS>(omL, sL, cL);
+    else
+        sincos_f64(omL, sL, cL);
     const double2 fctr = make_double2(cL, -sL);
     const double m = 4.0 * omL * sL * scale;
     const double2 fctr2 = make_double2(m * cL, -m * sL);

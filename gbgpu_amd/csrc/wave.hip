@@ -8,6 +8,8 @@
 // fftshifted, scaled spectra to HBM (run_wave) or reads the binary's
 // data/PSD window with coalesced loads and reduces <d|h>, <h|h> over the
 // group (wave_ll).
+#include <hipcub/hipcub.hpp>
+
 #include "common.hpp"
 #include "wave_common.hpp"
 
@@ -31,8 +33,7 @@ __global__ void __launch_bounds__(WG_SIZE) k_wave(WaveArgs a) {
     constexpr bool XI_IN_BUF = NCH * sizeof(CT) * N >= 3 * sizeof(double) * N;
     constexpr int NXI = (THIRD && !XI_IN_BUF) ? N : 1;
     __shared__ CT sbuf[BPW * NCH * N];
-    __shared__ Src ssrc[BPW];
-    __shared__ typename P::Aux saux[BPW];
+    __shared__ Pack<P> spack[BPW];
     __shared__ double stot[BPW * WPB];
     __shared__ float s_v[BPW * 3 * NT];
     __shared__ double s_g[BPW * 3 * NT];
@@ -41,22 +42,19 @@ __global__ void __launch_bounds__(WG_SIZE) k_wave(WaveArgs a) {
     const int tid = threadIdx.x;
     const int slot = tid / TPB;
     const int lane = tid % TPB;
-    const long long b = (long long)blockIdx.x * BPW + slot;
-    const bool valid = b < a.B;
+    const long long item = (long long)blockIdx.x * BPW + slot;
+    const bool valid = item < a.B;
+    const long long b = (valid && a.order) ? (long long)a.order[item] : item;
 
-    if (lane == 0 && valid) {
-        const Src s0 = make_src(a.params, a.num_params, a.B, b, N, a.T, a.tdi2, a.armlength);
-        ssrc[slot] = s0;
-        saux[slot] = P::make_aux(s0);
-    }
+    if (valid) load_pack<P, TPB>(a, b, lane, spack + slot);
     __syncthreads();
-    const Src s = ssrc[slot];
+    const Src s = spack[slot].s;
     CT *buf = sbuf + slot * NCH * N;
     double *tot = stot + slot * WPB;
     ThirdScratch ts{XI_IN_BUF ? reinterpret_cast<double *>(buf) : s_xi + slot * 3 * NXI, s_v + slot * 3 * NT,
                     s_g + slot * 3 * NT};
 
-    binary_spectrum<N, NCH, THIRD, P>(s, saux[slot], valid, lane, a.tab, a.fstar_inv, a.scale, buf,
+    binary_spectrum<N, NCH, THIRD, P>(s, spack[slot].aux, valid, lane, a.tab, a.fstar_inv, a.scale, buf,
                                       static_cast<const CT *>(a.tw), ts, tot);
 
     const double2 fac = P::fac(s);
@@ -106,19 +104,79 @@ __global__ void __launch_bounds__(WG_SIZE) k_wave(WaveArgs a) {
 }
 
 // --------------------------------------------------------------- dispatch ---
+// Per-binary set-up into a stream-ordered workspace, then the waveform
+// kernel, then the workspace is released on the same stream.
+// Batches of at least this many binaries are processed in start-bin order.
+constexpr long long SORT_MIN_BATCH = 8192;
+
+template <class P, class Launch>
+static int with_pack(WaveArgs a, int N, hipStream_t st, Launch launch) {
+    const size_t B = (size_t)a.B;
+    const bool sort = a.B >= SORT_MIN_BATCH;
+    size_t sort_bytes = 0;
+    if (sort) {
+        hipError_t e = hipcub::DeviceRadixSort::SortPairs(nullptr, sort_bytes, (unsigned *)nullptr,
+                                                          (unsigned *)nullptr, (int *)nullptr, (int *)nullptr,
+                                                          (int)a.B, 0, 32, st);
+        if (e != hipSuccess) return hip_fail(e, "SortPairs(size)");
+    }
+    auto al = [](size_t n) { return (n + 255) & ~(size_t)255; };
+    const size_t bytes = al(sizeof(Pack<P>) * B) + (sort ? 2 * al(4 * B) + 2 * al(4 * B) + al(sort_bytes) : 0);
+    char *ws = nullptr;
+    hipError_t e = hipMallocAsync((void **)&ws, bytes, st);
+    if (e != hipSuccess) return hip_fail(e, "hipMallocAsync(per-binary workspace)");
+    size_t off = 0;
+    auto carve = [&](size_t n) {
+        char *p = ws + off;
+        off += al(n);
+        return p;
+    };
+    Pack<P> *pk = (Pack<P> *)carve(sizeof(Pack<P>) * B);
+    unsigned *keys = nullptr, *keys_out = nullptr;
+    int *vals = nullptr, *order = nullptr;
+    void *tmp = nullptr;
+    if (sort) {
+        keys = (unsigned *)carve(4 * B);
+        keys_out = (unsigned *)carve(4 * B);
+        vals = (int *)carve(4 * B);
+        order = (int *)carve(4 * B);
+        tmp = carve(sort_bytes);
+    }
+    a.keys = keys;
+    a.vals = vals;
+    hipLaunchKernelGGL((k_setup<P>), dim3((unsigned)((a.B + 255) / 256)), dim3(256), 0, st, a, N, pk);
+    int rc = GBGPU_OK;
+    if (sort) {
+        e = hipcub::DeviceRadixSort::SortPairs(tmp, sort_bytes, keys, keys_out, vals, order, (int)a.B, 0, 32, st);
+        if (e != hipSuccess) rc = hip_fail(e, "SortPairs");
+    }
+    a.keys = nullptr;
+    a.vals = nullptr;
+    a.pack = pk;
+    a.order = order;
+    if (!rc) rc = launch(a);
+    e = hipFreeAsync(ws, st);
+    if (rc) return rc;
+    if (e != hipSuccess) return hip_fail(e, "hipFreeAsync(per-binary workspace)");
+    return GBGPU_OK;
+}
+
 template <int MODE, bool THIRD, class P>
 static int launch_wave(int N, const WaveArgs &a, hipStream_t st) {
 #define GBGPU_CASE(NN)                                                                              \
     case NN: {                                                                                      \
         if constexpr (fits_lds<NN, MODE, THIRD, P>()) {                                             \
             const long long grid = (a.B + Geo<NN>::BPW - 1) / Geo<NN>::BPW;                         \
-            hipLaunchKernelGGL((k_wave<NN, MODE, THIRD, P>), dim3((unsigned)grid), dim3(WG_SIZE), 0, \
-                               st, a);                                                              \
+            return with_pack<P>(a, NN, st, [&](const WaveArgs &aa) -> int {                         \
+                hipLaunchKernelGGL((k_wave<NN, MODE, THIRD, P>), dim3((unsigned)grid), dim3(WG_SIZE), \
+                                   0, st, aa);                                                      \
+                GBGPU_CHECK_LAUNCH("k_wave");                                                       \
+                return GBGPU_OK;                                                                    \
+            });                                                                                     \
         } else {                                                                                    \
             set_error("FP64 third-body run_wave supports N <= 1024 (LDS limit)");                   \
             return GBGPU_E_ARG;                                                                     \
         }                                                                                           \
-        break;                                                                                      \
     }
     switch (N) {
         GBGPU_CASE(32)
@@ -133,8 +191,6 @@ static int launch_wave(int N, const WaveArgs &a, hipStream_t st) {
             return GBGPU_E_ARG;
     }
 #undef GBGPU_CASE
-    GBGPU_CHECK_LAUNCH("k_wave");
-    return GBGPU_OK;
 }
 
 template <int MODE>

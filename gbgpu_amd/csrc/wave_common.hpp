@@ -46,7 +46,48 @@ struct WaveArgs {
     int phase_marg;
     double *loglike;
     double2 *d_h, *h_h;
+    const void *pack;  // per-binary constants written by k_setup (Pack<P>[B])
+    const int *order;  // processing order (binaries sorted by start bin) or null
+    unsigned *keys;    // k_setup output: start-bin sort keys, or null
+    int *vals;         // k_setup output: binary indices, or null
 };
+
+// Per-binary constants for one precision policy, computed once per binary
+// by k_setup (64 binaries per wavefront) and copied into LDS by the
+// waveform kernels.
+template <class P>
+struct Pack {
+    Src s;
+    typename P::Aux aux;
+};
+
+template <class P>
+__global__ void __launch_bounds__(256) k_setup(WaveArgs a, int N, Pack<P> *__restrict__ out) {
+    const long long b = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (b >= a.B) return;
+    Pack<P> pk;
+    pk.s = make_src(a.params, a.num_params, a.B, b, N, a.T, a.tdi2, a.armlength);
+    pk.aux = P::make_aux(pk.s);
+    out[b] = pk;
+    if (a.keys) {
+        // order by start bin: neighbouring workgroups then read overlapping
+        // data/PSD windows, which stay resident in L2
+        long long k = pk.s.start + (1LL << 31);
+        k = k < 0 ? 0 : (k > 0xffffffffLL ? 0xffffffffLL : k);
+        a.keys[b] = (unsigned)k;
+        a.vals[b] = (int)b;
+    }
+}
+
+// Cooperative copy of binary b's Pack into LDS by the TPB lanes owning it.
+template <class P, int TPB>
+__device__ __forceinline__ void load_pack(const WaveArgs &a, long long b, int lane, Pack<P> *dst) {
+    static_assert(sizeof(Pack<P>) % 4 == 0, "Pack must be word-sized");
+    constexpr int W = sizeof(Pack<P>) / 4;
+    const unsigned *src = reinterpret_cast<const unsigned *>(static_cast<const Pack<P> *>(a.pack) + b);
+    unsigned *d = reinterpret_cast<unsigned *>(dst);
+    for (int i = lane; i < W; i += TPB) d[i] = src[i];
+}
 
 // Exclusive prefix sum over the TPB lanes owning one binary.  `tot` is LDS
 // scratch of WPB doubles for this binary.  Uniform control flow required.
@@ -113,15 +154,14 @@ __device__ __forceinline__ void binary_spectrum(const Src &s, const typename P::
 #pragma unroll
         for (int q = 0; q < SPT; q++) {
             const int n = lane + q * TPB;
+            double kP[3] = {0.0, 0.0, 0.0};
+            if (valid) load_kP(s, tab, N, n, kP);
 #pragma unroll
             for (int sc = 0; sc < 3; sc++) {
                 double xi = 0.0, gg = 0.0;
                 float v = 0.0f;
                 if (valid) {
-                    const double t = tab[n];
-                    const double P0 = tab[(1 + 3 * sc) * N + n], P1 = tab[(2 + 3 * sc) * N + n],
-                                 P2 = tab[(3 + 3 * sc) * N + n];
-                    xi = t - (s.k[0] * P0 + s.k[1] * P1 + s.k[2] * P2) * (1.0 / C_SI);
+                    xi = tab[ROW_T * N + n] - kP[sc];
                     const double vd = third_vlos(s, xi);
                     v = (float)vd;
                     gg = vd * (s.f0 + s.fdot * xi + 0.5 * s.fddot * xi * xi);
@@ -203,7 +243,7 @@ __device__ __forceinline__ void ll_partial(const typename P::CT *buf, double2 fa
         if (j < 0 || j >= a.L) continue;
         const double2 hA = cmul(P::up(buf[idx]), fac), hE = cmul(P::up(buf[N + idx]), fac);
         const double2 dA = a.dA[doff + j], dE = a.dE[doff + j];
-        const double iA = 1.0 / a.SA[noff + j], iE = 1.0 / a.SE[noff + j];
+        const double iA = rcp_f64(a.SA[noff + j]), iE = rcp_f64(a.SE[noff + j]);
         // conj(d) h / S
         dhr += (dA.x * hA.x + dA.y * hA.y) * iA + (dE.x * hE.x + dE.y * hE.y) * iE;
         dhi += (dA.x * hA.y - dA.y * hA.x) * iA + (dE.x * hE.y - dE.y * hE.x) * iE;

