@@ -77,6 +77,10 @@ SIGNATURES = {
         [c_void_p, c_void_p, c_void_p, c_void_p, c_int, c_int, ctypes.POINTER(WaveCfg), ctypes.POINTER(Data),
          c_double, c_int, c_void_p],
     ),
+    "gbgpu_swap_ll_diff": (
+        c_int,
+        [c_void_p, c_void_p, c_void_p, c_int, c_int, ctypes.POINTER(WaveCfg), ctypes.POINTER(Data), c_void_p],
+    ),
     "gbgpu_last_error": (ctypes.c_char_p, []),
     "gbgpu_abi_version": (c_int, []),
 }

@@ -361,6 +361,64 @@ class GBGPUBase(abc.ABC):
         self.d_h = d_h
         return ll.cpu().numpy()
 
+    def swap_likelihood_difference(self, params_remove, params_add, data, psd, phase_marginalize=False,
+                                   start_freq_ind=0, data_index=None, noise_index=None, return_terms=False,
+                                   **kwargs):
+        """Log-likelihood change of replacing binaries by proposals.
+
+        For each proposal p, ``data`` holds the residual r (data minus the
+        current global template, which includes binary ``params_remove[:, p]``)
+        and the result is ``logL(r + h_remove - h_add) - logL(r)``, i.e.
+        Re<r|h_a> - Re<r|h_r> - (<h_a|h_a> + <h_r|h_r> - 2 Re<h_r|h_a>)/2 with
+        the reference's inner product (src/gbgpu/cutils/gbgpu_utils.cu:156-183).
+        The reference has no such method; arguments mirror ``get_ll``
+        (src/gbgpu/gbgpu.py:580-735): params are (num_params, P), ``data`` and
+        ``psd`` are [A, E] (1-D, or 2-D with per-proposal data_index /
+        noise_index), ``start_freq_ind`` as in get_ll.  Remaining kwargs go to
+        the waveform (N, T, dt, tdi2, precision).  Returns (P,) float64, plus
+        (P, 5) terms when ``return_terms``.
+        """
+        if phase_marginalize:
+            raise NotImplementedError("phase marginalisation is not defined for a swap difference")
+        torch = _torch()
+        self._check_hooks()
+        tdi2 = kwargs.pop("tdi2", False)
+        precision = kwargs.pop("precision", None)
+        N = kwargs.pop("N", None)
+        T = kwargs.pop("T", 4 * YRSID_SI)
+        dt = kwargs.pop("dt", 10.0)
+        oversample = kwargs.pop("oversample", 1)
+        if kwargs:
+            raise TypeError(f"unexpected keyword arguments {list(kwargs)}")
+        pr = np.atleast_2d(np.asarray(params_remove, dtype=np.float64))
+        pa = np.atleast_2d(np.asarray(params_add, dtype=np.float64))
+        if pr.shape != pa.shape:
+            raise ValueError("params_remove and params_add must have the same shape")
+        self._check_data(data, data_index, noise_index)
+        P = pr.shape[1]
+        pairs = np.empty((pr.shape[0], 2 * P))
+        pairs[:, 0::2] = pr
+        pairs[:, 1::2] = pa
+        if N is None:
+            N_r = self.special_get_N(pr[0], pr[1], int(T / dt) * dt, *pr[9:], oversample=oversample)
+            N_a = self.special_get_N(pa[0], pa[1], int(T / dt) * dt, *pa[9:], oversample=oversample)
+            N = int(max(np.max(N_r), np.max(N_a)))
+        p, N, T = self._prepare(*pairs[:9], pairs[9:], N, T, dt, oversample)
+        dA, dE, SA, SE, di, ni, L = self._prep_data(data, psd, data_index, noise_index, P)
+        cfg, _tab = self._cfg(N, T, tdi2, precision)
+        dev = p.device
+        dll = torch.empty(P, dtype=torch.float64, device=dev)
+        terms = torch.empty((P, 5), dtype=torch.float64, device=dev) if return_terms else None
+        dat = _capi.Data(A=dA.data_ptr(), E=dE.data_ptr(), psdA=SA.data_ptr(), psdE=SE.data_ptr(),
+                         data_index=di.data_ptr(), noise_index=ni.data_ptr(), data_length=int(L),
+                         start_freq_ind=int(start_freq_ind), df=float(1 / T))
+        rc = _capi.load().gbgpu_swap_ll_diff(dll.data_ptr(), _capi.ptr(terms), p.data_ptr(), p.shape[0], P, cfg, dat,
+                                             _capi.stream_handle())
+        _capi.check(rc, "gbgpu_swap_ll_diff")
+        if return_terms:
+            return dll.cpu().numpy(), terms.cpu().numpy()
+        return dll.cpu().numpy()
+
     # -------------------------------------------------- global templates ---
     def fill_global_template(self, group_index, templates, A, E, start_inds, N=None, start_freq_ind=0):
         """Sum binaries into per-group A/E arrays (src/gbgpu/gbgpu.py:737-837).

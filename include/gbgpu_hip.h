@@ -115,6 +115,18 @@ int gbgpu_wave_ll(double *loglike, gbgpu_cmplx *d_h, gbgpu_cmplx *h_h, const dou
                   int num_params, int num_bin, const gbgpu_wave_cfg *cfg, const gbgpu_data *data,
                   double d_d, int phase_marginalize, void *stream);
 
+/* swap_likelihood_difference (no native counterpart in the reference; its
+ * likelihood algebra, src/gbgpu/gbgpu.py:728 with the inner product of
+ * src/gbgpu/cutils/gbgpu_utils.cu:156-183): for proposal p, remove binary
+ * h_r and add h_a against the residual r held in `data`:
+ *   dll[p] = Re<r|h_a> - Re<r|h_r> - (<h_a|h_a> + <h_r|h_r> - 2 Re<h_r|h_a>)/2.
+ * params_pairs is (num_params, 2*num_prop) with column 2p the removed and
+ * column 2p+1 the added binary (same N).  terms (num_prop, 5) may be NULL;
+ * else it receives Re<r|h_r>, Re<r|h_a>, <h_r|h_r>, <h_a|h_a>, Re<h_r|h_a>.
+ * data->data_index / noise_index are per proposal. */
+int gbgpu_swap_ll_diff(double *dll, double *terms, const double *params_pairs, int num_params, int num_prop,
+                       const gbgpu_wave_cfg *cfg, const gbgpu_data *data, void *stream);
+
 /* ---- library -------------------------------------------------------- */
 
 /* Message describing the last failure on the calling thread. */
