@@ -54,6 +54,18 @@ class Data(ctypes.Structure):
     ]
 
 
+class BatchCfg(ctypes.Structure):
+    """``gbgpu_batch_cfg``."""
+
+    _fields_ = [
+        ("T", c_double),
+        ("armlength", c_double),
+        ("tdi2", c_int),
+        ("precision", c_int),
+        ("orbit_tables", c_void_p * 7),
+    ]
+
+
 # name -> (restype, argtypes) for every symbol declared in include/gbgpu_hip.h
 SIGNATURES = {
     "gbgpu_get_ll": (
@@ -80,6 +92,17 @@ SIGNATURES = {
     "gbgpu_swap_ll_diff": (
         c_int,
         [c_void_p, c_void_p, c_void_p, c_int, c_int, ctypes.POINTER(WaveCfg), ctypes.POINTER(Data), c_void_p],
+    ),
+    "gbgpu_batch_workspace_bytes": (c_longlong, [c_int, c_int]),
+    "gbgpu_wave_ll_batched": (
+        c_int,
+        [c_void_p, c_void_p, c_void_p, c_void_p, c_int, c_int, c_void_p, ctypes.POINTER(BatchCfg),
+         ctypes.POINTER(Data), c_double, c_int, c_void_p, c_void_p],
+    ),
+    "gbgpu_perturb_params": (
+        c_int,
+        [c_void_p, c_void_p, c_int, c_int, c_double, ctypes.c_ulonglong, ctypes.c_ulonglong, c_void_p, c_int,
+         c_void_p, c_void_p, c_void_p],
     ),
     "gbgpu_last_error": (ctypes.c_char_p, []),
     "gbgpu_abi_version": (c_int, []),

@@ -1,0 +1,166 @@
+"""Uneven-N likelihood batches and their graph-captured sampler loop.
+
+``BatchedLogLike`` keeps the data, PSD, per-N orbit tables and the library
+workspace resident on the GPU and evaluates the fused waveform + likelihood
+for a batch in which every binary has its own N (one persistent launch,
+``gbgpu_wave_ll_batched``).  ``run_loop`` evaluates a sequence of calls whose
+parameters are seeded Gaussian relative jumps of a base point
+(``gbgpu_perturb_params``; N is re-derived from the jumped f0 by the step
+rule), either eagerly or as one captured hipGraph (``capture_loop``).
+"""
+
+from __future__ import annotations
+
+import numpy as np
+
+from . import _capi
+
+DEFAULT_F_EDGES = (1e-3, 3e-3, 6e-3, 10e-3)
+DEFAULT_N_LEVELS = (128, 256, 512, 1024, 2048)
+
+
+def _torch():
+    import torch
+
+    return torch
+
+
+class BatchedLogLike:
+    """Resident uneven-N log-likelihood evaluator.
+
+    Args:
+        gb: a ``GBGPU`` instance (supplies orbits, d_d and the device).
+        data, psd: [A, E] arrays (1-D), numpy or torch.
+        T, dt: observation time and cadence (T is reduced to int(T/dt)*dt).
+        precision: "fp64" or "mixed".
+        N_levels, f_edges: the N step rule used when N is derived from f0:
+            N_levels[i] for f0 < f_edges[i], the last level above.
+    """
+
+    def __init__(self, gb, data, psd, T, dt, precision="mixed", start_freq_ind=0, N_levels=DEFAULT_N_LEVELS,
+                 f_edges=DEFAULT_F_EDGES, phase_marginalize=False):
+        torch = _torch()
+        if gb.d_d is None:
+            raise ValueError("gb.d_d must be set before building a likelihood")
+        self.gb = gb
+        self.T = int(T / dt) * dt
+        self.precision = precision
+        self.N_levels = np.asarray(N_levels, dtype=np.int32)
+        self.f_edges = np.asarray(f_edges, dtype=np.float64)
+        if len(self.N_levels) != len(self.f_edges) + 1:
+            raise ValueError("need len(N_levels) == len(f_edges) + 1")
+        self.phase_marginalize = bool(phase_marginalize)
+        dev = gb._device()
+        self.A = gb._to_dev(data[0], np.complex128).reshape(-1)
+        self.E = gb._to_dev(data[1], np.complex128).reshape(-1)
+        self.SA = gb._to_dev(psd[0], np.float64).reshape(-1)
+        self.SE = gb._to_dev(psd[1], np.float64).reshape(-1)
+        self.L = int(self.A.numel())
+        self.cfg = _capi.BatchCfg(T=float(self.T), armlength=float(gb.orbits.armlength), tdi2=0,
+                                  precision=1 if precision == "mixed" else 0)
+        self._tabs = {}
+        for N in self.N_levels:
+            self._add_table(int(N))
+        self.data = _capi.Data(A=self.A.data_ptr(), E=self.E.data_ptr(), psdA=self.SA.data_ptr(),
+                               psdE=self.SE.data_ptr(), data_index=None, noise_index=None, data_length=self.L,
+                               start_freq_ind=int(start_freq_ind), df=float(1.0 / self.T))
+        self.dev = dev
+        self._ws = None
+        self._graph = None
+
+    def _add_table(self, N):
+        k = int(np.log2(N)) - 5
+        if N not in self._tabs:
+            if N & (N - 1) or not 32 <= N <= 2048:
+                raise ValueError(f"N={N}: powers of two in [32, 2048] only")
+            self._tabs[N] = self.gb._table(self.T, N)
+            self.cfg.orbit_tables[k] = self._tabs[N].data_ptr()
+
+    def _workspace(self, B):
+        torch = _torch()
+        need = _capi.load().gbgpu_batch_workspace_bytes(B, self.cfg.precision)
+        if self._ws is None or self._ws.numel() < need:
+            self._ws = torch.empty(int(need), dtype=torch.uint8, device=self.dev)
+        return self._ws
+
+    def N_of(self, f0):
+        """N from f0 by the step rule."""
+        return self.N_levels[np.searchsorted(self.f_edges, np.abs(np.asarray(f0)), side="right")]
+
+    def __call__(self, params, N=None, d_h=False):
+        """log-likelihood of (9, B) params with per-binary N (default: from f0)."""
+        torch = _torch()
+        p = params if torch.is_tensor(params) else torch.from_numpy(np.ascontiguousarray(params, np.float64))
+        p = p.to(self.dev).contiguous()
+        B = p.shape[1]
+        if N is None:
+            N = self.N_of(p[1].cpu().numpy())
+        Nn = np.asarray(N.cpu().numpy() if torch.is_tensor(N) else N, dtype=np.int32)
+        for n in np.unique(Nn):
+            self._add_table(int(n))
+        Nt = torch.from_numpy(Nn).to(self.dev) if not torch.is_tensor(N) else N.to(self.dev, torch.int32)
+        ll = torch.empty(B, dtype=torch.float64, device=self.dev)
+        dh = torch.empty(B, dtype=torch.complex128, device=self.dev) if d_h else None
+        hh = torch.empty(B, dtype=torch.complex128, device=self.dev) if d_h else None
+        self._launch(ll, dh, hh, p, Nt, B)
+        return (ll, dh, hh) if d_h else ll
+
+    def _launch(self, ll, dh, hh, p, Nt, B):
+        ws = self._workspace(B)
+        rc = _capi.load().gbgpu_wave_ll_batched(ll.data_ptr(), _capi.ptr(dh), _capi.ptr(hh), p.data_ptr(), 9, B,
+                                                Nt.data_ptr(), self.cfg, self.data, float(np.real(self.gb.d_d)),
+                                                int(self.phase_marginalize), ws.data_ptr(), _capi.stream_handle())
+        _capi.check(rc, "gbgpu_wave_ll_batched")
+
+    # ------------------------------------------------------ sampler loop ---
+    def _loop_buffers(self, params, n_calls):
+        torch = _torch()
+        base = params if torch.is_tensor(params) else torch.from_numpy(np.ascontiguousarray(params, np.float64))
+        base = base.to(self.dev).contiguous()
+        B = base.shape[1]
+        out = torch.empty((n_calls, B), dtype=torch.float64, device=self.dev)
+        pj = torch.empty_like(base)
+        Nj = torch.empty(B, dtype=torch.int32, device=self.dev)
+        edges = np.ascontiguousarray(self.f_edges)
+        levels = np.ascontiguousarray(self.N_levels)
+        return base, out, pj, Nj, edges, levels
+
+    def _loop_body(self, base, out, pj, Nj, edges, levels, n_calls, sigma, seed):
+        lib = _capi.load()
+        B = base.shape[1]
+        st = _capi.stream_handle()
+        for i in range(n_calls):
+            rc = lib.gbgpu_perturb_params(pj.data_ptr(), base.data_ptr(), base.shape[0], B, float(sigma), int(seed),
+                                          int(i), Nj.data_ptr(), len(edges), edges.ctypes.data, levels.ctypes.data,
+                                          st)
+            _capi.check(rc, "gbgpu_perturb_params")
+            self._launch(out[i], None, None, pj, Nj, B)
+
+    def run_loop(self, params, n_calls, sigma=1e-3, seed=0):
+        """n_calls evaluations at seeded jumps of ``params``; (n_calls, B) tensor."""
+        bufs = self._loop_buffers(params, n_calls)
+        self._workspace(bufs[0].shape[1])
+        self._loop_body(*bufs, n_calls, sigma, seed)
+        return bufs[1]
+
+    def capture_loop(self, params, n_calls, sigma=1e-3, seed=0):
+        """Capture the n_calls loop in one graph; returns (graph, out).
+
+        ``graph.replay()`` re-runs every call and rewrites ``out``.  One eager
+        call is made first so that the per-N tables exist before capture.
+        """
+        torch = _torch()
+        bufs = self._loop_buffers(params, n_calls)
+        base, out = bufs[0], bufs[1]
+        self._workspace(base.shape[1])
+        self._loop_body(*bufs, 1, sigma, seed)  # warm: twiddles, occupancy query
+        torch.cuda.synchronize()
+        g = torch.cuda.CUDAGraph()
+        s = torch.cuda.Stream()
+        s.wait_stream(torch.cuda.current_stream())
+        with torch.cuda.stream(s):
+            with torch.cuda.graph(g, stream=s):
+                self._loop_body(*bufs, n_calls, sigma, seed)
+        torch.cuda.current_stream().wait_stream(s)
+        self._graph = (g, bufs)
+        return g, out

@@ -1,0 +1,392 @@
+// Uneven-N batches in one persistent launch (gfx950).
+//
+// The reference evaluates one N per call (src/gbgpu/gbgpu.py:205-207 takes
+// N.max(); the native get_ll takes a scalar M, gbgpu_utils.cu:126), so an
+// uneven batch costs one call per N.  Here every binary carries its own N
+// (a power of two in [32, 2048]) and a single call does:
+//   k_batch_reset    clear the control block (no memset: graph-capture safe)
+//   k_batch_setup    per-binary constants (Pack) and a histogram of N buckets
+//   k_batch_scan     per-bucket list offsets and work-item offsets, largest N
+//                    first so the longest items start earliest
+//   k_batch_scatter  binaries into per-bucket lists
+//   k_batch_ll       persistent workgroups pull work items (BPW binaries of
+//                    one bucket) from an atomic queue and run the fused
+//                    waveform + likelihood for that bucket's N.
+// All scratch lives in a caller-provided workspace, so the sequence can be
+// captured in a hipGraph and replayed (the per-N orbit tables and twiddles
+// must have been created by an earlier, uncaptured call).
+// Which workgroup runs a binary depends on scheduling; each binary's result
+// does not, so results are deterministic.
+#include "common.hpp"
+#include "wave_common.hpp"
+
+namespace gbgpu {
+
+int check_data(const gbgpu_data *data);
+void set_data(WaveArgs &a, const gbgpu_data *data);
+
+constexpr int NBUCKET = 7;  // N = 32 << k, k < 7
+enum { CTL_COUNT = 0, CTL_CURSOR = 8, CTL_LISTOFF = 16, CTL_ITEMOFF = 24, CTL_ITEMS = 32, CTL_TOTAL = 40,
+       CTL_WORK = 41, CTL_WORDS = 48 };
+
+__host__ __device__ constexpr int bucket_bpw(int k) {
+    return WG_SIZE / (((32 << k) / 4 < WG_SIZE) ? (32 << k) / 4 : WG_SIZE);
+}
+
+__device__ __forceinline__ int bucket_of(int N) {
+    if (N < 32 || N > 2048 || (N & (N - 1))) return -1;
+    return 26 - __clz(N);  // log2(N) - 5 for 32-bit N
+}
+
+struct BatchArgs {
+    WaveArgs a;
+    const double *tabs[NBUCKET];
+    const void *tws[NBUCKET];
+    const int *Nb;
+    int *list;
+    int *ctl;
+};
+
+__global__ void k_batch_reset(int *ctl) {
+    for (int i = threadIdx.x; i < CTL_WORDS; i += blockDim.x) ctl[i] = 0;
+}
+
+template <class P>
+__global__ void __launch_bounds__(256) k_batch_setup(BatchArgs ba, Pack<P> *__restrict__ pack) {
+    const long long b = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (b >= ba.a.B) return;
+    const int N = ba.Nb[b];
+    const int k = bucket_of(N);
+    if (k < 0 || ba.tabs[k] == nullptr) {
+        // unsupported N or no orbit table for it: flag the result, skip the binary
+        const double nan = __builtin_nan("");
+        ba.a.loglike[b] = nan;
+        if (ba.a.d_h) ba.a.d_h[b] = make_double2(nan, nan);
+        if (ba.a.h_h) ba.a.h_h[b] = make_double2(nan, nan);
+        return;
+    }
+    Pack<P> pk;
+    pk.s = make_src(ba.a.params, ba.a.num_params, ba.a.B, b, N, ba.a.T, ba.a.tdi2, ba.a.armlength);
+    pk.aux = P::make_aux(pk.s);
+    pack[b] = pk;
+    atomicAdd(&ba.ctl[CTL_COUNT + k], 1);
+}
+
+__global__ void k_batch_scan(int *ctl) {
+    if (threadIdx.x != 0) return;
+    int list_run = 0, item_run = 0;
+    for (int k = NBUCKET - 1; k >= 0; k--) {
+        const int c = ctl[CTL_COUNT + k];
+        const int bpw = bucket_bpw(k);
+        ctl[CTL_LISTOFF + k] = list_run;
+        ctl[CTL_ITEMOFF + k] = item_run;
+        ctl[CTL_ITEMS + k] = (c + bpw - 1) / bpw;
+        list_run += c;
+        item_run += (c + bpw - 1) / bpw;
+    }
+    ctl[CTL_TOTAL] = item_run;
+}
+
+__global__ void k_batch_scatter(BatchArgs ba) {
+    const long long b = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (b >= ba.a.B) return;
+    const int k = bucket_of(ba.Nb[b]);
+    if (k < 0 || ba.tabs[k] == nullptr) return;
+    const int pos = atomicAdd(&ba.ctl[CTL_CURSOR + k], 1);
+    ba.list[ba.ctl[CTL_LISTOFF + k] + pos] = (int)b;
+}
+
+template <int N, class P>
+struct BatchLds {
+    using G = Geo<N>;
+    static constexpr size_t pack_bytes = (sizeof(Pack<P>) * G::BPW + 15) & ~(size_t)15;
+    static constexpr size_t tot_bytes = (sizeof(double) * G::BPW * G::WPB + 15) & ~(size_t)15;
+    static constexpr size_t buf_bytes = sizeof(typename P::CT) * G::BPW * 2 * N;
+    static constexpr size_t total = pack_bytes + tot_bytes + buf_bytes;
+};
+
+template <class P>
+constexpr size_t batch_lds_bytes() {
+    size_t m = BatchLds<32, P>::total;
+    m = m > BatchLds<64, P>::total ? m : BatchLds<64, P>::total;
+    m = m > BatchLds<128, P>::total ? m : BatchLds<128, P>::total;
+    m = m > BatchLds<256, P>::total ? m : BatchLds<256, P>::total;
+    m = m > BatchLds<512, P>::total ? m : BatchLds<512, P>::total;
+    m = m > BatchLds<1024, P>::total ? m : BatchLds<1024, P>::total;
+    m = m > BatchLds<2048, P>::total ? m : BatchLds<2048, P>::total;
+    return m;
+}
+
+// One work item: up to BPW binaries of bucket k (N = 32 << k).
+template <int N, class P>
+__device__ __noinline__ void batch_item(const BatchArgs &ba, int k, int local, int count, int list_off,
+                                           char *lds) {
+    using G = Geo<N>;
+    using CT = typename P::CT;
+    using L = BatchLds<N, P>;
+    constexpr int TPB = G::TPB, BPW = G::BPW, WPB = G::WPB;
+    Pack<P> *spack = reinterpret_cast<Pack<P> *>(lds);
+    double *stot = reinterpret_cast<double *>(lds + L::pack_bytes);
+    CT *sbuf = reinterpret_cast<CT *>(lds + L::pack_bytes + L::tot_bytes);
+    const int tid = threadIdx.x;
+    const int slot = tid / TPB;
+    const int lane = tid % TPB;
+    const int idx = local * BPW + slot;
+    const bool valid = idx < count;
+    const long long b = valid ? (long long)ba.list[list_off + idx] : 0;
+    if (valid) load_pack<P, TPB>(ba.a, b, lane, spack + slot);
+    __syncthreads();
+    const Src &s = spack[slot].s;  // read in place from LDS
+    CT *buf = sbuf + slot * 2 * N;
+    double *tot = stot + slot * WPB;
+    ThirdScratch ts{nullptr, nullptr, nullptr};
+    binary_spectrum<N, 2, false, P>(s, spack[slot].aux, valid, lane, ba.tabs[k], ba.a.fstar_inv, 0.5 * ba.a.T / N,
+                                    buf, static_cast<const CT *>(ba.tws[k]), ts, tot);
+    const double2 fac = P::fac(s);
+    double dhr = 0.0, dhi = 0.0, hh = 0.0;
+    if (valid) {
+        const long long doff = (ba.a.di ? (long long)ba.a.di[b] : 0LL) * ba.a.L;
+        const long long noff = (ba.a.ni ? (long long)ba.a.ni[b] : 0LL) * ba.a.L;
+        ll_partial<N, P>(buf, fac, lane, s.start - ba.a.start_freq_ind, doff, noff, ba.a, dhr, dhi, hh);
+    }
+    dhr = group_total<TPB>(dhr, lane, tot);
+    dhi = group_total<TPB>(dhi, lane, tot);
+    hh = group_total<TPB>(hh, lane, tot);
+    if (valid && lane == 0) {
+        dhr *= ba.a.df4;
+        dhi *= ba.a.df4;
+        hh *= ba.a.df4;
+        if (ba.a.d_h) ba.a.d_h[b] = make_double2(dhr, dhi);
+        if (ba.a.h_h) ba.a.h_h[b] = make_double2(hh, 0.0);
+        const double dh_use = ba.a.phase_marg ? sqrt(dhr * dhr + dhi * dhi) : dhr;
+        ba.a.loglike[b] = -0.5 * (ba.a.d_d + hh - 2.0 * dh_use);
+    }
+    __syncthreads();  // LDS is reused by the next item
+}
+
+template <class P>
+__device__ __forceinline__ void batch_loop(const BatchArgs &ba, char *lds) {
+    __shared__ int s_item;
+    const int total = ba.ctl[CTL_TOTAL];
+    for (;;) {
+        if (threadIdx.x == 0) s_item = atomicAdd(&ba.ctl[CTL_WORK], 1);
+        __syncthreads();
+        const int item = s_item;
+        __syncthreads();
+        if (item >= total) break;  // uniform: every thread read the same item
+        int k = NBUCKET - 1;
+        while (k > 0 && item >= ba.ctl[CTL_ITEMOFF + k] + ba.ctl[CTL_ITEMS + k]) k--;
+        const int local = item - ba.ctl[CTL_ITEMOFF + k];
+        const int count = ba.ctl[CTL_COUNT + k];
+        const int loff = ba.ctl[CTL_LISTOFF + k];
+        switch (k) {
+            case 0: batch_item<32, P>(ba, k, local, count, loff, lds); break;
+            case 1: batch_item<64, P>(ba, k, local, count, loff, lds); break;
+            case 2: batch_item<128, P>(ba, k, local, count, loff, lds); break;
+            case 3: batch_item<256, P>(ba, k, local, count, loff, lds); break;
+            case 4: batch_item<512, P>(ba, k, local, count, loff, lds); break;
+            case 5: batch_item<1024, P>(ba, k, local, count, loff, lds); break;
+            default: batch_item<2048, P>(ba, k, local, count, loff, lds); break;
+        }
+    }
+}
+
+// The per-N item bodies are separate (non-inlined) functions so each gets
+// its own register allocation; they read the arguments in place from the
+// kernel-argument segment instead of a private copy.  Occupancy targets:
+// 3 waves/SIMD for FP64 (~150 VGPRs needed), 4 for mixed.
+__global__ void __launch_bounds__(WG_SIZE) __attribute__((amdgpu_waves_per_eu(3, 8)))
+k_batch_ll_f64(BatchArgs ba) {
+    (void)ba;
+    extern __shared__ __align__(16) char lds[];
+    batch_loop<F64>(*(const BatchArgs *)__builtin_amdgcn_kernarg_segment_ptr(), lds);
+}
+
+__global__ void __launch_bounds__(WG_SIZE) __attribute__((amdgpu_waves_per_eu(4, 8)))
+k_batch_ll_mixed(BatchArgs ba) {
+    (void)ba;
+    extern __shared__ __align__(16) char lds[];
+    batch_loop<Mixed>(*(const BatchArgs *)__builtin_amdgcn_kernarg_segment_ptr(), lds);
+}
+
+template <class P>
+struct BatchKernel;
+template <>
+struct BatchKernel<F64> {
+    static void *fn() { return (void *)k_batch_ll_f64; }
+};
+template <>
+struct BatchKernel<Mixed> {
+    static void *fn() { return (void *)k_batch_ll_mixed; }
+};
+
+template <class P>
+size_t batch_ws_bytes(long long B) {
+    auto al = [](size_t n) { return (n + 255) & ~(size_t)255; };
+    return al(sizeof(Pack<P>) * (size_t)B) + al(sizeof(int) * (size_t)B) + al(sizeof(int) * CTL_WORDS);
+}
+
+template <class P>
+static int launch_batch(BatchArgs ba, void *workspace, hipStream_t st) {
+    const long long B = ba.a.B;
+    char *ws = static_cast<char *>(workspace);
+    bool own = false;
+    const size_t bytes = batch_ws_bytes<P>(B);
+    if (!ws) {
+        hipError_t e = hipMallocAsync((void **)&ws, bytes, st);
+        if (e != hipSuccess) return hip_fail(e, "hipMallocAsync(batch workspace)");
+        own = true;
+    }
+    auto al = [](size_t n) { return (n + 255) & ~(size_t)255; };
+    Pack<P> *pack = reinterpret_cast<Pack<P> *>(ws);
+    ba.list = reinterpret_cast<int *>(ws + al(sizeof(Pack<P>) * (size_t)B));
+    ba.ctl = reinterpret_cast<int *>(ws + al(sizeof(Pack<P>) * (size_t)B) + al(sizeof(int) * (size_t)B));
+    ba.a.pack = pack;
+    const unsigned gb = (unsigned)((B + 255) / 256);
+    hipLaunchKernelGGL(k_batch_reset, dim3(1), dim3(64), 0, st, ba.ctl);
+    hipLaunchKernelGGL((k_batch_setup<P>), dim3(gb), dim3(256), 0, st, ba, pack);
+    hipLaunchKernelGGL(k_batch_scan, dim3(1), dim3(64), 0, st, ba.ctl);
+    hipLaunchKernelGGL(k_batch_scatter, dim3(gb), dim3(256), 0, st, ba);
+    // persistent grid: as many workgroups as can be resident at once
+    static int blocks_per_cu[2] = {0, 0};
+    static int num_cu = 0;
+    const int pi = sizeof(typename P::CT) == 16 ? 0 : 1;
+    const size_t lds = batch_lds_bytes<P>();
+    if (!blocks_per_cu[pi]) {
+        int dev = 0;
+        (void)hipGetDevice(&dev);
+        (void)hipDeviceGetAttribute(&num_cu, hipDeviceAttributeMultiprocessorCount, dev);
+        (void)hipFuncSetAttribute(BatchKernel<P>::fn(), hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+        int nb = 0;
+        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, BatchKernel<P>::fn(), WG_SIZE, lds) != hipSuccess ||
+            nb < 1)
+            nb = 1;
+        blocks_per_cu[pi] = nb;
+    }
+    const long long grid = (long long)blocks_per_cu[pi] * num_cu;
+    void *kargs[] = {&ba};
+    hipError_t e = hipLaunchKernel(BatchKernel<P>::fn(), dim3((unsigned)grid), dim3(WG_SIZE), kargs, lds, st);
+    if (e == hipSuccess) e = hipGetLastError();
+    hipError_t e2 = own ? hipFreeAsync(ws, st) : hipSuccess;
+    if (e != hipSuccess) return hip_fail(e, "k_batch_ll");
+    if (e2 != hipSuccess) return hip_fail(e2, "hipFreeAsync(batch workspace)");
+    return GBGPU_OK;
+}
+
+// ----------------------------------------------------- parameter jumps ---
+// Gaussian relative jumps out = in * (1 + sigma z), z ~ N(0, 1) from a
+// counter-based hash (splitmix64 of seed, call, row, binary) and Box-Muller;
+// the same (seed, call) always gives the same draw.  Optionally recomputes N
+// per binary from f0 with a step rule (N_levels[i] below f_edges[i]).
+__device__ __forceinline__ unsigned long long splitmix64(unsigned long long x) {
+    x += 0x9e3779b97f4a7c15ULL;
+    x = (x ^ (x >> 30)) * 0xbf58476d1ce4e5b9ULL;
+    x = (x ^ (x >> 27)) * 0x94d049bb133111ebULL;
+    return x ^ (x >> 31);
+}
+
+struct NRule {
+    int n_edges;
+    double edges[8];
+    int levels[9];
+};
+
+__global__ void k_perturb(double *out, const double *in, int num_params, long long B, double sigma,
+                          unsigned long long seed, unsigned long long call, int *N_out, NRule rule) {
+    const long long b = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (b >= B) return;
+    for (int r = 0; r < num_params; r++) {
+        const unsigned long long h1 = splitmix64(seed ^ splitmix64(call * 0x100000001b3ULL + (unsigned long long)r) ^
+                                                 splitmix64((unsigned long long)b * 0x2545f4914f6cdd1dULL));
+        const unsigned long long h2 = splitmix64(h1);
+        const double u1 = ((h1 >> 11) + 1) * (1.0 / 9007199254740993.0);  // (0, 1]
+        const double u2 = (h2 >> 11) * (1.0 / 9007199254740992.0);        // [0, 1)
+        const double z = sqrt(-2.0 * log(u1)) * cos(TWO_PI * u2);
+        out[r * B + b] = in[r * B + b] * (1.0 + sigma * z);
+    }
+    if (N_out) {
+        const double f0 = fabs(out[1 * B + b]);
+        int i = 0;
+        while (i < rule.n_edges && f0 >= rule.edges[i]) i++;
+        N_out[b] = rule.levels[i];
+    }
+}
+
+}  // namespace gbgpu
+
+using namespace gbgpu;
+
+extern "C" long long gbgpu_batch_workspace_bytes(int num_bin, int precision) {
+    if (num_bin < 0) return -1;
+    return precision == GBGPU_MIXED ? (long long)batch_ws_bytes<Mixed>(num_bin)
+                                    : (long long)batch_ws_bytes<F64>(num_bin);
+}
+
+extern "C" int gbgpu_wave_ll_batched(double *loglike, gbgpu_cmplx *d_h, gbgpu_cmplx *h_h, const double *params,
+                                     int num_params, int num_bin, const int *N_bin, const gbgpu_batch_cfg *cfg,
+                                     const gbgpu_data *data, double d_d, int phase_marginalize, void *workspace,
+                                     void *stream) {
+    if (!cfg || !params || !N_bin || !loglike) {
+        set_error("null cfg, params, N_bin or loglike");
+        return GBGPU_E_ARG;
+    }
+    if (num_params != GBGPU_NPARAMS) {
+        set_error("the batched path takes the 9 circular-binary parameter rows");
+        return GBGPU_E_ARG;
+    }
+    if (num_bin < 0 || !(cfg->T > 0) || !(cfg->armlength > 0) ||
+        (cfg->precision != GBGPU_FP64 && cfg->precision != GBGPU_MIXED)) {
+        set_error("bad num_bin, T, armlength or precision");
+        return GBGPU_E_ARG;
+    }
+    int rc = check_data(data);
+    if (rc) return rc;
+    if (num_bin == 0) return GBGPU_OK;
+    BatchArgs ba = {};
+    WaveArgs &a = ba.a;
+    a.params = params;
+    a.num_params = num_params;
+    a.B = num_bin;
+    a.T = cfg->T;
+    a.fstar_inv = 1.0 / (C_SI / (cfg->armlength * 2 * PI));
+    a.armlength = cfg->armlength;
+    a.tdi2 = cfg->tdi2;
+    set_data(a, data);
+    a.d_d = d_d;
+    a.phase_marg = phase_marginalize;
+    a.loglike = loglike;
+    a.d_h = reinterpret_cast<double2 *>(d_h);
+    a.h_h = reinterpret_cast<double2 *>(h_h);
+    ba.Nb = N_bin;
+    for (int k = 0; k < NBUCKET; k++) {
+        ba.tabs[k] = cfg->orbit_tables[k];
+        ba.tws[k] = nullptr;
+        if (ba.tabs[k]) {
+            const int N = 32 << k;
+            ba.tws[k] = cfg->precision == GBGPU_MIXED ? (const void *)twiddles_f32(N) : (const void *)twiddles_f64(N);
+            if (!ba.tws[k]) return hip_fail(hipErrorOutOfMemory, "twiddles");
+        }
+    }
+    hipStream_t st = static_cast<hipStream_t>(stream);
+    return cfg->precision == GBGPU_MIXED ? launch_batch<Mixed>(ba, workspace, st) : launch_batch<F64>(ba, workspace, st);
+}
+
+extern "C" int gbgpu_perturb_params(double *out, const double *in, int num_params, int num_bin, double sigma_rel,
+                                    unsigned long long seed, unsigned long long call_index, int *N_out,
+                                    int n_edges, const double *f_edges, const int *N_levels, void *stream) {
+    if (!out || !in || num_params < 2 || num_bin < 0 || n_edges < 0 || n_edges > 8 ||
+        (N_out && (!f_edges || !N_levels) && n_edges > 0)) {
+        set_error("gbgpu_perturb_params: bad arguments");
+        return GBGPU_E_ARG;
+    }
+    if (num_bin == 0) return GBGPU_OK;
+    NRule rule = {};
+    rule.n_edges = N_out ? n_edges : 0;
+    for (int i = 0; i < rule.n_edges; i++) rule.edges[i] = f_edges[i];
+    for (int i = 0; N_out && i <= rule.n_edges; i++) rule.levels[i] = N_levels[i];
+    hipLaunchKernelGGL(k_perturb, dim3((unsigned)((num_bin + 255) / 256)), dim3(256), 0,
+                       static_cast<hipStream_t>(stream), out, in, num_params, (long long)num_bin, sigma_rel, seed,
+                       call_index, N_out, rule);
+    GBGPU_CHECK_LAUNCH("k_perturb");
+    return GBGPU_OK;
+}

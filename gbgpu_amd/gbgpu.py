@@ -334,6 +334,9 @@ class GBGPUBase(abc.ABC):
             raise TypeError(f"unexpected keyword arguments {list(kwargs)}")
         params = np.asarray(params) if not torch.is_tensor(params) else params.cpu().numpy()
         self._check_data(data, data_index, noise_index)
+        if N is not None and np.ndim(N) == 1:
+            return self._get_ll_uneven(params, data, psd, np.asarray(N), T, dt, tdi2, precision, phase_marginalize,
+                                       start_freq_ind, data_index, noise_index)
         p, N, T = self._prepare(*params[:9], params[9:], N, T, dt, oversample)
         B = p.shape[1]
         self.num_bin, self.N, self.df = B, N, 1 / T
@@ -418,6 +421,27 @@ class GBGPUBase(abc.ABC):
         if return_terms:
             return dll.cpu().numpy(), terms.cpu().numpy()
         return dll.cpu().numpy()
+
+    def _get_ll_uneven(self, params, data, psd, N, T, dt, tdi2, precision, phase_marginalize, start_freq_ind,
+                       data_index, noise_index):
+        """get_ll with one N per binary: one persistent launch over N buckets."""
+        from .batch import BatchedLogLike
+
+        if tdi2 or data_index is not None or noise_index is not None or params.shape[0] != 9:
+            raise NotImplementedError("per-binary N supports circular binaries, TDI1 and a single data stream")
+        if len(N) != params.shape[1]:
+            raise ValueError("N must have one entry per binary")
+        bl = BatchedLogLike(self, data, psd, T, dt, precision=precision or self.precision,
+                            start_freq_ind=start_freq_ind, N_levels=(int(N.min()),), f_edges=(),
+                            phase_marginalize=phase_marginalize)
+        ll, d_h, h_h = bl(params, N=N.astype(np.int32), d_h=True)
+        self.num_bin, self.N, self.df = params.shape[1], N, 1 / bl.T
+        if phase_marginalize:
+            self.non_marg_d_h = d_h.cpu().numpy()
+            d_h = _torch().abs(d_h).to(_torch().complex128)
+        self.h_h, self.d_h = h_h, d_h
+        self._wave_state = None
+        return ll.cpu().numpy()
 
     # -------------------------------------------------- global templates ---
     def fill_global_template(self, group_index, templates, A, E, start_inds, N=None, start_freq_ind=0):

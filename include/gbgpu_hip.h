@@ -127,6 +127,45 @@ int gbgpu_wave_ll(double *loglike, gbgpu_cmplx *d_h, gbgpu_cmplx *h_h, const dou
 int gbgpu_swap_ll_diff(double *dll, double *terms, const double *params_pairs, int num_params, int num_prop,
                        const gbgpu_wave_cfg *cfg, const gbgpu_data *data, void *stream);
 
+/* ---- uneven-N batches -------------------------------------------------- */
+
+/* Configuration of a batch whose binaries carry their own N. */
+typedef struct gbgpu_batch_cfg {
+    double T;                      /* observation time, already int(T/dt)*dt */
+    double armlength;              /* metres */
+    int tdi2;
+    int precision;                 /* GBGPU_FP64 or GBGPU_MIXED */
+    const double *orbit_tables[7]; /* table for N = 32 << k (k < 7), NULL when no binary uses it */
+} gbgpu_batch_cfg;
+
+/* Workspace bytes gbgpu_wave_ll_batched needs for num_bin binaries. */
+long long gbgpu_batch_workspace_bytes(int num_bin, int precision);
+
+/* Fused wave + log-likelihood for a batch with per-binary N (N_bin, device
+ * int32, powers of two in [32, 2048]) in one persistent launch: binaries are
+ * bucketed by N on the device and workgroups pull N-bucketed work items
+ * from an atomic queue, largest N first.  The reference needs one run_wave +
+ * get_ll call per distinct N (src/gbgpu/gbgpu.py:205-207, gbgpu_utils.cu:126).
+ * Outputs as gbgpu_wave_ll; binaries with an unsupported N or a NULL orbit
+ * table get NaN.  workspace: gbgpu_batch_workspace_bytes() of device memory,
+ * or NULL to allocate stream-ordered scratch.  With a workspace and after one
+ * earlier call on the device (which creates the per-N twiddle tables), the
+ * call performs no allocation, memset or host synchronisation, so it can be
+ * captured in a hipGraph. */
+int gbgpu_wave_ll_batched(double *loglike, gbgpu_cmplx *d_h, gbgpu_cmplx *h_h, const double *params,
+                          int num_params, int num_bin, const int *N_bin, const gbgpu_batch_cfg *cfg,
+                          const gbgpu_data *data, double d_d, int phase_marginalize, void *workspace,
+                          void *stream);
+
+/* Sampler utility with no counterpart in the reference: seeded Gaussian
+ * relative jumps out = in * (1 + sigma_rel * z) for every parameter row
+ * (counter-based: the same seed and call_index give the same draw), and,
+ * when N_out is non-NULL, N per binary from f0 (row 1) by a step rule:
+ * N_levels[i] for f0 < f_edges[i], N_levels[n_edges] above (n_edges <= 8). */
+int gbgpu_perturb_params(double *out, const double *in, int num_params, int num_bin, double sigma_rel,
+                         unsigned long long seed, unsigned long long call_index, int *N_out, int n_edges,
+                         const double *f_edges, const int *N_levels, void *stream);
+
 /* ---- library -------------------------------------------------------- */
 
 /* Message describing the last failure on the calling thread. */

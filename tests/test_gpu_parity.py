@@ -402,3 +402,107 @@ def test_swap_third_body_proposal():
         for i in range(B)
     ])
     assert np.allclose(dll, ref, rtol=1e-7, atol=1e-6)
+
+
+# --------------------------------------------------------- uneven N batch ---
+def _uneven_batch(B, seed):
+    from gbgpu_amd.utils.utility import baseline_N
+
+    rng = np.random.default_rng(seed)
+    p = draw_params(B, rng, f_lo=1e-4, f_hi=20e-3, log_f=True)
+    return p, baseline_N(p[1]).astype(np.int32), rng
+
+
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_uneven_batch_matches_per_N_calls(precision):
+    """One persistent launch over N buckets == one standalone call per N
+    (same kernels per binary) and == the oracle within the precision's bar."""
+    B = 300
+    p, N, rng = _uneven_batch(B, 600)
+    T = 4 * YR
+    A, E, SA, SE, df, L = make_data(p[:, :20], 512, T, DT, rng)
+    gb = _gb(precision=precision)
+    gb.d_d = float(4 * df * (np.sum(np.abs(A) ** 2 / SA) + np.sum(np.abs(E) ** 2 / SE)))
+    ll = gb.get_ll(p, [A, E], [SA, SE], N=N, T=T, dt=DT)
+    d_h_b = gb.d_h.cpu().numpy()
+    ref = np.empty(B)
+    for n in np.unique(N):
+        sel = N == n
+        ref[sel] = gb.get_ll(p[:, sel], [A, E], [SA, SE], N=int(n), T=T, dt=DT)
+    assert np.array_equal(ll, ref)  # identical arithmetic per binary
+    # oracle on a sample of every bucket
+    idx = np.concatenate([np.flatnonzero(N == n)[:4] for n in np.unique(N)])
+    h_h_b = gb.h_h.cpu().numpy()
+    for n in np.unique(N[idx]):
+        sel = idx[N[idx] == n]
+        o = F.run_wave(*p[:, sel], N=int(n), T=T, dt=DT)
+        d_h, h_h = F.inner_terms(o["AETf"][:, 0], o["AETf"][:, 1], o["start_inds"], A, E, SA, SE, df)
+        ll_o = F.loglike(gb.d_d, d_h, h_h)
+        bar = 1e-6 if precision == "mixed" else 1e-8
+        assert np.all(np.abs(ll[sel] - ll_o) <= bar * (np.abs(h_h) + np.abs(d_h)) + bar), (n, np.abs(ll[sel] - ll_o))
+
+
+def test_perturb_params_matches_numpy_restatement():
+    """The seeded jump generator is a fixed function of (seed, call, row, binary)."""
+    import torch
+
+    from gbgpu_amd import _capi
+
+    B, R = 1000, 9
+    base = np.random.default_rng(1).uniform(0.5, 2.0, (R, B))
+    out = torch.empty((R, B), dtype=torch.float64, device="cuda")
+    Nout = torch.empty(B, dtype=torch.int32, device="cuda")
+    edges = np.array([0.8, 1.2, 1.6])
+    levels = np.array([128, 256, 512, 1024], dtype=np.int32)
+    lib = _capi.load()
+    rc = lib.gbgpu_perturb_params(out.data_ptr(), torch.from_numpy(base).cuda().data_ptr(), R, B, 1e-3, 12345, 7,
+                                  Nout.data_ptr(), 3, edges.ctypes.data, levels.ctypes.data, _capi.stream_handle())
+    _capi.check(rc, "perturb")
+    M = np.uint64(0xFFFFFFFFFFFFFFFF)
+
+    def sm(x):
+        x = (x + np.uint64(0x9E3779B97F4A7C15)) & M
+        x = ((x ^ (x >> np.uint64(30))) * np.uint64(0xBF58476D1CE4E5B9)) & M
+        x = ((x ^ (x >> np.uint64(27))) * np.uint64(0x94D049BB133111EB)) & M
+        return x ^ (x >> np.uint64(31))
+
+    with np.errstate(over="ignore"):
+        b = np.arange(B, dtype=np.uint64)
+        exp = np.empty((R, B))
+        for r in range(R):
+            h1 = sm(np.uint64(12345) ^ sm(np.uint64(7) * np.uint64(0x100000001B3) + np.uint64(r)) ^
+                    sm(b * np.uint64(0x2545F4914F6CDD1D)))
+            h2 = sm(h1)
+            u1 = ((h1 >> np.uint64(11)) + np.uint64(1)).astype(np.float64) * (1.0 / 9007199254740993.0)
+            u2 = (h2 >> np.uint64(11)).astype(np.float64) * (1.0 / 9007199254740992.0)
+            exp[r] = base[r] * (1.0 + 1e-3 * np.sqrt(-2.0 * np.log(u1)) * np.cos(2 * np.pi * u2))
+    got = out.cpu().numpy()
+    assert np.allclose(got, exp, rtol=1e-13, atol=0)
+    assert np.array_equal(Nout.cpu().numpy(), levels[np.searchsorted(edges, np.abs(got[1]), side="right")])
+
+
+def test_graph_captured_loop_replays_match_eager():
+    """Captured loop of jump + batched likelihood: every replay equals the eager loop."""
+    import torch
+
+    from gbgpu_amd.batch import BatchedLogLike
+
+    B = 256
+    p, N, rng = _uneven_batch(B, 700)
+    T = 4 * YR
+    A, E, SA, SE, df, L = make_data(p[:, :10], 256, T, DT, rng)
+    gb = _gb(precision="mixed")
+    gb.d_d = float(4 * df * (np.sum(np.abs(A) ** 2 / SA) + np.sum(np.abs(E) ** 2 / SE)))
+    bl = BatchedLogLike(gb, [A, E], [SA, SE], T, DT, precision="mixed")
+    eager = bl.run_loop(p, 5, sigma=1e-3, seed=3).clone()
+    torch.cuda.synchronize()
+    g, out = bl.capture_loop(p, 5, sigma=1e-3, seed=3)
+    for rep in range(3):
+        out.zero_()
+        torch.cuda.synchronize()
+        g.replay()
+        torch.cuda.synchronize()
+        assert torch.equal(out, eager), rep
+        _ = bl(p)  # eager work between replays
+        torch.cuda.synchronize()
+    assert torch.all(torch.isfinite(eager))
