@@ -153,6 +153,63 @@ def parity_sample(gb, params, A, E, SA, SE, d_d, df, N, T, dt, precision, n=64):
     return dict(n=n, max_abs_logL_err=float(np.max(np.abs(ll - ll_o))), max_rel_wave_err=float(werr.max()))
 
 
+def uneven_batch(gb, A, E, SA, SE, d_d, B, T, dt, n_calls, seed=77):
+    """Uneven-N batch: f0 log-uniform in [0.1, 20] mHz, N by the frequency
+    rule (128/256/512/1024/2048 split at 1/3/6/10 mHz), a captured graph of
+    n_calls likelihood calls at seeded 1e-3 relative jumps; FP64 and mixed."""
+    import torch
+
+    from gbgpu_amd.batch import BatchedLogLike
+    from gbgpu_amd.utils.utility import baseline_N
+    from oracle import fastgb as F
+    from tests.helpers import draw_params
+
+    rng = np.random.default_rng(seed)
+    p = draw_params(B, rng, f_lo=1e-4, f_hi=20e-3, log_f=True)
+    N = baseline_N(p[1])
+    res = {"num_bin": B, "calls": n_calls,
+           "N_fractions": {int(n): float(np.mean(N == n)) for n in np.unique(N)},
+           "mean_N": float(N.mean())}
+    lls = {}
+    for prec in ("fp64", "mixed"):
+        print(f"[bench] uneven batch {prec}", file=sys.stderr, flush=True)
+        bl = BatchedLogLike(gb, [A, E], [SA, SE], T, dt, precision=prec)
+        g, out = bl.capture_loop(p, n_calls, sigma=1e-3, seed=seed)
+        g.replay()
+        torch.cuda.synchronize()
+        ev0 = torch.cuda.Event(enable_timing=True)
+        ev1 = torch.cuda.Event(enable_timing=True)
+        reps = 2
+        ev0.record()
+        for _ in range(reps):
+            g.replay()
+        ev1.record()
+        torch.cuda.synchronize()
+        sec = ev0.elapsed_time(ev1) / 1e3 / reps
+        res[prec] = {"binaries_per_s": n_calls * B / sec, "ms_per_call": sec / n_calls * 1e3}
+        lls[prec] = out.clone()
+        del g, out, bl
+        torch.cuda.empty_cache()
+    res["speedup_mixed_vs_fp64"] = res["mixed"]["binaries_per_s"] / res["fp64"]["binaries_per_s"]
+    d = (lls["mixed"] - lls["fp64"]).abs()
+    res["logL_abs_err_mixed_vs_fp64"] = {"max": float(d.max()), "median": float(d.median()),
+                                         "frac_le_1e-3": float((d <= 1e-3).double().mean())}
+    # per-channel waveform error of the mixed path against the FP64 oracle,
+    # on a sample of every N bucket (unperturbed parameters)
+    errs = {"A": 0.0, "E": 0.0, "T": 0.0}
+    for n in np.unique(N):
+        sel = np.flatnonzero(N == n)[:16]
+        gb.run_wave(*p[:, sel], N=int(n), T=T, dt=dt, precision="mixed")
+        aet = gb.AETf.cpu().numpy()
+        o = F.run_wave(*p[:, sel], N=int(n), T=T, dt=dt)
+        for c, name in enumerate("AET"):
+            num = np.sqrt(np.sum(np.abs(aet[:, c] - o["AETf"][:, c]) ** 2, axis=1))
+            den = np.sqrt(np.sum(np.abs(o["AETf"][:, c]) ** 2, axis=1))
+            errs[name] = max(errs[name], float(np.max(num / den)))
+    res["mixed_max_rel_wave_err_per_channel"] = errs
+    return res
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -164,6 +221,8 @@ def main():
     ap.add_argument("--copy-probe", action="store_true",
                     help="first run a 1 GiB device-to-device copy (calibrates HBM byte counters)")
     ap.add_argument("--only", choices=["fp64", "mixed"], default=None, help="time one precision only")
+    ap.add_argument("--no-uneven", action="store_true", help="skip the uneven-N batch")
+    ap.add_argument("--uneven-calls", type=int, default=100, help="calls in the captured uneven-N loop")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build measures one GPU only; --gpus must be 1"}))
@@ -230,6 +289,10 @@ def main():
       except RuntimeError as e:
         mixed = dict(error=str(e)[:200])
 
+    uneven = None
+    if not args.no_uneven:
+        uneven = uneven_batch(gb, A, E, SA, SE, d_d, args.num_bin, T, dt, args.uneven_calls)
+
     cpu = None
     if not args.no_cpu:
         print("[bench] cpu baseline", file=sys.stderr, flush=True)
@@ -271,6 +334,7 @@ def main():
         "cpu_baseline": cpu,
         "parity": par64,
         "mixed": mixed,
+        "uneven_batch": uneven,
         "wall_ms_per_step": wall64 * 1e3,
     }
     print(json.dumps(out))
