@@ -93,6 +93,16 @@ SIGNATURES = {
         c_int,
         [c_void_p, c_void_p, c_void_p, c_int, c_int, ctypes.POINTER(WaveCfg), ctypes.POINTER(Data), c_void_p],
     ),
+    "gbgpu_wave_ll_grad": (
+        c_int,
+        [c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_int, c_int, ctypes.POINTER(WaveCfg),
+         ctypes.POINTER(Data), c_double, c_int, c_void_p],
+    ),
+    "gbgpu_fisher": (
+        c_int,
+        [c_void_p, c_void_p, c_void_p, c_void_p, c_int, c_int, ctypes.POINTER(WaveCfg), ctypes.POINTER(Data),
+         c_double, c_void_p],
+    ),
     "gbgpu_batch_workspace_bytes": (c_longlong, [c_int, c_int]),
     "gbgpu_wave_ll_batched": (
         c_int,

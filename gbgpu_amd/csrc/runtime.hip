@@ -57,4 +57,4 @@ const float2 *twiddles_f32(int N) { return get_twiddles<float2>(g_tw32, N); }
 
 extern "C" const char *gbgpu_last_error(void) { return gbgpu::g_last_error.c_str(); }
 
-extern "C" int gbgpu_abi_version(void) { return 1; }
+extern "C" int gbgpu_abi_version(void) { return 2; }

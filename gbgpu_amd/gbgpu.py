@@ -539,13 +539,112 @@ class GBGPUBase(abc.ABC):
                 params[np.asarray(ind_trans)] = trans(*params[np.asarray(ind_trans)])
         return params
 
-    def information_matrix(self, params, eps=1e-9, parameter_transforms={}, inds=None, N=1024, psd_func=None,
-                           psd_kwargs={}, easy_central_difference=False, return_gpu=False, **kwargs):
-        """Finite-difference information matrix (src/gbgpu/gbgpu.py:957-1138).
+    # ------------------------------------------------------- derivatives ---
+    def _deriv_setup(self, params, N, T, dt, oversample, kwargs_left):
+        if kwargs_left:
+            raise TypeError(f"unexpected keyword arguments {list(kwargs_left)}")
+        self._check_hooks()
+        if type(self)._kernel_variant != "circular":
+            raise NotImplementedError("derivatives are implemented for circular (9-parameter) binaries")
+        torch = _torch()
+        params = np.asarray(params) if not torch.is_tensor(params) else params.detach().cpu().numpy()
+        params = np.atleast_2d(np.asarray(params, dtype=np.float64))
+        if params.shape[0] != 9:
+            raise ValueError("derivatives need (9, num_bin) parameters")
+        return self._prepare(*params[:9], (), N, T, dt, oversample)
 
-        Waveforms come from the HIP run_wave; the 5-point (or central)
-        multiplicative-step stencil and the inner products follow the
-        reference.  ``psd_func`` defaults to the analytic A-channel PSD.
+    def get_ll_grad(self, params, data, psd, phase_marginalize=False, start_freq_ind=0, basis="sampling",
+                    tie_fddot=False, return_gpu=False, **kwargs):
+        """Log-likelihood and its gradient from the adjoint kernel (FP64).
+
+        Same likelihood and data conventions as ``get_ll`` (one A/E stream).
+        Returns ``(loglike (B,), grad (B, M))``: with basis "sampling" the
+        gradient is with respect to (ln amp, f0, fdot, fddot, phi0, cos iota,
+        psi, lam, sin beta); with "raw" with respect to the run_wave
+        parameters; ``tie_fddot`` folds fddot = 11/3 fdot^2/f0 into f0 and
+        fdot and drops its entry (M = 8).
+        """
+        from .derivatives import basis_jacobian
+
+        if self.d_d is None:
+            raise ValueError("self.d_d attribute must be set before computing log-Likelihood.")
+        torch = _torch()
+        N = kwargs.pop("N", None)
+        T = kwargs.pop("T", 4 * YRSID_SI)
+        dt = kwargs.pop("dt", 10.0)
+        tdi2 = kwargs.pop("tdi2", False)
+        oversample = kwargs.pop("oversample", 1)
+        p, N, T = self._deriv_setup(params, N, T, dt, oversample, kwargs)
+        B = p.shape[1]
+        self._check_data(data, None, None)
+        dA, dE, SA, SE, di, ni, L = self._prep_data(data, psd, None, None, B)
+        cfg, _tab = self._cfg(N, T, tdi2, "fp64")
+        ll = torch.empty(B, dtype=torch.float64, device=p.device)
+        g = torch.empty((B, 9), dtype=torch.float64, device=p.device)
+        dat = _capi.Data(A=dA.data_ptr(), E=dE.data_ptr(), psdA=SA.data_ptr(), psdE=SE.data_ptr(), data_index=None,
+                         noise_index=None, data_length=int(L), start_freq_ind=int(start_freq_ind), df=float(1 / T))
+        rc = _capi.load().gbgpu_wave_ll_grad(ll.data_ptr(), g.data_ptr(), None, None, p.data_ptr(), 9, B, cfg, dat,
+                                             float(np.real(self.d_d)), int(bool(phase_marginalize)),
+                                             _capi.stream_handle())
+        _capi.check(rc, "gbgpu_wave_ll_grad")
+        J = basis_jacobian(p, basis, tie_fddot)
+        g = (J.transpose(1, 2) @ g.unsqueeze(-1)).squeeze(-1)
+        if return_gpu:
+            return ll, g
+        return ll.cpu().numpy(), g.cpu().numpy()
+
+    def fisher_matrix(self, params, psd, data=None, start_freq_ind=0, basis="sampling", tie_fddot=False,
+                      return_gpu=False, **kwargs):
+        """Fisher matrices from exact forward tangents (FP64, N <= 1024).
+
+        F_kl = 4 df sum_{A,E} sum_f Re(conj(dh_k) dh_l) / S over each
+        template's bins within the PSD streams ``psd`` = [S_A, S_E] (indexed
+        like get_ll's data, from ``start_freq_ind``).  Returns (B, M, M) in
+        the basis chosen as for ``get_ll_grad``.
+        """
+        from .derivatives import basis_jacobian
+
+        torch = _torch()
+        N = kwargs.pop("N", None)
+        T = kwargs.pop("T", 4 * YRSID_SI)
+        dt = kwargs.pop("dt", 10.0)
+        tdi2 = kwargs.pop("tdi2", False)
+        oversample = kwargs.pop("oversample", 1)
+        p, N, T = self._deriv_setup(params, N, T, dt, oversample, kwargs)
+        B = p.shape[1]
+        SA = self._to_dev(psd[0], np.float64).reshape(-1)
+        SE = self._to_dev(psd[1], np.float64).reshape(-1)
+        if data is not None:
+            dA = self._to_dev(data[0], np.complex128).reshape(-1)
+            dE = self._to_dev(data[1], np.complex128).reshape(-1)
+            if dA.numel() != SA.numel():
+                raise ValueError("data and psd streams must have the same length")
+        cfg, _tab = self._cfg(N, T, tdi2, "fp64")
+        F = torch.empty((B, 9, 9), dtype=torch.float64, device=p.device)
+        dat = _capi.Data(A=None if data is None else dA.data_ptr(), E=None if data is None else dE.data_ptr(),
+                         psdA=SA.data_ptr(), psdE=SE.data_ptr(), data_index=None, noise_index=None,
+                         data_length=int(SA.numel()), start_freq_ind=int(start_freq_ind), df=float(1 / T))
+        rc = _capi.load().gbgpu_fisher(F.data_ptr(), None, None, p.data_ptr(), 9, B, cfg, dat,
+                                       float(np.real(self.d_d or 0.0)), _capi.stream_handle())
+        _capi.check(rc, "gbgpu_fisher")
+        J = basis_jacobian(p, basis, tie_fddot)
+        F = J.transpose(1, 2) @ F @ J
+        return F if return_gpu else F.cpu().numpy()
+
+    def information_matrix(self, params, eps=1e-9, parameter_transforms={}, inds=None, N=1024, psd_func=None,
+                           psd_kwargs={}, easy_central_difference=False, return_gpu=False, method="exact",
+                           **kwargs):
+        """Information matrix (src/gbgpu/gbgpu.py:957-1138).
+
+        As the reference, the derivative with respect to parameter i is taken
+        with a multiplicative step, dh_i = x_i dh/dx_i (x = ``params``, mapped
+        to waveform parameters by ``parameter_transforms`` after the step),
+        and F_ij = 4 df sum_{A,E} Re(conj(dh_i) dh_j) / psd_func(f) over the
+        template bins.  ``method="exact"`` takes dh/dx from the forward-tangent
+        kernel (the transforms' Jacobian by the reference's stencil on the
+        transforms alone); ``method="fd"`` differences HIP run_wave outputs with
+        the reference's 5-point (or central) stencil.  ``psd_func`` defaults to
+        the analytic A-channel PSD.
         """
         torch = _torch()
         if psd_func is None:
@@ -554,17 +653,81 @@ class GBGPUBase(abc.ABC):
             psd_func = A1TDISens.get_Sn
         if N is None:
             raise ValueError("N must be provided up front for Infromation Matrix.")
-        kwargs["N"] = N
         params = np.atleast_2d(np.asarray(params, dtype=np.float64))
         num_params, num_bins = params.shape
         if inds is None:
             inds = np.arange(num_params)
+        inds = np.asarray(inds)
+        exact_ok = method == "exact" and num_params == 9 and N <= 1024 and type(self)._kernel_variant == "circular"
+        if method not in ("exact", "fd"):
+            raise ValueError("method must be 'exact' or 'fd'")
+        if not exact_ok:
+            return self._information_matrix_fd(params, eps, parameter_transforms, inds, N, psd_func, psd_kwargs,
+                                               easy_central_difference, return_gpu, **kwargs)
+        from .derivatives import basis_jacobian
+
+        T = kwargs.pop("T", 4 * YRSID_SI)
+        dt = kwargs.pop("dt", 10.0)
+        tdi2 = kwargs.pop("tdi2", False)
+        oversample = kwargs.pop("oversample", 1)
+        if kwargs:
+            raise TypeError(f"unexpected keyword arguments {list(kwargs)}")
+        raw = self._apply_parameter_transforms(params.copy(), parameter_transforms)
+        # d raw / d ln x: identity scaled by x without transforms, else the
+        # reference's stencil applied to the transforms
+        if parameter_transforms:
+            Jx = np.zeros((num_bins, 9, num_params))
+            for j in range(num_params):
+                def tr(m):
+                    q = np.ones_like(params)
+                    q[j] += m * eps
+                    return self._apply_parameter_transforms(q * params, parameter_transforms)
+
+                if easy_central_difference:
+                    d = (tr(1) - tr(-1)) / (2 * eps)
+                else:
+                    d = (-tr(2) + tr(-2) + 8 * (tr(1) - tr(-1))) / (12 * eps)
+                Jx[:, :, j] = d.T
+        else:
+            Jx = np.broadcast_to(np.eye(9), (num_bins, 9, 9)) * params.T[:, None, :]
+        p, N, T = self._prepare(*raw[:9], (), N, T, dt, oversample)
+        df = 1.0 / T
+        start = np.rint(raw[1] * T).astype(np.int64) - N // 2
+        k0, k1 = int(start.min()), int(start.max()) + N
+        fgrid = (k0 + np.arange(k1 - k0)) * df
+        S = np.asarray(psd_func(fgrid, **psd_kwargs), dtype=np.float64)
+        St = torch.from_numpy(np.ascontiguousarray(S)).to(p.device)
+        cfg, _tab = self._cfg(N, T, tdi2, "fp64")
+        F = torch.empty((num_bins, 9, 9), dtype=torch.float64, device=p.device)
+        dat = _capi.Data(A=None, E=None, psdA=St.data_ptr(), psdE=St.data_ptr(), data_index=None, noise_index=None,
+                         data_length=int(St.numel()), start_freq_ind=k0, df=float(df))
+        rc = _capi.load().gbgpu_fisher(F.data_ptr(), None, None, p.data_ptr(), 9, num_bins, cfg, dat, 0.0,
+                                       _capi.stream_handle())
+        _capi.check(rc, "gbgpu_fisher")
+        J = basis_jacobian(p, "raw") @ torch.from_numpy(np.ascontiguousarray(Jx)).to(p.device)
+        info = J.transpose(1, 2) @ F @ J
+        sel = torch.from_numpy(inds).to(p.device)
+        info = info[:, sel][:, :, sel]
+        self.N, self.df, self.num_bin = N, df, num_bins
+        if return_gpu:
+            return info
+        return info.cpu().numpy()
+
+    def _information_matrix_fd(self, params, eps, parameter_transforms, inds, N, psd_func, psd_kwargs,
+                               easy_central_difference, return_gpu, **kwargs):
+        """The reference's finite-difference stencil over HIP run_wave outputs."""
+        torch = _torch()
+        kwargs["N"] = N
+        num_params, num_bins = params.shape
         nd = len(inds)
         dev = self._device()
         dh = torch.zeros((num_bins, nd, 2, N), dtype=torch.complex128, device=dev)
 
+        starts = []
+
         def wave(p):
             self.run_wave(*p, **kwargs)
+            starts.append(self.start_inds.clone())
             return torch.stack([self.A, self.E], dim=1).clone()
 
         for i, ind in enumerate(inds):
@@ -574,19 +737,23 @@ class GBGPUBase(abc.ABC):
                 p *= params
                 return self._apply_parameter_transforms(p, parameter_transforms)
 
+            starts.clear()
             up1 = wave(step(1))
             dn1 = wave(step(-1))
             if easy_central_difference:
                 dh[:, i] = (up1 - dn1) / (2 * eps)
             else:
                 up2 = wave(step(2))
-                if not bool(torch.all(self.start_inds == self.start_inds[0])):
-                    raise ValueError(
-                        "The user should decrease steps size (eps) because the frequency bins are changing during "
-                        "derivative calculation, which is not allowed."
-                    )
                 dn2 = wave(step(-2))
                 dh[:, i] = (-up2 + dn2 + 8 * (up1 - dn1)) / (12 * eps)
+            # every stencil point must put each binary on the same bins (the
+            # reference compares start bins across binaries instead,
+            # src/gbgpu/gbgpu.py:1092-1095, which only holds for identical ones)
+            if not all(bool(torch.equal(s, starts[0])) for s in starts[1:]):
+                raise ValueError(
+                    "The user should decrease steps size (eps) because the frequency bins are changing during "
+                    "derivative calculation, which is not allowed."
+                )
         fr = self.freqs.cpu().numpy()
         psd = torch.from_numpy(np.asarray(psd_func(fr, **psd_kwargs), dtype=np.float64)).to(dev)
         info = torch.zeros((num_bins, nd, nd), dtype=torch.float64, device=dev)

@@ -127,6 +127,35 @@ int gbgpu_wave_ll(double *loglike, gbgpu_cmplx *d_h, gbgpu_cmplx *h_h, const dou
 int gbgpu_swap_ll_diff(double *dll, double *terms, const double *params_pairs, int num_params, int num_prop,
                        const gbgpu_wave_cfg *cfg, const gbgpu_data *data, void *stream);
 
+/* ---- derivatives (FP64, 9-parameter sources) ------------------------------ */
+
+/* Parameter basis of every gradient and Fisher output ("sampling basis"):
+ *   0 ln amp, 1 f0, 2 fdot, 3 fddot, 4 phi0, 5 cos iota, 6 psi, 7 lambda,
+ *   8 sin beta,
+ * with f0, fdot and fddot independent (a tied fddot = 11/3 fdot^2/f0 follows
+ * by the chain rule on the host). */
+#define GBGPU_NGRAD 9
+
+/* Adjoint gradient of the fused likelihood (no counterpart in the reference,
+ * which has no derivatives beyond finite differences): loglike, d_h, h_h as
+ * gbgpu_wave_ll, and grad (num_bin, 9) row-major = d loglike / d theta in the
+ * sampling basis (for the phase-marginalised likelihood when
+ * phase_marginalize).  The backward pass pushes the residual back through the
+ * transform and recomputes the slow part; cfg->precision must be GBGPU_FP64. */
+int gbgpu_wave_ll_grad(double *loglike, double *grad, gbgpu_cmplx *d_h, gbgpu_cmplx *h_h, const double *params,
+                       int num_params, int num_bin, const gbgpu_wave_cfg *cfg, const gbgpu_data *data,
+                       double d_d, int phase_marginalize, void *stream);
+
+/* Replaces the finite-difference information_matrix
+ * (src/gbgpu/gbgpu.py:957-1138, F_ij = 4 df sum Re(conj(dh_i) dh_j)/S over
+ * A and E, :1119-1132) with exact forward tangents: fisher (num_bin, 9, 9)
+ * row-major in the sampling basis, against the PSD streams of `data`
+ * (data->A / data->E may be NULL).  With data, grad (num_bin, 9) and loglike
+ * (num_bin) are also available (either may be NULL; phase not marginalised).
+ * N <= 1024; cfg->precision must be GBGPU_FP64. */
+int gbgpu_fisher(double *fisher, double *grad, double *loglike, const double *params, int num_params, int num_bin,
+                 const gbgpu_wave_cfg *cfg, const gbgpu_data *data, double d_d, void *stream);
+
 /* ---- uneven-N batches -------------------------------------------------- */
 
 /* Configuration of a batch whose binaries carry their own N. */
@@ -171,7 +200,7 @@ int gbgpu_perturb_params(double *out, const double *in, int num_params, int num_
 /* Message describing the last failure on the calling thread. */
 const char *gbgpu_last_error(void);
 
-/* ABI version of this header (1). */
+/* ABI version of this header (2). */
 int gbgpu_abi_version(void);
 
 #ifdef __cplusplus
