@@ -210,6 +210,192 @@ def uneven_batch(gb, A, E, SA, SE, d_d, B, T, dt, n_calls, seed=77):
     return res
 
 
+def differentiation(gb, A, E, SA, SE, d_d, T, dt, steps, warmup, B=100_000, seed=3):
+    """Differentiation workload: 1e5 binaries, N = 256, 4 yr; adjoint
+    gradient (9 sampling-basis parameters) and per-binary 9x9 Fisher
+    matrices, timed separately; plus a Richardson central-difference check
+    of the gradient on 8 binaries."""
+    import torch
+
+    from gbgpu_amd import _capi
+    from gbgpu_amd.derivatives import FastGBLikelihood
+    from tests.helpers import draw_params
+
+    N = 256
+    rng = np.random.default_rng(seed)
+    p = draw_params(B, rng, f_lo=1e-3, f_hi=6e-3)
+    like = FastGBLikelihood(gb, [A, E], [SA, SE], N=N, T=T, dt=dt)
+    pd = torch.from_numpy(np.ascontiguousarray(p)).cuda()
+    print("[bench] differentiation", file=sys.stderr, flush=True)
+    fwd_cfg, _t = gb._cfg(N, like.T, False, "fp64")
+    ll = torch.empty(B, dtype=torch.float64, device="cuda")
+    lib = _capi.load()
+
+    def fwd():
+        _capi.check(lib.gbgpu_wave_ll(ll.data_ptr(), None, None, pd.data_ptr(), 9, B, fwd_cfg, like.data, like.d_d, 0,
+                                      _capi.stream_handle()), "wave_ll")
+
+    res = {"workload": "1e5 binaries, N=256, Tobs=4 yr, f0 U[1,6] mHz; gradient w.r.t. (ln amp, f0, fdot, fddot, "
+                       "phi0, cos iota, psi, lam, sin beta); 9x9 Fisher per binary", "num_bin": B}
+    for name, fn in (("forward", fwd), ("gradient", lambda: like.ll_and_grad(pd)), ("fisher", lambda: like.fisher(pd))):
+        for _ in range(warmup):
+            fn()
+        torch.cuda.synchronize()
+        with _capi.KernelTimer() as kt:
+            sec, _wall = time_calls(fn, steps, 0)
+        res[name] = {"per_s": B / sec, "ms_per_call": sec * 1e3, "kernel_ms": kt.mean_ms}
+    res["gradient_cost_vs_forward"] = res["forward"]["per_s"] / res["gradient"]["per_s"]
+    # central-difference check on 8 binaries (steps moving the phase ~1e-2 rad)
+    q = p[:, :8].copy()
+    _, g = gb.get_ll_grad(q, [A, E], [SA, SE], N=N, T=T, dt=dt)
+    F = gb.fisher_matrix(q, [SA, SE], N=N, T=T, dt=dt)
+    K = np.array([np.log(q[0]), q[1], q[2], q[3], q[4], np.cos(q[5]), q[6], q[7], np.sin(q[8])])
+    Ta = like.T
+    h = np.array([1e-3, 1e-2 / (2 * np.pi * Ta), 1e-2 / (np.pi * Ta**2), 1e-2 / (np.pi / 3 * Ta**3), 1e-2, 1e-3,
+                  1e-2, 1e-3, 1e-3])
+
+    def from_K(Kp):
+        return np.array([np.exp(Kp[0]), Kp[1], Kp[2], Kp[3], Kp[4], np.arccos(Kp[5]), Kp[6], Kp[7],
+                         np.arcsin(Kp[8])])
+
+    gfd = np.zeros_like(g)
+    for k in range(9):
+        D = []
+        for m in (1.0, 0.5, 0.25):
+            v = []
+            for sg in (1, -1):
+                Kp = K.copy()
+                Kp[k] = Kp[k] + sg * m * h[k]
+                v.append(gb.get_ll(from_K(Kp), [A, E], [SA, SE], N=N, T=T, dt=dt))
+            D.append((v[0] - v[1]) / (2 * m * h[k]))
+        gfd[:, k] = (16 * (4 * D[2] - D[1]) / 3 - (4 * D[1] - D[0]) / 3) / 15
+    scale = np.maximum(np.abs(gfd), np.sqrt(np.abs(np.diagonal(F, axis1=1, axis2=2))))
+    res["fd_check"] = {"n": 8, "max_rel_err": float(np.max(np.abs(g - gfd) / scale)),
+                       "stencil": "Richardson central differences, steps moving the phase ~1e-2 rad"}
+    return res
+
+
+def global_and_swap(gb, SA, SE, T, dt, B=10_000_000, P=100_000, chunk_samples=1 << 28, seed=11):
+    """Global template of 1e7 binaries (f0 log-uniform 0.1-10 mHz, N by the
+    frequency rule) summed deterministically into one 4-yr A/E pair, twice
+    (bit-identical check); then swap_likelihood_difference for 1e5 proposals
+    (remove binary, add a 1e-3 relative perturbed copy) against a noise
+    residual, circular and third-body (circular orbit: A2 U[0, 100] m/s,
+    P2 log-uniform 1-10 yr, phase U[0, 2 pi))."""
+    import torch
+
+    from gbgpu_amd import GBGPUThirdBody, _capi
+    from gbgpu_amd.noise import noise_realisation
+    from gbgpu_amd.utils.utility import baseline_N
+    from tests.helpers import draw_params
+
+    lib = _capi.load()
+    st = _capi.stream_handle()
+    rng = np.random.default_rng(seed)
+    p = draw_params(B, rng, f_lo=1e-4, f_hi=1e-2, log_f=True)
+    Nb = baseline_N(p[1])
+    Tadj = int(T / dt) * dt
+    L = SA.numel()
+    dev = SA.device
+    res = {"workload": "fill_global_template of 1e7 binaries (f0 log-uniform 0.1-10 mHz, N 128-1024 by frequency) "
+                       "into one 4-yr A/E pair; swap of 1e5 proposals, circular and third-body",
+           "num_bin": B, "N_fractions": {int(n): float(np.mean(Nb == n)) for n in np.unique(Nb)}}
+    buckets = [(int(n), np.flatnonzero(Nb == n)) for n in np.unique(Nb)]
+    maxN = max(n for n, _ in buckets)
+    C = max(1, chunk_samples // maxN)
+    aet = torch.empty((C, 3, maxN), dtype=torch.complex128, device=dev)
+    starts = torch.empty(C, dtype=torch.int64, device=dev)
+    gidx = torch.zeros(C, dtype=torch.int32, device=dev)
+    pd_all = torch.from_numpy(np.ascontiguousarray(p)).to(dev)
+
+    def generate():
+        total = torch.zeros((2, L), dtype=torch.complex128, device=dev)
+        fill_ms, fill_n, tmpl_bins = 0.0, 0, 0
+        for n, idx in buckets:
+            cfg, _tab = gb._cfg(n, Tadj, False, "fp64")
+            for c0 in range(0, len(idx), C):
+                sel = torch.from_numpy(idx[c0:c0 + C]).to(dev)
+                pc = pd_all[:, sel].contiguous()
+                m = pc.shape[1]
+                a = aet[:m].view(-1)[: m * 3 * n].view(m, 3, n)
+                _capi.check(lib.gbgpu_run_wave(None, a.data_ptr(), starts.data_ptr(), pc.data_ptr(), 9, m, cfg, st),
+                            "run_wave")
+                # the gather-sum adds into the (zeroed) global arrays; chunks and
+                # buckets always come in the same order, so the sum is reproducible
+                with _capi.KernelTimer() as kt:
+                    _capi.check(lib.gbgpu_fill_global_rows(total[0].data_ptr(), total[1].data_ptr(),
+                                                           a[:, 0].data_ptr(), a[:, 1].data_ptr(), 3 * n,
+                                                           starts.data_ptr(), 0, n, m, gidx.data_ptr(), L, 1, st),
+                                "fill_global_rows")
+                fill_ms += kt.total_ms
+                fill_n += kt.count
+                tmpl_bins += m * n
+        return total, fill_ms, tmpl_bins
+
+    print("[bench] global template 1e7", file=sys.stderr, flush=True)
+    generate()  # warm (twiddles, tables, allocator)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    tot1, fill_ms, tmpl_bins = generate()
+    torch.cuda.synchronize()
+    t_gen = time.perf_counter() - t0
+    tot2, _, _ = generate()
+    torch.cuda.synchronize()
+    # fill traffic per chunk: template A/E rows read (32 B/bin), start bins
+    # (8 B/row), and the touched part of the A/E output read and written
+    # (64 B/bin, counted once per bucket over the whole stream: an upper bound)
+    fill_bytes = 32.0 * tmpl_bins + 8.0 * B + 64.0 * L * len(buckets)
+    res["generate"] = {"seconds": t_gen, "binaries_per_s": B / t_gen}
+    res["fill"] = {"kernel_ms_total": fill_ms, "algorithmic_GB_per_s": fill_bytes / (fill_ms / 1e3) / 1e9,
+                   "hbm_peak_GB_per_s": HBM_PEAK_GBS, "template_bins": tmpl_bins}
+    res["bit_identical_repeat"] = bool(torch.equal(tot1, tot2))
+    del aet, tot2
+    torch.cuda.empty_cache()
+
+    # swap against a noise residual
+    df = 1.0 / Tadj
+    r_A = torch.from_numpy(noise_realisation(SA.cpu().numpy(), df, rng)).to(dev)
+    r_E = torch.from_numpy(noise_realisation(SE.cpu().numpy(), df, rng)).to(dev)
+    pr = p[:, :P].copy()
+    pa = pr * (1.0 + 1e-3 * rng.standard_normal(pr.shape))
+    NP = Nb[:P]
+    dat = _capi.Data(A=r_A.data_ptr(), E=r_E.data_ptr(), psdA=SA.data_ptr(), psdE=SE.data_ptr(), data_index=None,
+                     noise_index=None, data_length=L, start_freq_ind=0, df=df)
+    gb3 = GBGPUThirdBody()
+    A2 = rng.uniform(0, 100, P)
+    P2 = 10 ** rng.uniform(0, 1, P)
+    varpi = rng.uniform(0, 2 * np.pi, P)
+    extra = np.array(gb3.prepare_additional_args(A2, varpi, np.zeros(P), P2, np.zeros(P)))
+    for label, nparams in (("circular", 9), ("third_body", 14)):
+        print(f"[bench] swap {label}", file=sys.stderr, flush=True)
+        calls = []
+        for n in np.unique(NP):
+            sel = np.flatnonzero(NP == n)
+            q_r, q_a = pr[:, sel], pa[:, sel]
+            if nparams == 14:
+                q_r = np.concatenate([q_r, extra[:, sel]])
+                q_a = np.concatenate([q_a, extra[:, sel]])
+            pairs = np.empty((nparams, 2 * len(sel)))
+            pairs[:, 0::2] = q_r
+            pairs[:, 1::2] = q_a
+            cfg, tab = gb._cfg(int(n), Tadj, False, "fp64")
+            calls.append((torch.from_numpy(pairs).to(dev), len(sel), cfg, tab,
+                          torch.empty(len(sel), dtype=torch.float64, device=dev)))
+
+        def swap_all():
+            for pp, m, cfg, _tab, dll in calls:
+                _capi.check(lib.gbgpu_swap_ll_diff(dll.data_ptr(), None, pp.data_ptr(), nparams, m, cfg, dat, st),
+                            "swap")
+
+        swap_all()
+        torch.cuda.synchronize()
+        with _capi.KernelTimer() as kt:
+            sec, _ = time_calls(swap_all, 5, 0)
+        res[f"swap_{label}"] = {"proposals": P, "proposals_per_s": P / sec, "ms_per_pass": sec * 1e3,
+                                "kernel_ms_per_pass": kt.total_ms / 5}
+    return res
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -223,6 +409,8 @@ def main():
     ap.add_argument("--only", choices=["fp64", "mixed"], default=None, help="time one precision only")
     ap.add_argument("--no-uneven", action="store_true", help="skip the uneven-N batch")
     ap.add_argument("--uneven-calls", type=int, default=100, help="calls in the captured uneven-N loop")
+    ap.add_argument("--no-deriv", action="store_true", help="skip the differentiation workload")
+    ap.add_argument("--no-global", action="store_true", help="skip the global-template and swap workload")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build measures one GPU only; --gpus must be 1"}))
@@ -268,11 +456,16 @@ def main():
     if args.only == "mixed":
         sec64, wall64 = float("nan"), float("nan")
     else:
-        sec64, wall64 = time_calls(call64, args.steps, args.warmup)
+        for _ in range(args.warmup):
+            call64()
+        torch.cuda.synchronize()
+        with _capi.KernelTimer() as kt64:
+            sec64, wall64 = time_calls(call64, args.steps, 0)
     par64 = parity_sample(gb, params, A, E, SA, SE, d_d, df, N, T, dt, "fp64") if args.only is None else None
     value = B / sec64
     flops = flop_per_binary(N) * B
-    achieved_tflops = flops / sec64 / 1e12
+    kernel_s = kt64.mean_ms / 1e3 if args.only != "mixed" else float("nan")
+    achieved_tflops = flops / kernel_s / 1e12
 
     mixed = None
     if args.only == "fp64":
@@ -292,6 +485,14 @@ def main():
     uneven = None
     if not args.no_uneven:
         uneven = uneven_batch(gb, A, E, SA, SE, d_d, args.num_bin, T, dt, args.uneven_calls)
+
+    deriv = None
+    if not args.no_deriv:
+        deriv = differentiation(gb, A, E, SA, SE, d_d, T, dt, args.steps, args.warmup)
+
+    glob = None
+    if not args.no_global:
+        glob = global_and_swap(gb, SA, SE, T, dt)
 
     cpu = None
     if not args.no_cpu:
@@ -327,14 +528,17 @@ def main():
             "unit": "TFLOP/s",
             "frac": achieved_tflops / FP64_PEAK_TFLOPS,
             "traffic": None,
-            "kernel": "k_wave_f64<256, MODE_LL>",
+            "kernel": "k_wave<256, MODE_LL, circular, F64>",
+            "kernel_ms": kernel_s * 1e3,
             "flop_per_binary": flop_per_binary(N),
-            "algorithmic_hbm_GBs": bytes_per_binary(N) * B / sec64 / 1e9,
+            "algorithmic_hbm_GBs": bytes_per_binary(N) * B / kernel_s / 1e9,
         },
         "cpu_baseline": cpu,
         "parity": par64,
         "mixed": mixed,
         "uneven_batch": uneven,
+        "differentiation": deriv,
+        "global_template_and_swap": glob,
         "wall_ms_per_step": wall64 * 1e3,
     }
     print(json.dumps(out))

@@ -116,6 +116,8 @@ SIGNATURES = {
     ),
     "gbgpu_last_error": (ctypes.c_char_p, []),
     "gbgpu_abi_version": (c_int, []),
+    "gbgpu_kernel_timing": (c_int, [c_int]),
+    "gbgpu_kernel_timing_read": (c_int, [ctypes.POINTER(c_double), ctypes.POINTER(c_int)]),
 }
 
 _lib = None
@@ -159,3 +161,28 @@ def stream_handle(device=None) -> int:
     import torch
 
     return torch.cuda.current_stream(device).cuda_stream
+
+
+class KernelTimer:
+    """Context manager summing the library's dominant-kernel durations
+    (HIP events on the launch stream) over the calls made inside it."""
+
+    def __enter__(self):
+        check(load().gbgpu_kernel_timing(1), "gbgpu_kernel_timing")
+        self.total_ms = 0.0
+        self.count = 0
+        return self
+
+    def __exit__(self, *exc):
+        t = c_double(0.0)
+        n = c_int(0)
+        rc = load().gbgpu_kernel_timing_read(ctypes.byref(t), ctypes.byref(n))
+        load().gbgpu_kernel_timing(0)
+        check(rc, "gbgpu_kernel_timing_read")
+        self.total_ms, self.count = t.value, n.value
+        return False
+
+    @property
+    def mean_ms(self):
+        return self.total_ms / self.count if self.count else float("nan")
+

@@ -379,6 +379,7 @@ static int launch_grad(int N, WaveArgs a, double *grad, hipStream_t st) {
     case NN: {                                                                                                \
         const long long grid = (a.B + Geo<NN>::BPW - 1) / Geo<NN>::BPW;                                       \
         return with_pack<F64>(a, NN, st, [&](const WaveArgs &aa) -> int {                                     \
+            KernelTimer kt(st);                                                                               \
             hipLaunchKernelGGL((k_grad<NN>), dim3((unsigned)grid), dim3(WG_SIZE), 0, st, aa, grad);           \
             GBGPU_CHECK_LAUNCH("k_grad");                                                                     \
             return GBGPU_OK;                                                                                  \
@@ -408,6 +409,7 @@ static int launch_fisher_n(WaveArgs a, double *fisher, double *grad, hipStream_t
         // one binary per workgroup, in the caller's order (the pack is indexed by binary)
         WaveArgs a2 = aa;
         a2.order = nullptr;
+        KernelTimer kt(st);
         hipLaunchKernelGGL((k_fisher<N>), dim3((unsigned)a2.B), dim3(256), lds, st, a2, fisher, grad);
         GBGPU_CHECK_LAUNCH("k_fisher");
         return GBGPU_OK;

@@ -265,7 +265,11 @@ static int launch_batch(BatchArgs ba, void *workspace, hipStream_t st) {
     }
     const long long grid = (long long)blocks_per_cu[pi] * num_cu;
     void *kargs[] = {&ba};
-    hipError_t e = hipLaunchKernel(BatchKernel<P>::fn(), dim3((unsigned)grid), dim3(WG_SIZE), kargs, lds, st);
+    hipError_t e;
+    {
+        KernelTimer kt(st);
+        e = hipLaunchKernel(BatchKernel<P>::fn(), dim3((unsigned)grid), dim3(WG_SIZE), kargs, lds, st);
+    }
     if (e == hipSuccess) e = hipGetLastError();
     hipError_t e2 = own ? hipFreeAsync(ws, st) : hipSuccess;
     if (e != hipSuccess) return hip_fail(e, "k_batch_ll");

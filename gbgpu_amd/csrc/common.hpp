@@ -68,6 +68,25 @@ const float2 *twiddles_f32(int N);
 
 inline bool is_pow2(long long n) { return n > 0 && (n & (n - 1)) == 0; }
 
+// Optional timing of the dominant kernel of each call (gbgpu_kernel_timing):
+// HIP events recorded on the launch stream immediately around the launch.
+bool timing_enabled();
+void timing_push(hipEvent_t e0, hipEvent_t e1);
+
+struct KernelTimer {
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    hipStream_t st;
+    explicit KernelTimer(hipStream_t s) : st(s) {
+        if (!timing_enabled()) return;
+        if (hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess ||
+            hipEventRecord(e0, st) != hipSuccess)
+            e0 = nullptr;
+    }
+    ~KernelTimer() {
+        if (e0 && hipEventRecord(e1, st) == hipSuccess) timing_push(e0, e1);
+    }
+};
+
 }  // namespace gbgpu
 
 #define GBGPU_CHECK_LAUNCH(where)                                   \

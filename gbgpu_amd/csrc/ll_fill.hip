@@ -203,8 +203,11 @@ static int fill_rows_impl(double2 *Aglob, double2 *Eglob, const double2 *rowsA, 
     }
     const long long tpg = (L + 63) / 64;
     const long long nt = tpg * G;
-    hipLaunchKernelGGL(k_gather_sum, dim3((unsigned)((nt + 3) / 4)), dim3(256), 0, st, Aglob, Eglob, rowsA, rowsE,
-                       row_stride, keys_sorted, perm, B, M, L, tpg, nt);
+    {
+        KernelTimer kt(st);
+        hipLaunchKernelGGL(k_gather_sum, dim3((unsigned)((nt + 3) / 4)), dim3(256), 0, st, Aglob, Eglob, rowsA,
+                           rowsE, row_stride, keys_sorted, perm, B, M, L, tpg, nt);
+    }
     e = hipGetLastError();
     hipError_t e2 = hipFreeAsync(ws, st);
     if (e != hipSuccess) return hip_fail(e, "k_gather_sum");
@@ -232,6 +235,7 @@ extern "C" int gbgpu_get_ll(gbgpu_cmplx *d_h, gbgpu_cmplx *h_h, const gbgpu_cmpl
     }
     hipStream_t st = static_cast<hipStream_t>(stream);
     const long long waves = ((long long)num_bin + 7) / 8;
+    KernelTimer kt(st);
     hipLaunchKernelGGL(k_get_ll, dim3((unsigned)((waves + 3) / 4)), dim3(256), 0, st, (double2 *)d_h, (double2 *)h_h,
                        (const double2 *)A_template, (const double2 *)E_template, (const double2 *)A_data,
                        (const double2 *)E_data, A_psd, E_psd, 4.0 * df, start_ind, M, (long long)num_bin,

@@ -53,8 +53,59 @@ const C *get_twiddles(std::map<std::pair<int, int>, void *> &cache, int N) {
 const double2 *twiddles_f64(int N) { return get_twiddles<double2>(g_tw64, N); }
 const float2 *twiddles_f32(int N) { return get_twiddles<float2>(g_tw32, N); }
 
+namespace {
+std::mutex g_timing_mutex;
+bool g_timing = false;
+std::vector<std::pair<hipEvent_t, hipEvent_t>> g_events;
+}  // namespace
+
+bool timing_enabled() {
+    std::lock_guard<std::mutex> lock(g_timing_mutex);
+    return g_timing;
+}
+
+void timing_push(hipEvent_t e0, hipEvent_t e1) {
+    std::lock_guard<std::mutex> lock(g_timing_mutex);
+    g_events.emplace_back(e0, e1);
+}
+
 }  // namespace gbgpu
 
-extern "C" const char *gbgpu_last_error(void) { return gbgpu::g_last_error.c_str(); }
+using namespace gbgpu;
+
+extern "C" int gbgpu_kernel_timing(int enable) {
+    std::lock_guard<std::mutex> lock(g_timing_mutex);
+    for (auto &ev : g_events) {
+        (void)hipEventDestroy(ev.first);
+        (void)hipEventDestroy(ev.second);
+    }
+    g_events.clear();
+    g_timing = enable != 0;
+    return GBGPU_OK;
+}
+
+extern "C" int gbgpu_kernel_timing_read(double *total_ms, int *count) {
+    std::vector<std::pair<hipEvent_t, hipEvent_t>> evs;
+    {
+        std::lock_guard<std::mutex> lock(g_timing_mutex);
+        evs.swap(g_events);
+    }
+    double tot = 0.0;
+    int rc = GBGPU_OK;
+    for (auto &ev : evs) {
+        float ms = 0.0f;
+        hipError_t e = hipEventSynchronize(ev.second);
+        if (e == hipSuccess) e = hipEventElapsedTime(&ms, ev.first, ev.second);
+        if (e != hipSuccess && rc == GBGPU_OK) rc = hip_fail(e, "kernel timing");
+        tot += ms;
+        (void)hipEventDestroy(ev.first);
+        (void)hipEventDestroy(ev.second);
+    }
+    if (total_ms) *total_ms = tot;
+    if (count) *count = (int)evs.size();
+    return rc;
+}
+
+extern "C" const char *gbgpu_last_error(void) { return g_last_error.c_str(); }
 
 extern "C" int gbgpu_abi_version(void) { return 2; }

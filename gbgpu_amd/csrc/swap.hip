@@ -139,6 +139,7 @@ static int launch_swap(int N, WaveArgs a, double *dll, double *terms, hipStream_
     case NN: {                                                                                             \
         if constexpr (!THIRD || NN <= 1024) {                                                              \
             const long long grid = (a.B + SwapGeo<NN>::BPW - 1) / SwapGeo<NN>::BPW;                        \
+            KernelTimer kt(st);                                                                            \
             hipLaunchKernelGGL((k_swap<NN, THIRD, P>), dim3((unsigned)grid), dim3(SwapGeo<NN>::WG), 0, st, \
                                a, dll, terms);                                                             \
         } else {                                                                                           \

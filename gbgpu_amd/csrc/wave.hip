@@ -111,6 +111,7 @@ static int launch_wave(int N, const WaveArgs &a, hipStream_t st) {
         if constexpr (fits_lds<NN, MODE, THIRD, P>()) {                                             \
             const long long grid = (a.B + Geo<NN>::BPW - 1) / Geo<NN>::BPW;                         \
             return with_pack<P>(a, NN, st, [&](const WaveArgs &aa) -> int {                         \
+                KernelTimer kt(st);                                                                 \
                 hipLaunchKernelGGL((k_wave<NN, MODE, THIRD, P>), dim3((unsigned)grid), dim3(WG_SIZE), \
                                    0, st, aa);                                                      \
                 GBGPU_CHECK_LAUNCH("k_wave");                                                       \

@@ -203,6 +203,18 @@ const char *gbgpu_last_error(void);
 /* ABI version of this header (2). */
 int gbgpu_abi_version(void);
 
+/* Measurement hook with no counterpart in the reference: while enabled,
+ * every call records a pair of HIP events on its stream immediately around
+ * the launch of its dominant kernel (the waveform, likelihood, gradient,
+ * Fisher, persistent batch, gather-sum or swap kernel).  Enabling or
+ * disabling drops the pairs recorded so far.  Not for use under graph
+ * capture. */
+int gbgpu_kernel_timing(int enable);
+
+/* Waits for the recorded pairs, returns their summed duration (ms) and
+ * count, and drops them. */
+int gbgpu_kernel_timing_read(double *total_ms, int *count);
+
 #ifdef __cplusplus
 }
 #endif
