@@ -172,24 +172,29 @@ def uneven_batch(gb, A, E, SA, SE, d_d, B, T, dt, n_calls, seed=77):
            "mean_N": float(N.mean())}
     lls = {}
     for prec in ("fp64", "mixed"):
-        print(f"[bench] uneven batch {prec}", file=sys.stderr, flush=True)
-        bl = BatchedLogLike(gb, [A, E], [SA, SE], T, dt, precision=prec)
-        g, out = bl.capture_loop(p, n_calls, sigma=1e-3, seed=seed)
-        g.replay()
-        torch.cuda.synchronize()
-        ev0 = torch.cuda.Event(enable_timing=True)
-        ev1 = torch.cuda.Event(enable_timing=True)
-        reps = 2
-        ev0.record()
-        for _ in range(reps):
+        for strat in ("buckets", "persistent"):
+            print(f"[bench] uneven batch {prec} {strat}", file=sys.stderr, flush=True)
+            bl = BatchedLogLike(gb, [A, E], [SA, SE], T, dt, precision=prec, strategy=strat)
+            g, out = bl.capture_loop(p, n_calls, sigma=1e-3, seed=seed)
             g.replay()
-        ev1.record()
-        torch.cuda.synchronize()
-        sec = ev0.elapsed_time(ev1) / 1e3 / reps
-        res[prec] = {"binaries_per_s": n_calls * B / sec, "ms_per_call": sec / n_calls * 1e3}
-        lls[prec] = out.clone()
-        del g, out, bl
-        torch.cuda.empty_cache()
+            torch.cuda.synchronize()
+            ev0 = torch.cuda.Event(enable_timing=True)
+            ev1 = torch.cuda.Event(enable_timing=True)
+            reps = 2
+            ev0.record()
+            for _ in range(reps):
+                g.replay()
+            ev1.record()
+            torch.cuda.synchronize()
+            sec = ev0.elapsed_time(ev1) / 1e3 / reps
+            r = {"binaries_per_s": n_calls * B / sec, "ms_per_call": sec / n_calls * 1e3}
+            if strat == "buckets":
+                res[prec] = r
+                lls[prec] = out.clone()
+            else:
+                res[prec]["persistent_strategy"] = r
+            del g, out, bl
+            torch.cuda.empty_cache()
     res["speedup_mixed_vs_fp64"] = res["mixed"]["binaries_per_s"] / res["fp64"]["binaries_per_s"]
     d = (lls["mixed"] - lls["fp64"]).abs()
     res["logL_abs_err_mixed_vs_fp64"] = {"max": float(d.max()), "median": float(d.median()),

@@ -17,6 +17,8 @@ LIB_PATH = Path(__file__).resolve().parent / "libgbgpu_hip.so"
 
 GBGPU_FP64 = 0
 GBGPU_MIXED = 1
+GBGPU_BATCH_BUCKETS = 0
+GBGPU_BATCH_PERSISTENT = 1
 ORBIT_ROWS = 22
 
 c_void_p = ctypes.c_void_p
@@ -63,6 +65,7 @@ class BatchCfg(ctypes.Structure):
         ("tdi2", c_int),
         ("precision", c_int),
         ("orbit_tables", c_void_p * 7),
+        ("strategy", c_int),
     ]
 
 

@@ -2,8 +2,8 @@
 
 ``BatchedLogLike`` keeps the data, PSD, per-N orbit tables and the library
 workspace resident on the GPU and evaluates the fused waveform + likelihood
-for a batch in which every binary has its own N (one persistent launch,
-``gbgpu_wave_ll_batched``).  ``run_loop`` evaluates a sequence of calls whose
+for a batch in which every binary has its own N (``gbgpu_wave_ll_batched``:
+device-side N bucketing, then one launch per bucket or one persistent launch).  ``run_loop`` evaluates a sequence of calls whose
 parameters are seeded Gaussian relative jumps of a base point
 (``gbgpu_perturb_params``; N is re-derived from the jumped f0 by the step
 rule), either eagerly or as one captured hipGraph (``capture_loop``).
@@ -35,10 +35,14 @@ class BatchedLogLike:
         precision: "fp64" or "mixed".
         N_levels, f_edges: the N step rule used when N is derived from f0:
             N_levels[i] for f0 < f_edges[i], the last level above.
+        strategy: "buckets" (one launch of the uniform kernel per N bucket)
+            or "persistent" (one persistent launch over all buckets).
     """
 
+    STRATEGIES = {"buckets": _capi.GBGPU_BATCH_BUCKETS, "persistent": _capi.GBGPU_BATCH_PERSISTENT}
+
     def __init__(self, gb, data, psd, T, dt, precision="mixed", start_freq_ind=0, N_levels=DEFAULT_N_LEVELS,
-                 f_edges=DEFAULT_F_EDGES, phase_marginalize=False):
+                 f_edges=DEFAULT_F_EDGES, phase_marginalize=False, strategy="buckets"):
         torch = _torch()
         if gb.d_d is None:
             raise ValueError("gb.d_d must be set before building a likelihood")
@@ -56,8 +60,10 @@ class BatchedLogLike:
         self.SA = gb._to_dev(psd[0], np.float64).reshape(-1)
         self.SE = gb._to_dev(psd[1], np.float64).reshape(-1)
         self.L = int(self.A.numel())
+        if strategy not in self.STRATEGIES:
+            raise ValueError(f"strategy must be one of {list(self.STRATEGIES)}")
         self.cfg = _capi.BatchCfg(T=float(self.T), armlength=float(gb.orbits.armlength), tdi2=0,
-                                  precision=1 if precision == "mixed" else 0)
+                                  precision=1 if precision == "mixed" else 0, strategy=self.STRATEGIES[strategy])
         self._tabs = {}
         for N in self.N_levels:
             self._add_table(int(N))

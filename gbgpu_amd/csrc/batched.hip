@@ -1,22 +1,28 @@
-// Uneven-N batches in one persistent launch (gfx950).
+// Uneven-N batches (gfx950).
 //
 // The reference evaluates one N per call (src/gbgpu/gbgpu.py:205-207 takes
 // N.max(); the native get_ll takes a scalar M, gbgpu_utils.cu:126), so an
 // uneven batch costs one call per N.  Here every binary carries its own N
-// (a power of two in [32, 2048]) and a single call does:
-//   k_batch_reset    clear the control block (no memset: graph-capture safe)
-//   k_batch_setup    per-binary constants (Pack) and a histogram of N buckets
-//   k_batch_scan     per-bucket list offsets and work-item offsets, largest N
-//                    first so the longest items start earliest
-//   k_batch_scatter  binaries into per-bucket lists
-//   k_batch_ll       persistent workgroups pull work items (BPW binaries of
-//                    one bucket) from an atomic queue and run the fused
-//                    waveform + likelihood for that bucket's N.
-// All scratch lives in a caller-provided workspace, so the sequence can be
-// captured in a hipGraph and replayed (the per-N orbit tables and twiddles
-// must have been created by an earlier, uncaptured call).
-// Which workgroup runs a binary depends on scheduling; each binary's result
-// does not, so results are deterministic.
+// (a power of two in [32, 2048]) and one call does, on the device:
+//   k_batch_reset    clear the control block (a fill kernel, no memset)
+//   k_batch_setup    per-binary constants (Pack) and a histogram over
+//                    (N bucket, log-spaced start-bin cell)
+//   k_batch_scan     offsets of every histogram cell (one workgroup), and
+//                    per-bucket counts and list offsets
+//   k_batch_scatter  binaries into one list ordered by bucket, then by
+//                    start-bin cell (neighbouring binaries read
+//                    neighbouring data windows, which stay in L2)
+// and then either
+//   strategy 0: one launch of the uniform fused kernel per N bucket, each a
+//               grid-stride loop over its segment of the list, reading the
+//               segment's size and offset from the control block; or
+//   strategy 1: one persistent launch whose workgroups pull work items (BPW
+//               binaries of one bucket, largest N first) from an atomic queue.
+// All scratch lives in a caller-provided workspace and nothing depends on a
+// host-side count, so the sequence can be captured in a hipGraph and
+// replayed (the per-N orbit tables and twiddles must have been created by an
+// earlier, uncaptured call).  Which workgroup runs a binary depends on
+// scheduling; each binary's result does not, so results are deterministic.
 #include "common.hpp"
 #include "wave_common.hpp"
 
@@ -25,9 +31,13 @@ namespace gbgpu {
 int check_data(const gbgpu_data *data);
 void set_data(WaveArgs &a, const gbgpu_data *data);
 
+int launch_ll_segment(int N, int precision, const WaveArgs &a, hipStream_t st);
+
 constexpr int NBUCKET = 7;  // N = 32 << k, k < 7
-enum { CTL_COUNT = 0, CTL_CURSOR = 8, CTL_LISTOFF = 16, CTL_ITEMOFF = 24, CTL_ITEMS = 32, CTL_TOTAL = 40,
-       CTL_WORK = 41, CTL_WORDS = 48 };
+constexpr int NCB = 4096;   // start-bin cells per bucket (log-spaced)
+constexpr int NCELL = NBUCKET * NCB;
+enum { CTL_COUNT = 0, CTL_LISTOFF = 16, CTL_ITEMOFF = 24, CTL_ITEMS = 32, CTL_TOTAL = 40, CTL_WORK = 41,
+       CTL_HIST = 64, CTL_CUR = CTL_HIST + NCELL, CTL_WORDS = CTL_CUR + NCELL };
 
 __host__ __device__ constexpr int bucket_bpw(int k) {
     return WG_SIZE / (((32 << k) / 4 < WG_SIZE) ? (32 << k) / 4 : WG_SIZE);
@@ -44,11 +54,27 @@ struct BatchArgs {
     const void *tws[NBUCKET];
     const int *Nb;
     int *list;
+    int *cell;  // histogram cell per binary (-1: not evaluated)
     int *ctl;
 };
 
 __global__ void k_batch_reset(int *ctl) {
-    for (int i = threadIdx.x; i < CTL_WORDS; i += blockDim.x) ctl[i] = 0;
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < CTL_WORDS) ctl[i] = 0;
+}
+
+// Histogram cell of a binary: bucket k, then its start bin within the data
+// on a logarithmic grid of NCB cells.  Galactic binaries crowd at low
+// frequency; log-spaced cells keep the per-cell counts (and so the atomic
+// contention on each counter) even, while neighbouring binaries still land
+// in neighbouring cells.
+__device__ __forceinline__ int batch_cell(int k, long long start, const WaveArgs &a) {
+    long long j = start - a.start_freq_ind;
+    j = j < 0 ? 0 : (j >= a.L ? a.L - 1 : j);
+    const float x = __log2f(1.0f + (float)j) / __log2f(1.0f + (float)a.L);
+    int c = (int)(x * (float)NCB);
+    c = c < 0 ? 0 : (c >= NCB ? NCB - 1 : c);
+    return k * NCB + c;
 }
 
 template <class P>
@@ -63,37 +89,76 @@ __global__ void __launch_bounds__(256) k_batch_setup(BatchArgs ba, Pack<P> *__re
         ba.a.loglike[b] = nan;
         if (ba.a.d_h) ba.a.d_h[b] = make_double2(nan, nan);
         if (ba.a.h_h) ba.a.h_h[b] = make_double2(nan, nan);
+        ba.cell[b] = -1;
         return;
     }
     Pack<P> pk;
     pk.s = make_src(ba.a.params, ba.a.num_params, ba.a.B, b, N, ba.a.T, ba.a.tdi2, ba.a.armlength);
     pk.aux = P::make_aux(pk.s);
     pack[b] = pk;
-    atomicAdd(&ba.ctl[CTL_COUNT + k], 1);
+    const int cell = batch_cell(k, pk.s.start, ba.a);
+    ba.cell[b] = cell;
+    atomicAdd(&ba.ctl[CTL_HIST + cell], 1);
 }
 
-__global__ void k_batch_scan(int *ctl) {
-    if (threadIdx.x != 0) return;
-    int list_run = 0, item_run = 0;
-    for (int k = NBUCKET - 1; k >= 0; k--) {
-        const int c = ctl[CTL_COUNT + k];
-        const int bpw = bucket_bpw(k);
-        ctl[CTL_LISTOFF + k] = list_run;
-        ctl[CTL_ITEMOFF + k] = item_run;
-        ctl[CTL_ITEMS + k] = (c + bpw - 1) / bpw;
-        list_run += c;
-        item_run += (c + bpw - 1) / bpw;
+// Exclusive scan of the histogram (one workgroup of 1024 threads), the
+// scatter cursors, per-bucket counts and list offsets (ascending bucket), and
+// the persistent strategy's work-item table (descending bucket).
+__global__ void __launch_bounds__(1024) k_batch_scan(int *ctl) {
+    __shared__ int part[1024];
+    __shared__ int s_total;
+    constexpr int PER = (NCELL + 1023) / 1024;
+    const int t = threadIdx.x;
+    const int lo = t * PER, hi = (t + 1) * PER < NCELL ? (t + 1) * PER : NCELL;
+    int sum = 0;
+    for (int i = lo; i < hi; i++) sum += ctl[CTL_HIST + i];
+    part[t] = sum;
+    __syncthreads();
+    if (t == 0) {
+        int run = 0;
+        for (int i = 0; i < 1024; i++) {
+            const int v = part[i];
+            part[i] = run;
+            run += v;
+        }
+        s_total = run;
     }
-    ctl[CTL_TOTAL] = item_run;
+    __syncthreads();
+    int run = part[t];
+    for (int i = lo; i < hi; i++) {
+        const int v = ctl[CTL_HIST + i];
+        ctl[CTL_HIST + i] = run;  // cell offset
+        ctl[CTL_CUR + i] = run;   // scatter cursor
+        run += v;
+    }
+    __syncthreads();
+    if (t == 0) {
+        // bucket k spans [offset of cell k NCB, offset of cell (k+1) NCB)
+        for (int k = 0; k < NBUCKET; k++) {
+            const int o0 = ctl[CTL_HIST + k * NCB];
+            const int o1 = k + 1 < NBUCKET ? ctl[CTL_HIST + (k + 1) * NCB] : s_total;
+            ctl[CTL_LISTOFF + k] = o0;
+            ctl[CTL_COUNT + k] = o1 - o0;
+        }
+        int item_run = 0;
+        for (int k = NBUCKET - 1; k >= 0; k--) {
+            const int c = ctl[CTL_COUNT + k];
+            const int bpw = bucket_bpw(k);
+            ctl[CTL_ITEMOFF + k] = item_run;
+            ctl[CTL_ITEMS + k] = (c + bpw - 1) / bpw;
+            item_run += (c + bpw - 1) / bpw;
+        }
+        ctl[CTL_TOTAL] = item_run;
+    }
 }
 
 __global__ void k_batch_scatter(BatchArgs ba) {
     const long long b = (long long)blockIdx.x * blockDim.x + threadIdx.x;
     if (b >= ba.a.B) return;
-    const int k = bucket_of(ba.Nb[b]);
-    if (k < 0 || ba.tabs[k] == nullptr) return;
-    const int pos = atomicAdd(&ba.ctl[CTL_CURSOR + k], 1);
-    ba.list[ba.ctl[CTL_LISTOFF + k] + pos] = (int)b;
+    const int cell = ba.cell[b];
+    if (cell < 0) return;
+    const int pos = atomicAdd(&ba.ctl[CTL_CUR + cell], 1);
+    ba.list[pos] = (int)b;
 }
 
 template <int N, class P>
@@ -223,30 +288,11 @@ struct BatchKernel<Mixed> {
 template <class P>
 size_t batch_ws_bytes(long long B) {
     auto al = [](size_t n) { return (n + 255) & ~(size_t)255; };
-    return al(sizeof(Pack<P>) * (size_t)B) + al(sizeof(int) * (size_t)B) + al(sizeof(int) * CTL_WORDS);
+    return al(sizeof(Pack<P>) * (size_t)B) + 2 * al(sizeof(int) * (size_t)B) + al(sizeof(int) * CTL_WORDS);
 }
 
 template <class P>
-static int launch_batch(BatchArgs ba, void *workspace, hipStream_t st) {
-    const long long B = ba.a.B;
-    char *ws = static_cast<char *>(workspace);
-    bool own = false;
-    const size_t bytes = batch_ws_bytes<P>(B);
-    if (!ws) {
-        hipError_t e = hipMallocAsync((void **)&ws, bytes, st);
-        if (e != hipSuccess) return hip_fail(e, "hipMallocAsync(batch workspace)");
-        own = true;
-    }
-    auto al = [](size_t n) { return (n + 255) & ~(size_t)255; };
-    Pack<P> *pack = reinterpret_cast<Pack<P> *>(ws);
-    ba.list = reinterpret_cast<int *>(ws + al(sizeof(Pack<P>) * (size_t)B));
-    ba.ctl = reinterpret_cast<int *>(ws + al(sizeof(Pack<P>) * (size_t)B) + al(sizeof(int) * (size_t)B));
-    ba.a.pack = pack;
-    const unsigned gb = (unsigned)((B + 255) / 256);
-    hipLaunchKernelGGL(k_batch_reset, dim3(1), dim3(64), 0, st, ba.ctl);
-    hipLaunchKernelGGL((k_batch_setup<P>), dim3(gb), dim3(256), 0, st, ba, pack);
-    hipLaunchKernelGGL(k_batch_scan, dim3(1), dim3(64), 0, st, ba.ctl);
-    hipLaunchKernelGGL(k_batch_scatter, dim3(gb), dim3(256), 0, st, ba);
+static int launch_persistent(BatchArgs &ba, hipStream_t st) {
     // persistent grid: as many workgroups as can be resident at once
     static int blocks_per_cu[2] = {0, 0};
     static int num_cu = 0;
@@ -271,8 +317,57 @@ static int launch_batch(BatchArgs ba, void *workspace, hipStream_t st) {
         e = hipLaunchKernel(BatchKernel<P>::fn(), dim3((unsigned)grid), dim3(WG_SIZE), kargs, lds, st);
     }
     if (e == hipSuccess) e = hipGetLastError();
-    hipError_t e2 = own ? hipFreeAsync(ws, st) : hipSuccess;
     if (e != hipSuccess) return hip_fail(e, "k_batch_ll");
+    return GBGPU_OK;
+}
+
+// One launch of the uniform fused kernel per populated bucket.
+template <class P>
+static int launch_buckets(const BatchArgs &ba, int precision, hipStream_t st) {
+    for (int k = NBUCKET - 1; k >= 0; k--) {
+        if (!ba.tabs[k]) continue;
+        const int N = 32 << k;
+        WaveArgs a = ba.a;
+        a.tab = ba.tabs[k];
+        a.tw = ba.tws[k];
+        a.scale = 0.5 * a.T / N;
+        a.order = ba.list;
+        a.seg_count = ba.ctl + CTL_COUNT + k;
+        a.seg_off = ba.ctl + CTL_LISTOFF + k;
+        const int rc = launch_ll_segment(N, precision, a, st);
+        if (rc) return rc;
+    }
+    return GBGPU_OK;
+}
+
+template <class P>
+static int launch_batch(BatchArgs ba, int precision, int strategy, void *workspace, hipStream_t st) {
+    const long long B = ba.a.B;
+    char *ws = static_cast<char *>(workspace);
+    bool own = false;
+    const size_t bytes = batch_ws_bytes<P>(B);
+    if (!ws) {
+        hipError_t e = hipMallocAsync((void **)&ws, bytes, st);
+        if (e != hipSuccess) return hip_fail(e, "hipMallocAsync(batch workspace)");
+        own = true;
+    }
+    auto al = [](size_t n) { return (n + 255) & ~(size_t)255; };
+    Pack<P> *pack = reinterpret_cast<Pack<P> *>(ws);
+    ba.list = reinterpret_cast<int *>(ws + al(sizeof(Pack<P>) * (size_t)B));
+    ba.cell = reinterpret_cast<int *>(ws + al(sizeof(Pack<P>) * (size_t)B) + al(sizeof(int) * (size_t)B));
+    ba.ctl = reinterpret_cast<int *>(ws + al(sizeof(Pack<P>) * (size_t)B) + 2 * al(sizeof(int) * (size_t)B));
+    ba.a.pack = pack;
+    const unsigned gb = (unsigned)((B + 255) / 256);
+    hipLaunchKernelGGL(k_batch_reset, dim3((CTL_WORDS + 255) / 256), dim3(256), 0, st, ba.ctl);
+    hipLaunchKernelGGL((k_batch_setup<P>), dim3(gb), dim3(256), 0, st, ba, pack);
+    hipLaunchKernelGGL(k_batch_scan, dim3(1), dim3(1024), 0, st, ba.ctl);
+    hipLaunchKernelGGL(k_batch_scatter, dim3(gb), dim3(256), 0, st, ba);
+    hipError_t e = hipGetLastError();
+    int rc = e == hipSuccess ? GBGPU_OK : hip_fail(e, "batch set-up");
+    if (!rc) rc = strategy == GBGPU_BATCH_PERSISTENT ? launch_persistent<P>(ba, st)
+                                                     : launch_buckets<P>(ba, precision, st);
+    hipError_t e2 = own ? hipFreeAsync(ws, st) : hipSuccess;
+    if (rc) return rc;
     if (e2 != hipSuccess) return hip_fail(e2, "hipFreeAsync(batch workspace)");
     return GBGPU_OK;
 }
@@ -371,8 +466,13 @@ extern "C" int gbgpu_wave_ll_batched(double *loglike, gbgpu_cmplx *d_h, gbgpu_cm
             if (!ba.tws[k]) return hip_fail(hipErrorOutOfMemory, "twiddles");
         }
     }
+    if (cfg->strategy != GBGPU_BATCH_BUCKETS && cfg->strategy != GBGPU_BATCH_PERSISTENT) {
+        set_error("unknown batch strategy");
+        return GBGPU_E_ARG;
+    }
     hipStream_t st = static_cast<hipStream_t>(stream);
-    return cfg->precision == GBGPU_MIXED ? launch_batch<Mixed>(ba, workspace, st) : launch_batch<F64>(ba, workspace, st);
+    return cfg->precision == GBGPU_MIXED ? launch_batch<Mixed>(ba, cfg->precision, cfg->strategy, workspace, st)
+                                         : launch_batch<F64>(ba, cfg->precision, cfg->strategy, workspace, st);
 }
 
 extern "C" int gbgpu_perturb_params(double *out, const double *in, int num_params, int num_bin, double sigma_rel,

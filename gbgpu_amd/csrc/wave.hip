@@ -23,29 +23,47 @@ constexpr bool fits_lds() {
     return !(THIRD && MODE == MODE_WAVE && N > 1024 && sizeof(typename P::CT) == 16);
 }
 
+// LDS of one workgroup of k_wave.
 template <int N, int MODE, bool THIRD, class P>
-__global__ void __launch_bounds__(WG_SIZE) k_wave(WaveArgs a) {
+struct WaveLds {
     using G = Geo<N>;
     using CT = typename P::CT;
-    constexpr int TPB = G::TPB, BPW = G::BPW, WPB = G::WPB, SPT = G::SPT;
-    constexpr int NCH = (MODE == MODE_WAVE) ? 3 : 2;
-    constexpr int NT = THIRD ? N : 1;
+    static constexpr int NCH = (MODE == MODE_WAVE) ? 3 : 2;
+    static constexpr int NT = THIRD ? N : 1;
     // xi shares the transform buffer when that is large enough
-    constexpr bool XI_IN_BUF = NCH * sizeof(CT) * N >= 3 * sizeof(double) * N;
-    constexpr int NXI = (THIRD && !XI_IN_BUF) ? N : 1;
-    __shared__ CT sbuf[BPW * NCH * N];
-    __shared__ Pack<P> spack[BPW];
-    __shared__ double stot[BPW * WPB];
-    __shared__ float s_v[BPW * 3 * NT];
-    __shared__ double s_g[BPW * 3 * NT];
-    __shared__ double s_xi[BPW * 3 * NXI];
+    static constexpr bool XI_IN_BUF = NCH * sizeof(CT) * N >= 3 * sizeof(double) * N;
+    static constexpr int NXI = (THIRD && !XI_IN_BUF) ? N : 1;
+    CT sbuf[G::BPW * NCH * N];
+    Pack<P> spack[G::BPW];
+    double stot[G::BPW * G::WPB];
+    float s_v[G::BPW * 3 * NT];
+    double s_g[G::BPW * 3 * NT];
+    double s_xi[G::BPW * 3 * NXI];
+};
 
-    const int tid = threadIdx.x;
+// One pass of a workgroup: items blk*BPW .. blk*BPW + BPW-1 of the
+// `count` items (item i is binary order[first + i], or i without an order).
+template <int N, int MODE, bool THIRD, class P>
+__device__ __forceinline__ void wave_pass(const WaveArgs &a, WaveLds<N, MODE, THIRD, P> &L, long long blk,
+                                          long long count, long long first, int tid, const double *tab,
+                                          const void *twp) {
+    using G = Geo<N>;
+    using W = WaveLds<N, MODE, THIRD, P>;
+    using CT = typename P::CT;
+    constexpr int TPB = G::TPB, BPW = G::BPW, WPB = G::WPB, SPT = G::SPT;
+    constexpr int NCH = W::NCH, NT = W::NT, NXI = W::NXI;
+    constexpr bool XI_IN_BUF = W::XI_IN_BUF;
+    CT *sbuf = L.sbuf;
+    Pack<P> *spack = L.spack;
+    double *stot = L.stot;
+    float *s_v = L.s_v;
+    double *s_g = L.s_g;
+    double *s_xi = L.s_xi;
     const int slot = tid / TPB;
     const int lane = tid % TPB;
-    const long long item = (long long)blockIdx.x * BPW + slot;
-    const bool valid = item < a.B;
-    const long long b = (valid && a.order) ? (long long)a.order[item] : item;
+    const long long item = blk * BPW + slot;
+    const bool valid = item < count;
+    const long long b = (valid && a.order) ? (long long)a.order[first + item] : item;
 
     if (valid) load_pack<P, TPB>(a, b, lane, spack + slot);
     __syncthreads();
@@ -55,33 +73,34 @@ __global__ void __launch_bounds__(WG_SIZE) k_wave(WaveArgs a) {
     ThirdScratch ts{XI_IN_BUF ? reinterpret_cast<double *>(buf) : s_xi + slot * 3 * NXI, s_v + slot * 3 * NT,
                     s_g + slot * 3 * NT};
 
-    binary_spectrum<N, NCH, THIRD, P>(s, spack[slot].aux, valid, lane, a.tab, a.fstar_inv, a.scale, buf,
-                                      static_cast<const CT *>(a.tw), ts, tot);
+    binary_spectrum<N, NCH, THIRD, P>(s, spack[slot].aux, valid, lane, tab, a.fstar_inv, a.scale, buf,
+                                      static_cast<const CT *>(twp), ts, tot);
 
     const double2 fac = P::fac(s);
     if constexpr (MODE == MODE_WAVE) {
-        if (!valid) return;
-        double2 *xyz = a.XYZf ? a.XYZf + b * 3 * N : nullptr;
-        double2 *aet = a.AETf ? a.AETf + b * 3 * N : nullptr;
+        if (valid) {
+            double2 *xyz = a.XYZf ? a.XYZf + b * 3 * N : nullptr;
+            double2 *aet = a.AETf ? a.AETf + b * 3 * N : nullptr;
 #pragma unroll
-        for (int q = 0; q < SPT; q++) {
-            const int m = lane + q * TPB;
-            const int idx = (m + N / 2) & (N - 1);  // fftshift
-            const double2 X = cmul(P::up(buf[idx]), fac);
-            const double2 Y = cmul(P::up(buf[N + idx]), fac);
-            const double2 Z = cmul(P::up(buf[2 * N + idx]), fac);
-            if (xyz) {
-                xyz[m] = X;
-                xyz[N + m] = Y;
-                xyz[2 * N + m] = Z;
+            for (int q = 0; q < SPT; q++) {
+                const int m = lane + q * TPB;
+                const int idx = (m + N / 2) & (N - 1);  // fftshift
+                const double2 X = cmul(P::up(buf[idx]), fac);
+                const double2 Y = cmul(P::up(buf[N + idx]), fac);
+                const double2 Z = cmul(P::up(buf[2 * N + idx]), fac);
+                if (xyz) {
+                    xyz[m] = X;
+                    xyz[N + m] = Y;
+                    xyz[2 * N + m] = Z;
+                }
+                if (aet) {
+                    aet[m] = cscale(csub(Z, X), INV_SQRT2);
+                    aet[N + m] = cscale(cadd(csub(X, cscale(Y, 2.0)), Z), INV_SQRT6);
+                    aet[2 * N + m] = cscale(cadd(cadd(X, Y), Z), INV_SQRT3);
+                }
             }
-            if (aet) {
-                aet[m] = cscale(csub(Z, X), INV_SQRT2);
-                aet[N + m] = cscale(cadd(csub(X, cscale(Y, 2.0)), Z), INV_SQRT6);
-                aet[2 * N + m] = cscale(cadd(cadd(X, Y), Z), INV_SQRT3);
-            }
+            if (lane == 0) a.start_inds[b] = s.start;
         }
-        if (lane == 0) a.start_inds[b] = s.start;
     } else {
         double dhr = 0.0, dhi = 0.0, hh = 0.0;
         if (valid) {
@@ -101,6 +120,34 @@ __global__ void __launch_bounds__(WG_SIZE) k_wave(WaveArgs a) {
             const double dh_use = a.phase_marg ? sqrt(dhr * dhr + dhi * dhi) : dhr;
             a.loglike[b] = -0.5 * (a.d_d + hh - 2.0 * dh_use);
         }
+    }
+}
+
+template <int N, int MODE, bool THIRD, class P>
+__global__ void __launch_bounds__(WG_SIZE) k_wave(WaveArgs a) {
+    __shared__ WaveLds<N, MODE, THIRD, P> lds;
+    wave_pass<N, MODE, THIRD, P>(a, lds, blockIdx.x, a.B, 0, threadIdx.x, a.tab, a.tw);
+}
+
+// Fused likelihood over one device-side segment of a batch: item i <
+// *a.seg_count is binary a.order[*a.seg_off + i]; the grid strides over the
+// segment.  The orbit-table and twiddle reads and the per-thread index
+// arithmetic are the same in every pass: opaque copies of the pointers and
+// of the thread index keep the compiler from hoisting all of them out of the
+// loop into registers (which would cost occupancy).
+template <int N, class P>
+__global__ void __launch_bounds__(WG_SIZE) k_wave_seg(WaveArgs a) {
+    constexpr int BPW = Geo<N>::BPW;
+    __shared__ WaveLds<N, MODE_LL, false, P> lds;
+    const long long count = *a.seg_count;
+    const long long first = *a.seg_off;
+    for (long long blk = blockIdx.x; blk * BPW < count; blk += gridDim.x) {
+        const double *tab = a.tab;
+        const void *twp = a.tw;
+        int tid = threadIdx.x;
+        asm volatile("" : "+s"(tab), "+s"(twp), "+v"(tid));
+        wave_pass<N, MODE_LL, false, P>(a, lds, blk, count, first, tid, tab, twp);
+        __syncthreads();  // LDS is reused by the next pass
     }
 }
 
@@ -135,6 +182,46 @@ static int launch_wave(int N, const WaveArgs &a, hipStream_t st) {
             return GBGPU_E_ARG;
     }
 #undef GBGPU_CASE
+}
+
+// Likelihood kernel over one device-side segment of a batch (the N-bucket
+// launches of gbgpu_wave_ll_batched): a.pack, a.order, a.seg_count and
+// a.seg_off are set by the caller; the segment's size is known only on the
+// device, so the grid covers a few passes of resident workgroups and strides.
+template <class P, int NN>
+static int launch_segment_n(const WaveArgs &a, hipStream_t st) {
+    static int grid = 0;
+    if (!grid) {
+        int dev = 0, ncu = 0;
+        (void)hipGetDevice(&dev);
+        (void)hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev);
+        grid = 16 * (ncu > 0 ? ncu : 256);  // several passes' worth of resident workgroups
+    }
+    long long g = (a.B + Geo<NN>::BPW - 1) / Geo<NN>::BPW;
+    if (g > grid) g = grid;
+    KernelTimer kt(st);
+    hipLaunchKernelGGL((k_wave_seg<NN, P>), dim3((unsigned)g), dim3(WG_SIZE), 0, st, a);
+    GBGPU_CHECK_LAUNCH("k_wave_seg");
+    return GBGPU_OK;
+}
+
+int launch_ll_segment(int N, int precision, const WaveArgs &a, hipStream_t st) {
+#define GBGPU_SEG(NN)                                                                                   \
+    case NN:                                                                                            \
+        return precision == GBGPU_MIXED ? launch_segment_n<Mixed, NN>(a, st) : launch_segment_n<F64, NN>(a, st);
+    switch (N) {
+        GBGPU_SEG(32)
+        GBGPU_SEG(64)
+        GBGPU_SEG(128)
+        GBGPU_SEG(256)
+        GBGPU_SEG(512)
+        GBGPU_SEG(1024)
+        GBGPU_SEG(2048)
+        default:
+            set_error("N must be a power of two in [32, 2048]");
+            return GBGPU_E_ARG;
+    }
+#undef GBGPU_SEG
 }
 
 template <int MODE>

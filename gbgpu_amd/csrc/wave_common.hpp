@@ -48,6 +48,10 @@ struct WaveArgs {
     double2 *d_h, *h_h;
     const void *pack;  // per-binary constants written by k_setup (Pack<P>[B])
     const int *order;  // processing order (binaries sorted by start bin) or null
+    // Segment of `order` to process, read on the device: item i < *seg_count
+    // is binary order[*seg_off + i].  Null: items 0..B-1 (one grid pass).
+    const int *seg_count;
+    const int *seg_off;
     unsigned *keys;    // k_setup output: start-bin sort keys, or null
     int *vals;         // k_setup output: binary indices, or null
 };

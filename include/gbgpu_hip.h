@@ -159,21 +159,27 @@ int gbgpu_fisher(double *fisher, double *grad, double *loglike, const double *pa
 /* ---- uneven-N batches -------------------------------------------------- */
 
 /* Configuration of a batch whose binaries carry their own N. */
+enum { GBGPU_BATCH_BUCKETS = 0, GBGPU_BATCH_PERSISTENT = 1 };
+
 typedef struct gbgpu_batch_cfg {
     double T;                      /* observation time, already int(T/dt)*dt */
     double armlength;              /* metres */
     int tdi2;
     int precision;                 /* GBGPU_FP64 or GBGPU_MIXED */
     const double *orbit_tables[7]; /* table for N = 32 << k (k < 7), NULL when no binary uses it */
+    int strategy;                  /* GBGPU_BATCH_BUCKETS: one launch per N bucket (default);
+                                      GBGPU_BATCH_PERSISTENT: one persistent launch */
 } gbgpu_batch_cfg;
 
 /* Workspace bytes gbgpu_wave_ll_batched needs for num_bin binaries. */
 long long gbgpu_batch_workspace_bytes(int num_bin, int precision);
 
 /* Fused wave + log-likelihood for a batch with per-binary N (N_bin, device
- * int32, powers of two in [32, 2048]) in one persistent launch: binaries are
- * bucketed by N on the device and workgroups pull N-bucketed work items
- * from an atomic queue, largest N first.  The reference needs one run_wave +
+ * int32, powers of two in [32, 2048]): binaries are bucketed on the device
+ * by N and start bin, then either each bucket runs the uniform fused kernel
+ * over its device-side segment (GBGPU_BATCH_BUCKETS) or one persistent
+ * launch pulls N-bucketed work items from an atomic queue, largest N first
+ * (GBGPU_BATCH_PERSISTENT).  The reference needs one run_wave +
  * get_ll call per distinct N (src/gbgpu/gbgpu.py:205-207, gbgpu_utils.cu:126).
  * Outputs as gbgpu_wave_ll; binaries with an unsupported N or a NULL orbit
  * table get NaN.  workspace: gbgpu_batch_workspace_bytes() of device memory,

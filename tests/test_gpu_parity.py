@@ -415,8 +415,10 @@ def _uneven_batch(B, seed):
 
 @pytest.mark.parametrize("precision", ["fp64", "mixed"])
 def test_uneven_batch_matches_per_N_calls(precision):
-    """One persistent launch over N buckets == one standalone call per N
-    (same kernels per binary) and == the oracle within the precision's bar."""
+    """The uneven-N batch (both strategies) == one standalone call per N
+    (same arithmetic per binary) and == the oracle within the precision's bar."""
+    from gbgpu_amd.batch import BatchedLogLike
+
     B = 300
     p, N, rng = _uneven_batch(B, 600)
     T = 4 * YR
@@ -424,7 +426,8 @@ def test_uneven_batch_matches_per_N_calls(precision):
     gb = _gb(precision=precision)
     gb.d_d = float(4 * df * (np.sum(np.abs(A) ** 2 / SA) + np.sum(np.abs(E) ** 2 / SE)))
     ll = gb.get_ll(p, [A, E], [SA, SE], N=N, T=T, dt=DT)
-    d_h_b = gb.d_h.cpu().numpy()
+    bl = BatchedLogLike(gb, [A, E], [SA, SE], T, DT, precision=precision, strategy="persistent")
+    assert np.array_equal(bl(p, N=N).cpu().numpy(), ll)
     ref = np.empty(B)
     for n in np.unique(N):
         sel = N == n
@@ -481,7 +484,34 @@ def test_perturb_params_matches_numpy_restatement():
     assert np.array_equal(Nout.cpu().numpy(), levels[np.searchsorted(edges, np.abs(got[1]), side="right")])
 
 
-def test_graph_captured_loop_replays_match_eager():
+def test_uneven_batch_strategies_agree_at_scale():
+    """60k binaries: the per-bucket launches run several grid-stride passes;
+    both strategies give bit-identical results."""
+    import torch
+
+    from gbgpu_amd.batch import BatchedLogLike
+
+    B = 60_000
+    p, N, rng = _uneven_batch(B, 650)
+    T = 4 * YR
+    A, E, SA, SE, df, L = make_data(p[:, :10], 256, T, DT, rng)
+    gb = _gb(precision="fp64")
+    gb.d_d = float(4 * df * (np.sum(np.abs(A) ** 2 / SA) + np.sum(np.abs(E) ** 2 / SE)))
+    print(f"uneven batch B={B}", flush=True)
+    out = {}
+    for strat in ("buckets", "persistent"):
+        bl = BatchedLogLike(gb, [A, E], [SA, SE], T, DT, precision="fp64", strategy=strat)
+        out[strat] = bl(p, N=N).cpu().numpy()
+    assert np.array_equal(out["buckets"], out["persistent"])
+    assert np.all(np.isfinite(out["buckets"]))
+    # a sample against standalone per-N calls
+    sel = np.flatnonzero(N == 128)[:500]
+    ref = gb.get_ll(p[:, sel], [A, E], [SA, SE], N=128, T=T, dt=DT)
+    assert np.array_equal(out["buckets"][sel], ref)
+
+
+@pytest.mark.parametrize("strategy", ["buckets", "persistent"])
+def test_graph_captured_loop_replays_match_eager(strategy):
     """Captured loop of jump + batched likelihood: every replay equals the eager loop."""
     import torch
 
@@ -493,7 +523,7 @@ def test_graph_captured_loop_replays_match_eager():
     A, E, SA, SE, df, L = make_data(p[:, :10], 256, T, DT, rng)
     gb = _gb(precision="mixed")
     gb.d_d = float(4 * df * (np.sum(np.abs(A) ** 2 / SA) + np.sum(np.abs(E) ** 2 / SE)))
-    bl = BatchedLogLike(gb, [A, E], [SA, SE], T, DT, precision="mixed")
+    bl = BatchedLogLike(gb, [A, E], [SA, SE], T, DT, precision="mixed", strategy=strategy)
     eager = bl.run_loop(p, 5, sigma=1e-3, seed=3).clone()
     torch.cuda.synchronize()
     g, out = bl.capture_loop(p, 5, sigma=1e-3, seed=3)
