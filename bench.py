@@ -32,6 +32,12 @@ YRSID_SI = 31558149.763545603
 JULIAN_YEAR_S = 31557600.0  # the workload's "1 yr" (4 yr -> 4,207,681 bins at dt = 15 s)
 FP64_PEAK_TFLOPS = 78.6
 HBM_PEAK_GBS = 8000.0
+# HBM bytes per launch of the fused FP64 kernel on the default workload
+# (1e6 binaries, N = 256), from rocprofv3 FETCH_SIZE and WRITE_SIZE passes of
+# this script (profiles/k_wave_256_fp64_uniform_1e6_counters.md):
+# 305,745 KiB x 2.0 + 32,015 KiB x 1.0, the factors being what the counters
+# read for a 1 GiB device copy (0.5 GiB fetched, 1 GiB written).
+MEASURED_TRAFFIC_BYTES_1E6_N256 = (305745 * 2.0 + 32015 * 1.0) * 1024
 
 
 def flop_per_binary(N: int) -> float:
@@ -532,7 +538,9 @@ def main():
             "peak": FP64_PEAK_TFLOPS,
             "unit": "TFLOP/s",
             "frac": achieved_tflops / FP64_PEAK_TFLOPS,
-            "traffic": None,
+            "traffic": MEASURED_TRAFFIC_BYTES_1E6_N256 if B == 1_000_000 else None,
+            "traffic_note": "HBM bytes per launch (rocprofv3 FETCH_SIZE x 2.0 + WRITE_SIZE x 1.0); the algorithmic "
+                            "bytes are 48 N + 112 per binary = 12.4 GB, served mostly from L2 / Infinity Cache",
             "kernel": "k_wave<256, MODE_LL, circular, F64>",
             "kernel_ms": kernel_s * 1e3,
             "flop_per_binary": flop_per_binary(N),

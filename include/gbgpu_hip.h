@@ -4,10 +4,14 @@
  * interleaved {re, im} float64 pairs (numpy complex128 layout, the same as the
  * reference's gcmplx::complex<double>, src/gbgpu/cutils/global.h:66).  Every
  * entry point is asynchronous on the caller's HIP stream (``stream`` is a
- * hipStream_t passed as void*, NULL = default stream), never synchronises the
- * device, allocates nothing on the hot path beyond its first-use caches, and
- * returns 0 on success or a negative GBGPU_E* code; gbgpu_last_error() then
- * holds a message for the calling thread.
+ * hipStream_t passed as void*, NULL = default stream) and never synchronises
+ * the device.  Per-call scratch (per-binary constants, sort keys) is
+ * stream-ordered: hipMallocAsync / hipFreeAsync on the call's stream, except
+ * for gbgpu_wave_ll_batched given a workspace, which allocates nothing.  The
+ * first call for a given N on a device also creates that N's twiddle table
+ * (a blocking upload, cached until the process ends).  Every call returns 0
+ * on success or a negative GBGPU_E* code; gbgpu_last_error() then holds a
+ * message for the calling thread.
  */
 #ifndef GBGPU_HIP_H
 #define GBGPU_HIP_H

@@ -36,4 +36,15 @@ How each function is pinned (the checks live in tests/):
   * third body: circular closed-form v_LOS and its analytic phase integral;
     Kepler solver against scipy root finding;
   * constants: notebook known answers (get_amplitude, get_fdot).
+
+Parity unpinned (nothing available here can fix them: they would need
+lisatools, which is absent, and the reference's tests hold no golden
+vectors):
+  * absolute waveform values with the reference's default orbits
+    (lisatools EqualArmlengthOrbits, interpolated from an orbit file); this
+    oracle and the product both use the analytic Cornish & Rubbo orbits with
+    the LISA.h constants, and every check above holds for any orbit model;
+  * get_N's SNR adjustment and the default PSD of information_matrix, which
+    use this build's own analytic A/X-channel sensitivity in place of
+    lisatools' A1TDISens / X1TDISens.
 """

@@ -28,7 +28,7 @@ class AnalyticOrbits:
     """Cornish & Rubbo (2003) rigid-rotating constellation, LISA.h constants.
 
     Oracle-side statement of the orbit model the product tables are built
-    from; pinned by closed-form checks in tests/test_orbits.py.
+    from; pinned by closed-form checks in tests/test_constants_orbits.py.
     """
 
     def __init__(self, armlength=LARM, ec=EC, kappa=KAPPA, lam=LAMBDA, fm=FM, t_max=10 * YRSID_SI):
