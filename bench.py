@@ -162,7 +162,8 @@ def parity_sample(gb, params, A, E, SA, SE, d_d, df, N, T, dt, precision, n=64):
 def uneven_batch(gb, A, E, SA, SE, d_d, B, T, dt, n_calls, seed=77):
     """Uneven-N batch: f0 log-uniform in [0.1, 20] mHz, N by the frequency
     rule (128/256/512/1024/2048 split at 1/3/6/10 mHz), a captured graph of
-    n_calls likelihood calls at seeded 1e-3 relative jumps; FP64 and mixed."""
+    n_calls likelihood calls at seeded 1e-3 relative jumps; FP64, mixed with
+    the FP64 accuracy guard at 1e-3 (the default), and mixed without it."""
     import torch
 
     from gbgpu_amd.batch import BatchedLogLike
@@ -177,10 +178,13 @@ def uneven_batch(gb, A, E, SA, SE, d_d, B, T, dt, n_calls, seed=77):
            "N_fractions": {int(n): float(np.mean(N == n)) for n in np.unique(N)},
            "mean_N": float(N.mean())}
     lls = {}
-    for prec in ("fp64", "mixed"):
-        for strat in ("buckets", "persistent"):
+    for prec, strat, guard in (("fp64", "buckets", None), ("fp64", "persistent", None),
+                               ("mixed", "buckets", 1e-3), ("mixed", "persistent", None),
+                               ("mixed_unguarded", "buckets", None)):
+        if True:
             print(f"[bench] uneven batch {prec} {strat}", file=sys.stderr, flush=True)
-            bl = BatchedLogLike(gb, [A, E], [SA, SE], T, dt, precision=prec, strategy=strat)
+            bl = BatchedLogLike(gb, [A, E], [SA, SE], T, dt, precision=prec.split("_")[0], strategy=strat,
+                                guard_tol=guard)
             g, out = bl.capture_loop(p, n_calls, sigma=1e-3, seed=seed)
             g.replay()
             torch.cuda.synchronize()
@@ -194,6 +198,9 @@ def uneven_batch(gb, A, E, SA, SE, d_d, B, T, dt, n_calls, seed=77):
             torch.cuda.synchronize()
             sec = ev0.elapsed_time(ev1) / 1e3 / reps
             r = {"binaries_per_s": n_calls * B / sec, "ms_per_call": sec / n_calls * 1e3}
+            if guard:
+                r["guard_tol"] = guard
+                r["fp64_reevaluated_per_call"] = sum(bl.guard_counts().values())
             if strat == "buckets":
                 res[prec] = r
                 lls[prec] = out.clone()
@@ -202,9 +209,12 @@ def uneven_batch(gb, A, E, SA, SE, d_d, B, T, dt, n_calls, seed=77):
             del g, out, bl
             torch.cuda.empty_cache()
     res["speedup_mixed_vs_fp64"] = res["mixed"]["binaries_per_s"] / res["fp64"]["binaries_per_s"]
-    d = (lls["mixed"] - lls["fp64"]).abs()
-    res["logL_abs_err_mixed_vs_fp64"] = {"max": float(d.max()), "median": float(d.median()),
-                                         "frac_le_1e-3": float((d <= 1e-3).double().mean())}
+    res["speedup_mixed_unguarded_vs_fp64"] = (res["mixed_unguarded"]["binaries_per_s"] /
+                                              res["fp64"]["binaries_per_s"])
+    for key in ("mixed", "mixed_unguarded"):
+        d = (lls[key] - lls["fp64"]).abs()
+        res[f"logL_abs_err_{key}_vs_fp64"] = {"max": float(d.max()), "median": float(d.median()),
+                                             "frac_le_1e-3": float((d <= 1e-3).double().mean())}
     # per-channel waveform error of the mixed path against the FP64 oracle,
     # on a sample of every N bucket (unperturbed parameters)
     errs = {"A": 0.0, "E": 0.0, "T": 0.0}

@@ -66,6 +66,7 @@ class BatchCfg(ctypes.Structure):
         ("precision", c_int),
         ("orbit_tables", c_void_p * 7),
         ("strategy", c_int),
+        ("guard_tol", c_double),
     ]
 
 
@@ -112,6 +113,7 @@ SIGNATURES = {
         [c_void_p, c_void_p, c_void_p, c_void_p, c_int, c_int, c_void_p, ctypes.POINTER(BatchCfg),
          ctypes.POINTER(Data), c_double, c_int, c_void_p, c_void_p],
     ),
+    "gbgpu_batch_guard_counts": (c_int, [c_void_p, c_int, c_int, c_void_p, c_void_p]),
     "gbgpu_perturb_params": (
         c_int,
         [c_void_p, c_void_p, c_int, c_int, c_double, ctypes.c_ulonglong, ctypes.c_ulonglong, c_void_p, c_int,

@@ -37,12 +37,17 @@ class BatchedLogLike:
             N_levels[i] for f0 < f_edges[i], the last level above.
         strategy: "buckets" (one launch of the uniform kernel per N bucket)
             or "persistent" (one persistent launch over all buckets).
+        guard_tol: mixed precision with the "buckets" strategy: binaries
+            whose log-likelihood error bound 2.5e-6 (<h|h> + |<d|h>|) exceeds
+            guard_tol are re-evaluated in FP64 within the same call (None or
+            0 disables; loud templates otherwise carry an absolute error
+            that grows with SNR^2).
     """
 
     STRATEGIES = {"buckets": _capi.GBGPU_BATCH_BUCKETS, "persistent": _capi.GBGPU_BATCH_PERSISTENT}
 
     def __init__(self, gb, data, psd, T, dt, precision="mixed", start_freq_ind=0, N_levels=DEFAULT_N_LEVELS,
-                 f_edges=DEFAULT_F_EDGES, phase_marginalize=False, strategy="buckets"):
+                 f_edges=DEFAULT_F_EDGES, phase_marginalize=False, strategy="buckets", guard_tol=1e-3):
         torch = _torch()
         if gb.d_d is None:
             raise ValueError("gb.d_d must be set before building a likelihood")
@@ -62,8 +67,10 @@ class BatchedLogLike:
         self.L = int(self.A.numel())
         if strategy not in self.STRATEGIES:
             raise ValueError(f"strategy must be one of {list(self.STRATEGIES)}")
+        self.guard_tol = float(guard_tol or 0.0) if precision == "mixed" else 0.0
         self.cfg = _capi.BatchCfg(T=float(self.T), armlength=float(gb.orbits.armlength), tdi2=0,
-                                  precision=1 if precision == "mixed" else 0, strategy=self.STRATEGIES[strategy])
+                                  precision=1 if precision == "mixed" else 0, strategy=self.STRATEGIES[strategy],
+                                  guard_tol=self.guard_tol)
         self._tabs = {}
         for N in self.N_levels:
             self._add_table(int(N))
@@ -72,6 +79,7 @@ class BatchedLogLike:
                                start_freq_ind=int(start_freq_ind), df=float(1.0 / self.T))
         self.dev = dev
         self._ws = None
+        self._last_B = 0
         self._graph = None
 
     def _add_table(self, N):
@@ -88,6 +96,17 @@ class BatchedLogLike:
         if self._ws is None or self._ws.numel() < need:
             self._ws = torch.empty(int(need), dtype=torch.uint8, device=self.dev)
         return self._ws
+
+    def guard_counts(self):
+        """Binaries per N (dict) the accuracy guard re-evaluated in FP64 in the
+        last call (synchronises the current stream)."""
+        counts = np.zeros(7, dtype=np.int32)
+        if self._ws is None:
+            return {}
+        _capi.check(_capi.load().gbgpu_batch_guard_counts(self._ws.data_ptr(), self._last_B, self.cfg.precision,
+                                                          counts.ctypes.data, _capi.stream_handle()),
+                    "gbgpu_batch_guard_counts")
+        return {32 << k: int(c) for k, c in enumerate(counts) if c}
 
     def N_of(self, f0):
         """N from f0 by the step rule."""
@@ -113,6 +132,7 @@ class BatchedLogLike:
 
     def _launch(self, ll, dh, hh, p, Nt, B):
         ws = self._workspace(B)
+        self._last_B = B
         rc = _capi.load().gbgpu_wave_ll_batched(ll.data_ptr(), _capi.ptr(dh), _capi.ptr(hh), p.data_ptr(), 9, B,
                                                 Nt.data_ptr(), self.cfg, self.data, float(np.real(self.gb.d_d)),
                                                 int(self.phase_marginalize), ws.data_ptr(), _capi.stream_handle())

@@ -31,13 +31,13 @@ namespace gbgpu {
 int check_data(const gbgpu_data *data);
 void set_data(WaveArgs &a, const gbgpu_data *data);
 
-int launch_ll_segment(int N, int precision, const WaveArgs &a, hipStream_t st);
+int launch_ll_segment(int N, int precision, int pack_precision, const WaveArgs &a, hipStream_t st);
 
 constexpr int NBUCKET = 7;  // N = 32 << k, k < 7
 constexpr int NCB = 4096;   // start-bin cells per bucket (log-spaced)
 constexpr int NCELL = NBUCKET * NCB;
 enum { CTL_COUNT = 0, CTL_LISTOFF = 16, CTL_ITEMOFF = 24, CTL_ITEMS = 32, CTL_TOTAL = 40, CTL_WORK = 41,
-       CTL_HIST = 64, CTL_CUR = CTL_HIST + NCELL, CTL_WORDS = CTL_CUR + NCELL };
+       CTL_REDO = 48, CTL_HIST = 64, CTL_CUR = CTL_HIST + NCELL, CTL_WORDS = CTL_CUR + NCELL };
 
 __host__ __device__ constexpr int bucket_bpw(int k) {
     return WG_SIZE / (((32 << k) / 4 < WG_SIZE) ? (32 << k) / 4 : WG_SIZE);
@@ -55,7 +55,9 @@ struct BatchArgs {
     const int *Nb;
     int *list;
     int *cell;  // histogram cell per binary (-1: not evaluated)
+    int *redo;  // accuracy-guard list (same per-bucket offsets as `list`)
     int *ctl;
+    double guard_tol;
 };
 
 __global__ void k_batch_reset(int *ctl) {
@@ -288,7 +290,7 @@ struct BatchKernel<Mixed> {
 template <class P>
 size_t batch_ws_bytes(long long B) {
     auto al = [](size_t n) { return (n + 255) & ~(size_t)255; };
-    return al(sizeof(Pack<P>) * (size_t)B) + 2 * al(sizeof(int) * (size_t)B) + al(sizeof(int) * CTL_WORDS);
+    return al(sizeof(Pack<P>) * (size_t)B) + 3 * al(sizeof(int) * (size_t)B) + al(sizeof(int) * CTL_WORDS);
 }
 
 template <class P>
@@ -321,9 +323,14 @@ static int launch_persistent(BatchArgs &ba, hipStream_t st) {
     return GBGPU_OK;
 }
 
-// One launch of the uniform fused kernel per populated bucket.
+// One launch of the uniform fused kernel per populated bucket.  With the
+// accuracy guard (mixed precision, guard_tol > 0), each mixed launch lists
+// the binaries whose error bound exceeds guard_tol and an FP64 launch over
+// that list (reading the same mixed per-binary constants) overwrites their
+// results.
 template <class P>
 static int launch_buckets(const BatchArgs &ba, int precision, hipStream_t st) {
+    const bool guard = precision == GBGPU_MIXED && ba.guard_tol > 0.0;
     for (int k = NBUCKET - 1; k >= 0; k--) {
         if (!ba.tabs[k]) continue;
         const int N = 32 << k;
@@ -334,8 +341,24 @@ static int launch_buckets(const BatchArgs &ba, int precision, hipStream_t st) {
         a.order = ba.list;
         a.seg_count = ba.ctl + CTL_COUNT + k;
         a.seg_off = ba.ctl + CTL_LISTOFF + k;
-        const int rc = launch_ll_segment(N, precision, a, st);
+        if (guard) {
+            a.redo_list = ba.redo;
+            a.redo_count = ba.ctl + CTL_REDO + k;
+            a.redo_tol = ba.guard_tol;
+        }
+        int rc = launch_ll_segment(N, precision, precision, a, st);
         if (rc) return rc;
+        if (guard) {
+            WaveArgs r = a;
+            r.redo_list = nullptr;
+            r.redo_count = nullptr;
+            r.order = ba.redo;
+            r.seg_count = ba.ctl + CTL_REDO + k;
+            r.tw = twiddles_f64(N);
+            if (!r.tw) return hip_fail(hipErrorOutOfMemory, "twiddles");
+            rc = launch_ll_segment(N, GBGPU_FP64, GBGPU_MIXED, r, st);
+            if (rc) return rc;
+        }
     }
     return GBGPU_OK;
 }
@@ -355,7 +378,8 @@ static int launch_batch(BatchArgs ba, int precision, int strategy, void *workspa
     Pack<P> *pack = reinterpret_cast<Pack<P> *>(ws);
     ba.list = reinterpret_cast<int *>(ws + al(sizeof(Pack<P>) * (size_t)B));
     ba.cell = reinterpret_cast<int *>(ws + al(sizeof(Pack<P>) * (size_t)B) + al(sizeof(int) * (size_t)B));
-    ba.ctl = reinterpret_cast<int *>(ws + al(sizeof(Pack<P>) * (size_t)B) + 2 * al(sizeof(int) * (size_t)B));
+    ba.redo = reinterpret_cast<int *>(ws + al(sizeof(Pack<P>) * (size_t)B) + 2 * al(sizeof(int) * (size_t)B));
+    ba.ctl = reinterpret_cast<int *>(ws + al(sizeof(Pack<P>) * (size_t)B) + 3 * al(sizeof(int) * (size_t)B));
     ba.a.pack = pack;
     const unsigned gb = (unsigned)((B + 255) / 256);
     hipLaunchKernelGGL(k_batch_reset, dim3((CTL_WORDS + 255) / 256), dim3(256), 0, st, ba.ctl);
@@ -457,6 +481,7 @@ extern "C" int gbgpu_wave_ll_batched(double *loglike, gbgpu_cmplx *d_h, gbgpu_cm
     a.d_h = reinterpret_cast<double2 *>(d_h);
     a.h_h = reinterpret_cast<double2 *>(h_h);
     ba.Nb = N_bin;
+    ba.guard_tol = cfg->guard_tol;
     for (int k = 0; k < NBUCKET; k++) {
         ba.tabs[k] = cfg->orbit_tables[k];
         ba.tws[k] = nullptr;
@@ -464,6 +489,9 @@ extern "C" int gbgpu_wave_ll_batched(double *loglike, gbgpu_cmplx *d_h, gbgpu_cm
             const int N = 32 << k;
             ba.tws[k] = cfg->precision == GBGPU_MIXED ? (const void *)twiddles_f32(N) : (const void *)twiddles_f64(N);
             if (!ba.tws[k]) return hip_fail(hipErrorOutOfMemory, "twiddles");
+            // the guard's FP64 pass needs the FP64 table (created here, outside any capture)
+            if (cfg->precision == GBGPU_MIXED && cfg->guard_tol > 0.0 && !twiddles_f64(N))
+                return hip_fail(hipErrorOutOfMemory, "twiddles");
         }
     }
     if (cfg->strategy != GBGPU_BATCH_BUCKETS && cfg->strategy != GBGPU_BATCH_PERSISTENT) {
@@ -473,6 +501,24 @@ extern "C" int gbgpu_wave_ll_batched(double *loglike, gbgpu_cmplx *d_h, gbgpu_cm
     hipStream_t st = static_cast<hipStream_t>(stream);
     return cfg->precision == GBGPU_MIXED ? launch_batch<Mixed>(ba, cfg->precision, cfg->strategy, workspace, st)
                                          : launch_batch<F64>(ba, cfg->precision, cfg->strategy, workspace, st);
+}
+
+extern "C" int gbgpu_batch_guard_counts(const void *workspace, int num_bin, int precision, int *counts,
+                                        void *stream) {
+    if (!workspace || !counts || num_bin < 0 || (precision != GBGPU_FP64 && precision != GBGPU_MIXED)) {
+        set_error("gbgpu_batch_guard_counts: bad arguments");
+        return GBGPU_E_ARG;
+    }
+    auto al = [](size_t n) { return (n + 255) & ~(size_t)255; };
+    const size_t pack = precision == GBGPU_MIXED ? sizeof(Pack<Mixed>) : sizeof(Pack<F64>);
+    const char *ctl = static_cast<const char *>(workspace) + al(pack * (size_t)num_bin) +
+                      3 * al(sizeof(int) * (size_t)num_bin);
+    hipStream_t st = static_cast<hipStream_t>(stream);
+    hipError_t e = hipMemcpyAsync(counts, ctl + sizeof(int) * CTL_REDO, sizeof(int) * NBUCKET, hipMemcpyDeviceToHost,
+                                  st);
+    if (e == hipSuccess) e = hipStreamSynchronize(st);
+    if (e != hipSuccess) return hip_fail(e, "gbgpu_batch_guard_counts");
+    return GBGPU_OK;
 }
 
 extern "C" int gbgpu_perturb_params(double *out, const double *in, int num_params, int num_bin, double sigma_rel,

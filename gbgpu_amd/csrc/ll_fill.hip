@@ -120,17 +120,25 @@ __device__ __forceinline__ long long lower_bound_u64(const unsigned long long *_
     return lo;
 }
 
-// One wavefront per (group, 64-bin tile).  rows: binary-major, row b starts
-// at rows + b * row_stride; sample i of binary b is rows[b * row_stride + i].
-__global__ void __launch_bounds__(256) k_gather_sum(double2 *__restrict__ Aglob, double2 *__restrict__ Eglob,
-                                                    const double2 *__restrict__ rowsA,
-                                                    const double2 *__restrict__ rowsE, long long row_stride,
-                                                    const unsigned long long *__restrict__ keys,
-                                                    const int *__restrict__ perm, long long B, int M, long long L,
-                                                    long long tiles_per_group, long long num_tiles) {
+// One workgroup per (group, 64-bin output tile).  rows: binary-major, row b
+// starts at rows + b * row_stride; sample i of binary b is
+// rows[b * row_stride + i].  The rows overlapping the tile (a contiguous range
+// of the sorted keys) are split into GATHER_WAVES contiguous parts, one per
+// wavefront; each wavefront adds its part in key order with GATHER_UNROLL
+// row loads in flight, and the partial sums are combined in wavefront order.
+// Every output bin's sum therefore has a fixed order: results are
+// bit-identical from run to run.
+constexpr int GATHER_WAVES = 4;
+constexpr int GATHER_UNROLL = 4;
+
+__global__ void __launch_bounds__(64 * GATHER_WAVES) k_gather_sum(
+    double2 *__restrict__ Aglob, double2 *__restrict__ Eglob, const double2 *__restrict__ rowsA,
+    const double2 *__restrict__ rowsE, long long row_stride, const unsigned long long *__restrict__ keys,
+    const int *__restrict__ perm, long long B, int M, long long L, long long tiles_per_group) {
+    __shared__ double4 part[GATHER_WAVES][64];
     const int lane = threadIdx.x & 63;
-    const long long tile = (long long)blockIdx.x * 4 + (threadIdx.x >> 6);
-    if (tile >= num_tiles) return;
+    const int w = threadIdx.x >> 6;
+    const long long tile = blockIdx.x;
     const long long g = tile / tiles_per_group;
     const long long j0 = (tile - g * tiles_per_group) * 64;
     const unsigned long long KS = (unsigned long long)(L + M);
@@ -138,27 +146,57 @@ __global__ void __launch_bounds__(256) k_gather_sum(double2 *__restrict__ Aglob,
     // binaries with start in [j0 - M + 1, j0 + 63], i.e. key in [base + j0 + 1, base + j0 + 63 + M]
     const long long lo = lower_bound_u64(keys, B, base + (unsigned long long)j0 + 1ULL);
     const long long hi = lower_bound_u64(keys, B, base + (unsigned long long)(j0 + 64 + M));
+    if (hi <= lo) return;  // uniform over the workgroup
+    const long long per = (hi - lo + GATHER_WAVES - 1) / GATHER_WAVES;
+    const long long r0 = lo + w * per;
+    const long long r1 = r0 + per < hi ? r0 + per : hi;
     const long long j = j0 + lane;
     double ar = 0.0, ai = 0.0, er = 0.0, ei = 0.0;
-    for (long long r = lo; r < hi; r++) {
-        const long long s = (long long)(keys[r] - base) - M;
-        const long long i = j - s;
-        if (i >= 0 && i < M) {
-            const long long off = (long long)perm[r] * row_stride + i;
-            const double2 a = rowsA[off], e = rowsE[off];
-            ar += a.x;
-            ai += a.y;
-            er += e.x;
-            ei += e.y;
+    for (long long r = r0; r < r1; r += GATHER_UNROLL) {
+        double2 va[GATHER_UNROLL], ve[GATHER_UNROLL];
+        bool ok[GATHER_UNROLL];
+#pragma unroll
+        for (int u = 0; u < GATHER_UNROLL; u++) {
+            const long long rr = r + u;
+            ok[u] = false;
+            if (rr < r1) {
+                const long long i = j - ((long long)(keys[rr] - base) - M);
+                if (i >= 0 && i < M) {
+                    const long long off = (long long)perm[rr] * row_stride + i;
+                    va[u] = rowsA[off];
+                    ve[u] = rowsE[off];
+                    ok[u] = true;
+                }
+            }
+        }
+#pragma unroll
+        for (int u = 0; u < GATHER_UNROLL; u++) {
+            if (ok[u]) {
+                ar += va[u].x;
+                ai += va[u].y;
+                er += ve[u].x;
+                ei += ve[u].y;
+            }
         }
     }
-    if (j < L && hi > lo) {
+    part[w][lane] = make_double4(ar, ai, er, ei);
+    __syncthreads();
+    if (w == 0 && j < L) {
+        double4 t = part[0][lane];
+#pragma unroll
+        for (int k = 1; k < GATHER_WAVES; k++) {
+            const double4 p = part[k][lane];
+            t.x += p.x;
+            t.y += p.y;
+            t.z += p.z;
+            t.w += p.w;
+        }
         const long long o = g * L + j;
         double2 A = Aglob[o], E = Eglob[o];
-        A.x += ar;
-        A.y += ai;
-        E.x += er;
-        E.y += ei;
+        A.x += t.x;
+        A.y += t.y;
+        E.x += t.z;
+        E.y += t.w;
         Aglob[o] = A;
         Eglob[o] = E;
     }
@@ -203,10 +241,15 @@ static int fill_rows_impl(double2 *Aglob, double2 *Eglob, const double2 *rowsA, 
     }
     const long long tpg = (L + 63) / 64;
     const long long nt = tpg * G;
+    if (nt > 0x7fffffffLL) {
+        (void)hipFreeAsync(ws, st);
+        set_error("fill: num_groups * ceil(data_length / 64) exceeds the launch limit");
+        return GBGPU_E_ARG;
+    }
     {
         KernelTimer kt(st);
-        hipLaunchKernelGGL(k_gather_sum, dim3((unsigned)((nt + 3) / 4)), dim3(256), 0, st, Aglob, Eglob, rowsA,
-                           rowsE, row_stride, keys_sorted, perm, B, M, L, tpg, nt);
+        hipLaunchKernelGGL(k_gather_sum, dim3((unsigned)nt), dim3(64 * GATHER_WAVES), 0, st, Aglob, Eglob, rowsA,
+                           rowsE, row_stride, keys_sorted, perm, B, M, L, tpg);
     }
     e = hipGetLastError();
     hipError_t e2 = hipFreeAsync(ws, st);

@@ -9,6 +9,7 @@
 // data/PSD window with coalesced loads and reduces <d|h>, <h|h> over the
 // group (wave_ll).
 #include <hipcub/hipcub.hpp>
+#include <type_traits>
 
 #include "common.hpp"
 #include "dispatch.hpp"
@@ -23,8 +24,10 @@ constexpr bool fits_lds() {
     return !(THIRD && MODE == MODE_WAVE && N > 1024 && sizeof(typename P::CT) == 16);
 }
 
-// LDS of one workgroup of k_wave.
-template <int N, int MODE, bool THIRD, class P>
+// LDS of one workgroup of k_wave.  PK is the precision policy whose Pack
+// layout the per-binary constants were written in (normally P; the FP64
+// re-evaluation of guarded mixed binaries reads the mixed packs).
+template <int N, int MODE, bool THIRD, class P, class PK = P>
 struct WaveLds {
     using G = Geo<N>;
     using CT = typename P::CT;
@@ -34,7 +37,7 @@ struct WaveLds {
     static constexpr bool XI_IN_BUF = NCH * sizeof(CT) * N >= 3 * sizeof(double) * N;
     static constexpr int NXI = (THIRD && !XI_IN_BUF) ? N : 1;
     CT sbuf[G::BPW * NCH * N];
-    Pack<P> spack[G::BPW];
+    Pack<PK> spack[G::BPW];
     double stot[G::BPW * G::WPB];
     float s_v[G::BPW * 3 * NT];
     double s_g[G::BPW * 3 * NT];
@@ -43,18 +46,26 @@ struct WaveLds {
 
 // One pass of a workgroup: items blk*BPW .. blk*BPW + BPW-1 of the
 // `count` items (item i is binary order[first + i], or i without an order).
-template <int N, int MODE, bool THIRD, class P>
-__device__ __forceinline__ void wave_pass(const WaveArgs &a, WaveLds<N, MODE, THIRD, P> &L, long long blk,
+template <class P, class PK>
+__device__ __forceinline__ typename P::Aux aux_of(const Pack<PK> &pk) {
+    if constexpr (std::is_same<P, PK>::value)
+        return pk.aux;
+    else
+        return P::make_aux(pk.s);
+}
+
+template <int N, int MODE, bool THIRD, class P, class PK = P>
+__device__ __forceinline__ void wave_pass(const WaveArgs &a, WaveLds<N, MODE, THIRD, P, PK> &L, long long blk,
                                           long long count, long long first, int tid, const double *tab,
                                           const void *twp) {
     using G = Geo<N>;
-    using W = WaveLds<N, MODE, THIRD, P>;
+    using W = WaveLds<N, MODE, THIRD, P, PK>;
     using CT = typename P::CT;
     constexpr int TPB = G::TPB, BPW = G::BPW, WPB = G::WPB, SPT = G::SPT;
     constexpr int NCH = W::NCH, NT = W::NT, NXI = W::NXI;
     constexpr bool XI_IN_BUF = W::XI_IN_BUF;
     CT *sbuf = L.sbuf;
-    Pack<P> *spack = L.spack;
+    Pack<PK> *spack = L.spack;
     double *stot = L.stot;
     float *s_v = L.s_v;
     double *s_g = L.s_g;
@@ -65,7 +76,7 @@ __device__ __forceinline__ void wave_pass(const WaveArgs &a, WaveLds<N, MODE, TH
     const bool valid = item < count;
     const long long b = (valid && a.order) ? (long long)a.order[first + item] : item;
 
-    if (valid) load_pack<P, TPB>(a, b, lane, spack + slot);
+    if (valid) load_pack<PK, TPB>(a, b, lane, spack + slot);
     __syncthreads();
     const Src s = spack[slot].s;
     CT *buf = sbuf + slot * NCH * N;
@@ -73,7 +84,7 @@ __device__ __forceinline__ void wave_pass(const WaveArgs &a, WaveLds<N, MODE, TH
     ThirdScratch ts{XI_IN_BUF ? reinterpret_cast<double *>(buf) : s_xi + slot * 3 * NXI, s_v + slot * 3 * NT,
                     s_g + slot * 3 * NT};
 
-    binary_spectrum<N, NCH, THIRD, P>(s, spack[slot].aux, valid, lane, tab, a.fstar_inv, a.scale, buf,
+    binary_spectrum<N, NCH, THIRD, P>(s, aux_of<P, PK>(spack[slot]), valid, lane, tab, a.fstar_inv, a.scale, buf,
                                       static_cast<const CT *>(twp), ts, tot);
 
     const double2 fac = P::fac(s);
@@ -119,6 +130,10 @@ __device__ __forceinline__ void wave_pass(const WaveArgs &a, WaveLds<N, MODE, TH
             if (a.h_h) a.h_h[b] = make_double2(hh, 0.0);
             const double dh_use = a.phase_marg ? sqrt(dhr * dhr + dhi * dhi) : dhr;
             a.loglike[b] = -0.5 * (a.d_d + hh - 2.0 * dh_use);
+            if (a.redo_list && GUARD_REL * (fabs(hh) + sqrt(dhr * dhr + dhi * dhi)) > a.redo_tol) {
+                const int pos = atomicAdd(a.redo_count, 1);
+                a.redo_list[first + pos] = (int)b;
+            }
         }
     }
 }
@@ -135,10 +150,10 @@ __global__ void __launch_bounds__(WG_SIZE) k_wave(WaveArgs a) {
 // arithmetic are the same in every pass: opaque copies of the pointers and
 // of the thread index keep the compiler from hoisting all of them out of the
 // loop into registers (which would cost occupancy).
-template <int N, class P>
+template <int N, class P, class PK>
 __global__ void __launch_bounds__(WG_SIZE) k_wave_seg(WaveArgs a) {
     constexpr int BPW = Geo<N>::BPW;
-    __shared__ WaveLds<N, MODE_LL, false, P> lds;
+    __shared__ WaveLds<N, MODE_LL, false, P, PK> lds;
     const long long count = *a.seg_count;
     const long long first = *a.seg_off;
     for (long long blk = blockIdx.x; blk * BPW < count; blk += gridDim.x) {
@@ -146,7 +161,7 @@ __global__ void __launch_bounds__(WG_SIZE) k_wave_seg(WaveArgs a) {
         const void *twp = a.tw;
         int tid = threadIdx.x;
         asm volatile("" : "+s"(tab), "+s"(twp), "+v"(tid));
-        wave_pass<N, MODE_LL, false, P>(a, lds, blk, count, first, tid, tab, twp);
+        wave_pass<N, MODE_LL, false, P, PK>(a, lds, blk, count, first, tid, tab, twp);
         __syncthreads();  // LDS is reused by the next pass
     }
 }
@@ -188,7 +203,7 @@ static int launch_wave(int N, const WaveArgs &a, hipStream_t st) {
 // launches of gbgpu_wave_ll_batched): a.pack, a.order, a.seg_count and
 // a.seg_off are set by the caller; the segment's size is known only on the
 // device, so the grid covers a few passes of resident workgroups and strides.
-template <class P, int NN>
+template <class P, class PK, int NN>
 static int launch_segment_n(const WaveArgs &a, hipStream_t st) {
     static int grid = 0;
     if (!grid) {
@@ -200,15 +215,22 @@ static int launch_segment_n(const WaveArgs &a, hipStream_t st) {
     long long g = (a.B + Geo<NN>::BPW - 1) / Geo<NN>::BPW;
     if (g > grid) g = grid;
     KernelTimer kt(st);
-    hipLaunchKernelGGL((k_wave_seg<NN, P>), dim3((unsigned)g), dim3(WG_SIZE), 0, st, a);
+    hipLaunchKernelGGL((k_wave_seg<NN, P, PK>), dim3((unsigned)g), dim3(WG_SIZE), 0, st, a);
     GBGPU_CHECK_LAUNCH("k_wave_seg");
     return GBGPU_OK;
 }
 
-int launch_ll_segment(int N, int precision, const WaveArgs &a, hipStream_t st) {
-#define GBGPU_SEG(NN)                                                                                   \
-    case NN:                                                                                            \
-        return precision == GBGPU_MIXED ? launch_segment_n<Mixed, NN>(a, st) : launch_segment_n<F64, NN>(a, st);
+// precision: arithmetic of the pass; pack_precision: layout of a.pack.
+int launch_ll_segment(int N, int precision, int pack_precision, const WaveArgs &a, hipStream_t st) {
+    if (precision == GBGPU_MIXED && pack_precision != GBGPU_MIXED) {
+        set_error("mixed segments need mixed per-binary constants");
+        return GBGPU_E_ARG;
+    }
+#define GBGPU_SEG(NN)                                                                           \
+    case NN:                                                                                    \
+        if (precision == GBGPU_MIXED) return launch_segment_n<Mixed, Mixed, NN>(a, st);         \
+        if (pack_precision == GBGPU_MIXED) return launch_segment_n<F64, Mixed, NN>(a, st);      \
+        return launch_segment_n<F64, F64, NN>(a, st);
     switch (N) {
         GBGPU_SEG(32)
         GBGPU_SEG(64)

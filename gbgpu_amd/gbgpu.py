@@ -431,9 +431,10 @@ class GBGPUBase(abc.ABC):
             raise NotImplementedError("per-binary N supports circular binaries, TDI1 and a single data stream")
         if len(N) != params.shape[1]:
             raise ValueError("N must have one entry per binary")
+        # same arithmetic as the uniform get_ll (no FP64 re-evaluation of loud binaries)
         bl = BatchedLogLike(self, data, psd, T, dt, precision=precision or self.precision,
                             start_freq_ind=start_freq_ind, N_levels=(int(N.min()),), f_edges=(),
-                            phase_marginalize=phase_marginalize)
+                            phase_marginalize=phase_marginalize, guard_tol=None)
         ll, d_h, h_h = bl(params, N=N.astype(np.int32), d_h=True)
         self.num_bin, self.N, self.df = params.shape[1], N, 1 / bl.T
         if phase_marginalize:

@@ -79,8 +79,10 @@ int gbgpu_get_ll(gbgpu_cmplx *d_h, gbgpu_cmplx *h_h, const gbgpu_cmplx *A_templa
 
 /* Replaces fill_global_wrap (src/gbgpu/cutils/gbgpu_utils.hh:6-7,
  * gbgpu_utils.cu:104-122): A_glob[g*L + start_b + i] += template[i*num_bin + b]
- * for 0 <= start_b + i < L.  Deterministic: each output bin is summed by one
- * thread in (start, binary) order, so results are bit-identical across runs
+ * for 0 <= start_b + i < L.  Deterministic: binaries are sorted by (group,
+ * start) and each output bin adds its contributions in a fixed order (four
+ * contiguous runs of the sorted list, combined in order), so results are
+ * bit-identical across runs
  * (the reference's GPU build uses unordered atomics).  num_groups (the
  * leading dimension of the caller's templates array, templates.shape[0] at
  * src/gbgpu/gbgpu.py:778) is the one argument the reference does not pass;
@@ -173,6 +175,9 @@ typedef struct gbgpu_batch_cfg {
     const double *orbit_tables[7]; /* table for N = 32 << k (k < 7), NULL when no binary uses it */
     int strategy;                  /* GBGPU_BATCH_BUCKETS: one launch per N bucket (default);
                                       GBGPU_BATCH_PERSISTENT: one persistent launch */
+    double guard_tol;              /* mixed precision with GBGPU_BATCH_BUCKETS: binaries whose
+                                      log-likelihood error bound 2.5e-6 (<h|h> + |<d|h>|) exceeds
+                                      guard_tol are re-evaluated in FP64; <= 0 disables */
 } gbgpu_batch_cfg;
 
 /* Workspace bytes gbgpu_wave_ll_batched needs for num_bin binaries. */
@@ -195,6 +200,11 @@ int gbgpu_wave_ll_batched(double *loglike, gbgpu_cmplx *d_h, gbgpu_cmplx *h_h, c
                           int num_params, int num_bin, const int *N_bin, const gbgpu_batch_cfg *cfg,
                           const gbgpu_data *data, double d_d, int phase_marginalize, void *workspace,
                           void *stream);
+
+/* Per-bucket counts (7 ints, N = 32 << k) of binaries the accuracy guard
+ * re-evaluated in FP64 in the last gbgpu_wave_ll_batched call that used this
+ * workspace.  Diagnostic: copies to host memory and synchronises `stream`. */
+int gbgpu_batch_guard_counts(const void *workspace, int num_bin, int precision, int *counts, void *stream);
 
 /* Sampler utility with no counterpart in the reference: seeded Gaussian
  * relative jumps out = in * (1 + sigma_rel * z) for every parameter row

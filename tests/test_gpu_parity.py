@@ -510,6 +510,38 @@ def test_uneven_batch_strategies_agree_at_scale():
     assert np.array_equal(out["buckets"][sel], ref)
 
 
+def test_mixed_batch_accuracy_guard():
+    """Guarded mixed batch: every log-likelihood within 1e-3 of the FP64 batch;
+    re-evaluated (loud) binaries bit-identical to it; quiet ones within the
+    guard's bound."""
+    from gbgpu_amd.batch import BatchedLogLike
+
+    B = 4000
+    p, N, rng = _uneven_batch(B, 660)
+    p[0, :200] = 10 ** rng.uniform(-21.5, -21, 200)  # loud templates
+    T = 4 * YR
+    A, E, SA, SE, df, L = make_data(p[:, :40], 256, T, DT, rng)
+    gb = _gb(precision="fp64")
+    gb.d_d = float(4 * df * (np.sum(np.abs(A) ** 2 / SA) + np.sum(np.abs(E) ** 2 / SE)))
+    print(f"guarded mixed batch B={B}", flush=True)
+    f64 = BatchedLogLike(gb, [A, E], [SA, SE], T, DT, precision="fp64")
+    ll64, dh64, hh64 = (x.cpu().numpy() for x in f64(p, N=N, d_h=True))
+    raw = BatchedLogLike(gb, [A, E], [SA, SE], T, DT, precision="mixed", guard_tol=None)
+    llraw = raw(p, N=N).cpu().numpy()
+    g = BatchedLogLike(gb, [A, E], [SA, SE], T, DT, precision="mixed", guard_tol=1e-3)
+    llg = g(p, N=N).cpu().numpy()
+    counts = g.guard_counts()
+    bound = 2.5e-6 * (np.abs(hh64) + np.abs(dh64))
+    loud = bound > 1e-3
+    print("guard counts", counts, "loud (FP64 estimate)", int(loud.sum()), "max raw err",
+          np.abs(llraw - ll64).max(), flush=True)
+    assert sum(counts.values()) >= int(loud.sum()) * 0.9 and sum(counts.values()) > 0
+    assert np.abs(llraw - ll64).max() > 1e-3  # without the guard the loud ones miss
+    assert np.abs(llg - ll64).max() <= 1e-3
+    redone = llg == ll64
+    assert np.all(redone[loud & (bound > 2e-3)])  # clearly loud: re-evaluated in FP64
+
+
 @pytest.mark.parametrize("strategy", ["buckets", "persistent"])
 def test_graph_captured_loop_replays_match_eager(strategy):
     """Captured loop of jump + batched likelihood: every replay equals the eager loop."""
