@@ -185,6 +185,7 @@ def uneven_batch(gb, A, E, SA, SE, d_d, B, T, dt, n_calls, seed=77):
             print(f"[bench] uneven batch {prec} {strat}", file=sys.stderr, flush=True)
             bl = BatchedLogLike(gb, [A, E], [SA, SE], T, dt, precision=prec.split("_")[0], strategy=strat,
                                 guard_tol=guard)
+            shapes = bl.autotune(p) if (strat == "buckets" and prec in ("fp64", "mixed")) else None
             g, out = bl.capture_loop(p, n_calls, sigma=1e-3, seed=seed)
             g.replay()
             torch.cuda.synchronize()
@@ -198,6 +199,8 @@ def uneven_batch(gb, A, E, SA, SE, d_d, B, T, dt, n_calls, seed=77):
             torch.cuda.synchronize()
             sec = ev0.elapsed_time(ev1) / 1e3 / reps
             r = {"binaries_per_s": n_calls * B / sec, "ms_per_call": sec / n_calls * 1e3}
+            if shapes is not None:
+                r["autotuned_tile_shape_per_N"] = shapes
             if guard:
                 r["guard_tol"] = guard
                 r["fp64_reevaluated_per_call"] = sum(bl.guard_counts().values())

@@ -114,6 +114,12 @@ SIGNATURES = {
          ctypes.POINTER(Data), c_double, c_int, c_void_p, c_void_p],
     ),
     "gbgpu_batch_guard_counts": (c_int, [c_void_p, c_int, c_int, c_void_p, c_void_p]),
+    "gbgpu_batch_set_variants": (c_int, [c_int, c_void_p]),
+    "gbgpu_batch_autotune": (
+        c_int,
+        [c_void_p, c_void_p, c_int, c_int, c_void_p, ctypes.POINTER(BatchCfg), ctypes.POINTER(Data), c_double,
+         c_void_p, c_void_p, c_void_p],
+    ),
     "gbgpu_perturb_params": (
         c_int,
         [c_void_p, c_void_p, c_int, c_int, c_double, ctypes.c_ulonglong, ctypes.c_ulonglong, c_void_p, c_int,

@@ -97,6 +97,37 @@ class BatchedLogLike:
             self._ws = torch.empty(int(need), dtype=torch.uint8, device=self.dev)
         return self._ws
 
+    def autotune(self, params, N=None):
+        """Time both tile shapes per N bucket on this batch and keep the faster
+        (process-wide, per precision); returns {N: shape} (0: min(N/4, 256)
+        threads per binary, 1: half of that).  Synchronises; not capturable."""
+        torch = _torch()
+        p = params if torch.is_tensor(params) else torch.from_numpy(np.ascontiguousarray(params, np.float64))
+        p = p.to(self.dev).contiguous()
+        B = p.shape[1]
+        if N is None:
+            N = self.N_of(p[1].cpu().numpy())
+        Nn = np.asarray(N.cpu().numpy() if torch.is_tensor(N) else N, dtype=np.int32)
+        for n in np.unique(Nn):
+            self._add_table(int(n))
+        Nt = torch.from_numpy(Nn).to(self.dev)
+        ll = torch.empty(B, dtype=torch.float64, device=self.dev)
+        ws = self._workspace(B)
+        chosen = np.zeros(7, dtype=np.int32)
+        rc = _capi.load().gbgpu_batch_autotune(ll.data_ptr(), p.data_ptr(), 9, B, Nt.data_ptr(), self.cfg, self.data,
+                                               float(np.real(self.gb.d_d)), ws.data_ptr(), chosen.ctypes.data,
+                                               _capi.stream_handle())
+        _capi.check(rc, "gbgpu_batch_autotune")
+        return {32 << k: int(c) for k, c in enumerate(chosen) if c >= 0}
+
+    def set_shapes(self, shapes):
+        """Pin the per-bucket tile shapes ({N: 0 or 1}; others 0) for this precision."""
+        v = np.zeros(7, dtype=np.int32)
+        for n, c in shapes.items():
+            v[int(np.log2(n)) - 5] = c
+        _capi.check(_capi.load().gbgpu_batch_set_variants(self.cfg.precision, v.ctypes.data),
+                    "gbgpu_batch_set_variants")
+
     def guard_counts(self):
         """Binaries per N (dict) the accuracy guard re-evaluated in FP64 in the
         last call (synchronises the current stream)."""

@@ -31,7 +31,7 @@ namespace gbgpu {
 int check_data(const gbgpu_data *data);
 void set_data(WaveArgs &a, const gbgpu_data *data);
 
-int launch_ll_segment(int N, int precision, int pack_precision, const WaveArgs &a, hipStream_t st);
+int launch_ll_segment(int N, int precision, int pack_precision, int variant, const WaveArgs &a, hipStream_t st);
 
 constexpr int NBUCKET = 7;  // N = 32 << k, k < 7
 constexpr int NCB = 4096;   // start-bin cells per bucket (log-spaced)
@@ -323,17 +323,27 @@ static int launch_persistent(BatchArgs &ba, hipStream_t st) {
     return GBGPU_OK;
 }
 
+// Tile shape of the per-bucket kernel, per (precision, bucket): 0 = Geo<N>,
+// 1 = Geo<N, 2>; set by gbgpu_batch_autotune.
+static int g_variant[2][NBUCKET] = {};
+
 // One launch of the uniform fused kernel per populated bucket.  With the
 // accuracy guard (mixed precision, guard_tol > 0), each mixed launch lists
 // the binaries whose error bound exceeds guard_tol and an FP64 launch over
 // that list (reading the same mixed per-binary constants) overwrites their
 // results.
+// variants: tile shape per bucket (NULL: the tuned table); bucket_events:
+// when non-NULL, a pair of HIP events is recorded around each bucket's
+// launches (2 * NBUCKET entries, created by the caller).
 template <class P>
-static int launch_buckets(const BatchArgs &ba, int precision, hipStream_t st) {
+static int launch_buckets(const BatchArgs &ba, int precision, hipStream_t st, const int *variants,
+                          hipEvent_t *bucket_events) {
     const bool guard = precision == GBGPU_MIXED && ba.guard_tol > 0.0;
     for (int k = NBUCKET - 1; k >= 0; k--) {
         if (!ba.tabs[k]) continue;
         const int N = 32 << k;
+        const int var = variants ? variants[k] : g_variant[precision == GBGPU_MIXED ? 1 : 0][k];
+        if (bucket_events) (void)hipEventRecord(bucket_events[2 * k], st);
         WaveArgs a = ba.a;
         a.tab = ba.tabs[k];
         a.tw = ba.tws[k];
@@ -346,7 +356,7 @@ static int launch_buckets(const BatchArgs &ba, int precision, hipStream_t st) {
             a.redo_count = ba.ctl + CTL_REDO + k;
             a.redo_tol = ba.guard_tol;
         }
-        int rc = launch_ll_segment(N, precision, precision, a, st);
+        int rc = launch_ll_segment(N, precision, precision, var, a, st);
         if (rc) return rc;
         if (guard) {
             WaveArgs r = a;
@@ -356,15 +366,17 @@ static int launch_buckets(const BatchArgs &ba, int precision, hipStream_t st) {
             r.seg_count = ba.ctl + CTL_REDO + k;
             r.tw = twiddles_f64(N);
             if (!r.tw) return hip_fail(hipErrorOutOfMemory, "twiddles");
-            rc = launch_ll_segment(N, GBGPU_FP64, GBGPU_MIXED, r, st);
+            rc = launch_ll_segment(N, GBGPU_FP64, GBGPU_MIXED, var, r, st);
             if (rc) return rc;
         }
+        if (bucket_events) (void)hipEventRecord(bucket_events[2 * k + 1], st);
     }
     return GBGPU_OK;
 }
 
 template <class P>
-static int launch_batch(BatchArgs ba, int precision, int strategy, void *workspace, hipStream_t st) {
+static int launch_batch(BatchArgs ba, int precision, int strategy, void *workspace, hipStream_t st,
+                        const int *variants = nullptr, hipEvent_t *bucket_events = nullptr) {
     const long long B = ba.a.B;
     char *ws = static_cast<char *>(workspace);
     bool own = false;
@@ -389,7 +401,7 @@ static int launch_batch(BatchArgs ba, int precision, int strategy, void *workspa
     hipError_t e = hipGetLastError();
     int rc = e == hipSuccess ? GBGPU_OK : hip_fail(e, "batch set-up");
     if (!rc) rc = strategy == GBGPU_BATCH_PERSISTENT ? launch_persistent<P>(ba, st)
-                                                     : launch_buckets<P>(ba, precision, st);
+                                                     : launch_buckets<P>(ba, precision, st, variants, bucket_events);
     hipError_t e2 = own ? hipFreeAsync(ws, st) : hipSuccess;
     if (rc) return rc;
     if (e2 != hipSuccess) return hip_fail(e2, "hipFreeAsync(batch workspace)");
@@ -445,10 +457,9 @@ extern "C" long long gbgpu_batch_workspace_bytes(int num_bin, int precision) {
                                     : (long long)batch_ws_bytes<F64>(num_bin);
 }
 
-extern "C" int gbgpu_wave_ll_batched(double *loglike, gbgpu_cmplx *d_h, gbgpu_cmplx *h_h, const double *params,
-                                     int num_params, int num_bin, const int *N_bin, const gbgpu_batch_cfg *cfg,
-                                     const gbgpu_data *data, double d_d, int phase_marginalize, void *workspace,
-                                     void *stream) {
+static int make_batch_args(BatchArgs &ba, double *loglike, gbgpu_cmplx *d_h, gbgpu_cmplx *h_h,
+                           const double *params, int num_params, int num_bin, const int *N_bin,
+                           const gbgpu_batch_cfg *cfg, const gbgpu_data *data, double d_d, int phase_marginalize) {
     if (!cfg || !params || !N_bin || !loglike) {
         set_error("null cfg, params, N_bin or loglike");
         return GBGPU_E_ARG;
@@ -462,10 +473,13 @@ extern "C" int gbgpu_wave_ll_batched(double *loglike, gbgpu_cmplx *d_h, gbgpu_cm
         set_error("bad num_bin, T, armlength or precision");
         return GBGPU_E_ARG;
     }
+    if (cfg->strategy != GBGPU_BATCH_BUCKETS && cfg->strategy != GBGPU_BATCH_PERSISTENT) {
+        set_error("unknown batch strategy");
+        return GBGPU_E_ARG;
+    }
     int rc = check_data(data);
     if (rc) return rc;
-    if (num_bin == 0) return GBGPU_OK;
-    BatchArgs ba = {};
+    ba = BatchArgs{};
     WaveArgs &a = ba.a;
     a.params = params;
     a.num_params = num_params;
@@ -494,13 +508,88 @@ extern "C" int gbgpu_wave_ll_batched(double *loglike, gbgpu_cmplx *d_h, gbgpu_cm
                 return hip_fail(hipErrorOutOfMemory, "twiddles");
         }
     }
-    if (cfg->strategy != GBGPU_BATCH_BUCKETS && cfg->strategy != GBGPU_BATCH_PERSISTENT) {
-        set_error("unknown batch strategy");
-        return GBGPU_E_ARG;
-    }
+    return GBGPU_OK;
+}
+
+extern "C" int gbgpu_wave_ll_batched(double *loglike, gbgpu_cmplx *d_h, gbgpu_cmplx *h_h, const double *params,
+                                     int num_params, int num_bin, const int *N_bin, const gbgpu_batch_cfg *cfg,
+                                     const gbgpu_data *data, double d_d, int phase_marginalize, void *workspace,
+                                     void *stream) {
+    BatchArgs ba;
+    int rc = make_batch_args(ba, loglike, d_h, h_h, params, num_params, num_bin, N_bin, cfg, data, d_d,
+                             phase_marginalize);
+    if (rc) return rc;
+    if (num_bin == 0) return GBGPU_OK;
     hipStream_t st = static_cast<hipStream_t>(stream);
     return cfg->precision == GBGPU_MIXED ? launch_batch<Mixed>(ba, cfg->precision, cfg->strategy, workspace, st)
                                          : launch_batch<F64>(ba, cfg->precision, cfg->strategy, workspace, st);
+}
+
+extern "C" int gbgpu_batch_autotune(double *loglike, const double *params, int num_params, int num_bin,
+                                    const int *N_bin, const gbgpu_batch_cfg *cfg, const gbgpu_data *data,
+                                    double d_d, void *workspace, int *chosen, void *stream) {
+    BatchArgs ba;
+    int rc = make_batch_args(ba, loglike, nullptr, nullptr, params, num_params, num_bin, N_bin, cfg, data, d_d, 0);
+    if (rc) return rc;
+    if (cfg->strategy != GBGPU_BATCH_BUCKETS) {
+        set_error("the autotuner tunes the per-bucket launches (GBGPU_BATCH_BUCKETS)");
+        return GBGPU_E_ARG;
+    }
+    const int pi = cfg->precision == GBGPU_MIXED ? 1 : 0;
+    hipStream_t st = static_cast<hipStream_t>(stream);
+    hipEvent_t ev[2 * NBUCKET];
+    for (int i = 0; i < 2 * NBUCKET; i++)
+        if (hipEventCreate(&ev[i]) != hipSuccess) return hip_fail(hipErrorOutOfMemory, "hipEventCreate");
+    constexpr int NVAR = 2, REPS = 3;
+    float best[NVAR][NBUCKET];
+    for (int v = 0; v < NVAR; v++)
+        for (int k = 0; k < NBUCKET; k++) best[v][k] = 3.0e38f;
+    for (int v = 0; v < NVAR && !rc; v++) {
+        int var[NBUCKET];
+        for (int k = 0; k < NBUCKET; k++) var[k] = v;
+        for (int r = 0; r < REPS && !rc; r++) {  // the first repetition warms up
+            rc = cfg->precision == GBGPU_MIXED
+                     ? launch_batch<Mixed>(ba, cfg->precision, cfg->strategy, workspace, st, var, ev)
+                     : launch_batch<F64>(ba, cfg->precision, cfg->strategy, workspace, st, var, ev);
+            if (rc) break;
+            hipError_t e = hipStreamSynchronize(st);
+            if (e != hipSuccess) {
+                rc = hip_fail(e, "autotune");
+                break;
+            }
+            for (int k = 0; k < NBUCKET && r > 0; k++) {
+                if (!ba.tabs[k]) continue;
+                float ms = 0.0f;
+                if (hipEventElapsedTime(&ms, ev[2 * k], ev[2 * k + 1]) == hipSuccess && ms < best[v][k])
+                    best[v][k] = ms;
+            }
+        }
+    }
+    for (int i = 0; i < 2 * NBUCKET; i++) (void)hipEventDestroy(ev[i]);
+    if (rc) return rc;
+    for (int k = 0; k < NBUCKET; k++) {
+        int pick = -1;
+        if (ba.tabs[k]) {
+            pick = best[1][k] < best[0][k] ? 1 : 0;
+            g_variant[pi][k] = pick;
+        }
+        if (chosen) chosen[k] = pick;
+    }
+    return GBGPU_OK;
+}
+
+extern "C" int gbgpu_batch_set_variants(int precision, const int *variants) {
+    if ((precision != GBGPU_FP64 && precision != GBGPU_MIXED) || !variants) {
+        set_error("gbgpu_batch_set_variants: bad arguments");
+        return GBGPU_E_ARG;
+    }
+    for (int k = 0; k < NBUCKET; k++)
+        if (variants[k] != 0 && variants[k] != 1) {
+            set_error("gbgpu_batch_set_variants: variants are 0 or 1");
+            return GBGPU_E_ARG;
+        }
+    for (int k = 0; k < NBUCKET; k++) g_variant[precision == GBGPU_MIXED ? 1 : 0][k] = variants[k];
+    return GBGPU_OK;
 }
 
 extern "C" int gbgpu_batch_guard_counts(const void *workspace, int num_bin, int precision, int *counts,

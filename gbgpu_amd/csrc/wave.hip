@@ -27,9 +27,9 @@ constexpr bool fits_lds() {
 // LDS of one workgroup of k_wave.  PK is the precision policy whose Pack
 // layout the per-binary constants were written in (normally P; the FP64
 // re-evaluation of guarded mixed binaries reads the mixed packs).
-template <int N, int MODE, bool THIRD, class P, class PK = P>
+template <int N, int MODE, bool THIRD, class P, class PK = P, int M = 1>
 struct WaveLds {
-    using G = Geo<N>;
+    using G = Geo<N, M>;
     using CT = typename P::CT;
     static constexpr int NCH = (MODE == MODE_WAVE) ? 3 : 2;
     static constexpr int NT = THIRD ? N : 1;
@@ -54,12 +54,12 @@ __device__ __forceinline__ typename P::Aux aux_of(const Pack<PK> &pk) {
         return P::make_aux(pk.s);
 }
 
-template <int N, int MODE, bool THIRD, class P, class PK = P>
-__device__ __forceinline__ void wave_pass(const WaveArgs &a, WaveLds<N, MODE, THIRD, P, PK> &L, long long blk,
+template <int N, int MODE, bool THIRD, class P, class PK = P, int M = 1>
+__device__ __forceinline__ void wave_pass(const WaveArgs &a, WaveLds<N, MODE, THIRD, P, PK, M> &L, long long blk,
                                           long long count, long long first, int tid, const double *tab,
                                           const void *twp) {
-    using G = Geo<N>;
-    using W = WaveLds<N, MODE, THIRD, P, PK>;
+    using G = Geo<N, M>;
+    using W = WaveLds<N, MODE, THIRD, P, PK, M>;
     using CT = typename P::CT;
     constexpr int TPB = G::TPB, BPW = G::BPW, WPB = G::WPB, SPT = G::SPT;
     constexpr int NCH = W::NCH, NT = W::NT, NXI = W::NXI;
@@ -84,8 +84,8 @@ __device__ __forceinline__ void wave_pass(const WaveArgs &a, WaveLds<N, MODE, TH
     ThirdScratch ts{XI_IN_BUF ? reinterpret_cast<double *>(buf) : s_xi + slot * 3 * NXI, s_v + slot * 3 * NT,
                     s_g + slot * 3 * NT};
 
-    binary_spectrum<N, NCH, THIRD, P>(s, aux_of<P, PK>(spack[slot]), valid, lane, tab, a.fstar_inv, a.scale, buf,
-                                      static_cast<const CT *>(twp), ts, tot);
+    binary_spectrum<N, NCH, THIRD, P, M>(s, aux_of<P, PK>(spack[slot]), valid, lane, tab, a.fstar_inv, a.scale,
+                                         buf, static_cast<const CT *>(twp), ts, tot);
 
     const double2 fac = P::fac(s);
     if constexpr (MODE == MODE_WAVE) {
@@ -117,7 +117,7 @@ __device__ __forceinline__ void wave_pass(const WaveArgs &a, WaveLds<N, MODE, TH
         if (valid) {
             const long long doff = (a.di ? (long long)a.di[b] : 0LL) * a.L;
             const long long noff = (a.ni ? (long long)a.ni[b] : 0LL) * a.L;
-            ll_partial<N, P>(buf, fac, lane, s.start - a.start_freq_ind, doff, noff, a, dhr, dhi, hh);
+            ll_partial<N, P, M>(buf, fac, lane, s.start - a.start_freq_ind, doff, noff, a, dhr, dhi, hh);
         }
         dhr = group_total<TPB>(dhr, lane, tot);
         dhi = group_total<TPB>(dhi, lane, tot);
@@ -150,10 +150,10 @@ __global__ void __launch_bounds__(WG_SIZE) k_wave(WaveArgs a) {
 // arithmetic are the same in every pass: opaque copies of the pointers and
 // of the thread index keep the compiler from hoisting all of them out of the
 // loop into registers (which would cost occupancy).
-template <int N, class P, class PK>
+template <int N, class P, class PK, int M>
 __global__ void __launch_bounds__(WG_SIZE) k_wave_seg(WaveArgs a) {
-    constexpr int BPW = Geo<N>::BPW;
-    __shared__ WaveLds<N, MODE_LL, false, P, PK> lds;
+    constexpr int BPW = Geo<N, M>::BPW;
+    __shared__ WaveLds<N, MODE_LL, false, P, PK, M> lds;
     const long long count = *a.seg_count;
     const long long first = *a.seg_off;
     for (long long blk = blockIdx.x; blk * BPW < count; blk += gridDim.x) {
@@ -161,7 +161,7 @@ __global__ void __launch_bounds__(WG_SIZE) k_wave_seg(WaveArgs a) {
         const void *twp = a.tw;
         int tid = threadIdx.x;
         asm volatile("" : "+s"(tab), "+s"(twp), "+v"(tid));
-        wave_pass<N, MODE_LL, false, P, PK>(a, lds, blk, count, first, tid, tab, twp);
+        wave_pass<N, MODE_LL, false, P, PK, M>(a, lds, blk, count, first, tid, tab, twp);
         __syncthreads();  // LDS is reused by the next pass
     }
 }
@@ -203,7 +203,7 @@ static int launch_wave(int N, const WaveArgs &a, hipStream_t st) {
 // launches of gbgpu_wave_ll_batched): a.pack, a.order, a.seg_count and
 // a.seg_off are set by the caller; the segment's size is known only on the
 // device, so the grid covers a few passes of resident workgroups and strides.
-template <class P, class PK, int NN>
+template <class P, class PK, int NN, int M>
 static int launch_segment_n(const WaveArgs &a, hipStream_t st) {
     static int grid = 0;
     if (!grid) {
@@ -212,25 +212,31 @@ static int launch_segment_n(const WaveArgs &a, hipStream_t st) {
         (void)hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev);
         grid = 16 * (ncu > 0 ? ncu : 256);  // several passes' worth of resident workgroups
     }
-    long long g = (a.B + Geo<NN>::BPW - 1) / Geo<NN>::BPW;
+    long long g = (a.B + Geo<NN, M>::BPW - 1) / Geo<NN, M>::BPW;
     if (g > grid) g = grid;
     KernelTimer kt(st);
-    hipLaunchKernelGGL((k_wave_seg<NN, P, PK>), dim3((unsigned)g), dim3(WG_SIZE), 0, st, a);
+    hipLaunchKernelGGL((k_wave_seg<NN, P, PK, M>), dim3((unsigned)g), dim3(WG_SIZE), 0, st, a);
     GBGPU_CHECK_LAUNCH("k_wave_seg");
     return GBGPU_OK;
 }
 
-// precision: arithmetic of the pass; pack_precision: layout of a.pack.
-int launch_ll_segment(int N, int precision, int pack_precision, const WaveArgs &a, hipStream_t st) {
+template <class P, class PK, int NN>
+static int launch_segment_m(int variant, const WaveArgs &a, hipStream_t st) {
+    return variant == 1 ? launch_segment_n<P, PK, NN, 2>(a, st) : launch_segment_n<P, PK, NN, 1>(a, st);
+}
+
+// precision: arithmetic of the pass; pack_precision: layout of a.pack;
+// variant: tile shape (0: Geo<N>, 1: Geo<N, 2>).
+int launch_ll_segment(int N, int precision, int pack_precision, int variant, const WaveArgs &a, hipStream_t st) {
     if (precision == GBGPU_MIXED && pack_precision != GBGPU_MIXED) {
         set_error("mixed segments need mixed per-binary constants");
         return GBGPU_E_ARG;
     }
 #define GBGPU_SEG(NN)                                                                           \
     case NN:                                                                                    \
-        if (precision == GBGPU_MIXED) return launch_segment_n<Mixed, Mixed, NN>(a, st);         \
-        if (pack_precision == GBGPU_MIXED) return launch_segment_n<F64, Mixed, NN>(a, st);      \
-        return launch_segment_n<F64, F64, NN>(a, st);
+        if (precision == GBGPU_MIXED) return launch_segment_m<Mixed, Mixed, NN>(variant, a, st); \
+        if (pack_precision == GBGPU_MIXED) return launch_segment_m<F64, Mixed, NN>(variant, a, st); \
+        return launch_segment_m<F64, F64, NN>(variant, a, st);
     switch (N) {
         GBGPU_SEG(32)
         GBGPU_SEG(64)

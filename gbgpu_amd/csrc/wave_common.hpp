@@ -14,12 +14,14 @@ enum { MODE_WAVE = 0, MODE_LL = 1 };
 
 constexpr int WG_SIZE = 256;
 
-// A group of TPB = min(N/4, 256) threads owns one binary; each thread holds
-// SPT = N/TPB samples (4 up to N = 1024, 8 at 2048).  Workgroups are 256
-// threads holding BPW binaries.
-template <int N>
+// A group of TPB = min(N/4, 256) / M threads owns one binary; each thread
+// holds SPT = N/TPB samples (4M up to N = 1024, 8M at 2048).  Workgroups are
+// 256 threads holding BPW binaries.  M = 1 is the default tile; M = 2 (half
+// the threads per binary, twice the binaries per workgroup) is the other
+// shape the batch autotuner can pick.
+template <int N, int M = 1>
 struct Geo {
-    static constexpr int TPB = (N / 4 < WG_SIZE) ? N / 4 : WG_SIZE;
+    static constexpr int TPB = ((N / 4 < WG_SIZE) ? N / 4 : WG_SIZE) / M;
     static constexpr int WG = WG_SIZE;
     static constexpr int BPW = WG / TPB;
     static constexpr int SPT = N / TPB;
@@ -155,12 +157,12 @@ struct ThirdScratch {
 // natural FFT order, before fftshift and before P::fac()).  NCH == 3 gives
 // X, Y, Z; NCH == 2 gives A, E.  Invalid slots (beyond the batch) write
 // zeros and still take part in every barrier.
-template <int N, int NCH, bool THIRD, class P>
+template <int N, int NCH, bool THIRD, class P, int M = 1>
 __device__ __forceinline__ void binary_spectrum(const Src &s, const typename P::Aux &aux, bool valid, int lane,
                                                 const double *__restrict__ tab, double fstar_inv, double scale,
                                                 typename P::CT *buf, const typename P::CT *__restrict__ tw,
                                                 ThirdScratch ts, double *tot) {
-    using G = Geo<N>;
+    using G = Geo<N, M>;
     using CT = typename P::CT;
     constexpr int TPB = G::TPB, SPT = G::SPT;
     if constexpr (THIRD) {
@@ -246,11 +248,11 @@ __device__ __forceinline__ void binary_spectrum(const Src &s, const typename P::
 // Accumulates this thread's share of <d|h> (complex) and <h|h> for a binary
 // whose A/E spectra sit in buf (natural order), over output bins
 // m = lane + q TPB (fftshifted), data bin j = j0 + m.
-template <int N, class P>
+template <int N, class P, int M = 1>
 __device__ __forceinline__ void ll_partial(const typename P::CT *buf, double2 fac, int lane, long long j0,
                                            long long doff, long long noff, const WaveArgs &a, double &dhr,
                                            double &dhi, double &hh) {
-    using G = Geo<N>;
+    using G = Geo<N, M>;
 #pragma unroll
     for (int q = 0; q < G::SPT; q++) {
         const int m = lane + q * G::TPB;

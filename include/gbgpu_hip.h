@@ -201,6 +201,22 @@ int gbgpu_wave_ll_batched(double *loglike, gbgpu_cmplx *d_h, gbgpu_cmplx *h_h, c
                           const gbgpu_data *data, double d_d, int phase_marginalize, void *workspace,
                           void *stream);
 
+/* Autotuner of the per-bucket launches (GBGPU_BATCH_BUCKETS): runs the batch
+ * with each tile shape (0: min(N/4, 256) threads per binary; 1: half as many
+ * threads per binary, twice the binaries per workgroup) three times, times
+ * every bucket with HIP events, and keeps the faster shape per (precision,
+ * bucket) for all later calls in the process.  chosen (7 ints, N = 32 << k,
+ * may be NULL) receives the pick, -1 for buckets without an orbit table.
+ * loglike receives the last run's results.  Synchronises `stream`; call it
+ * outside graph capture.  No counterpart in the reference. */
+int gbgpu_batch_autotune(double *loglike, const double *params, int num_params, int num_bin, const int *N_bin,
+                         const gbgpu_batch_cfg *cfg, const gbgpu_data *data, double d_d, void *workspace,
+                         int *chosen, void *stream);
+
+/* Sets the per-bucket tile shapes (7 ints, 0 or 1 as for the autotuner) used
+ * by GBGPU_BATCH_BUCKETS calls of the given precision. */
+int gbgpu_batch_set_variants(int precision, const int *variants);
+
 /* Per-bucket counts (7 ints, N = 32 << k) of binaries the accuracy guard
  * re-evaluated in FP64 in the last gbgpu_wave_ll_batched call that used this
  * workspace.  Diagnostic: copies to host memory and synchronises `stream`. */

@@ -542,6 +542,38 @@ def test_mixed_batch_accuracy_guard():
     assert np.all(redone[loud & (bound > 2e-3)])  # clearly loud: re-evaluated in FP64
 
 
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_batch_autotune_shapes_agree(precision):
+    """Both tile shapes give the same log-likelihoods (to summation order);
+    the autotuner's pick is reproducible in its results."""
+    from gbgpu_amd.batch import BatchedLogLike
+
+    B = 20_000
+    p, N, rng = _uneven_batch(B, 670)
+    T = 4 * YR
+    A, E, SA, SE, df, L = make_data(p[:, :10], 256, T, DT, rng)
+    gb = _gb(precision=precision)
+    gb.d_d = float(4 * df * (np.sum(np.abs(A) ** 2 / SA) + np.sum(np.abs(E) ** 2 / SE)))
+    print(f"autotune {precision} B={B}", flush=True)
+    bl = BatchedLogLike(gb, [A, E], [SA, SE], T, DT, precision=precision, guard_tol=None)
+    out = {}
+    try:
+        for shape in (0, 1):
+            bl.set_shapes({n: shape for n in (32, 64, 128, 256, 512, 1024, 2048)})
+            out[shape] = bl(p, N=N).cpu().numpy()
+        chosen = bl.autotune(p, N=N)
+        print("chosen", chosen, flush=True)
+        tuned = bl(p, N=N).cpu().numpy()
+        assert np.array_equal(tuned, bl(p, N=N).cpu().numpy())
+    finally:
+        bl.set_shapes({})  # back to the default shapes for the rest of the session
+    scale = np.abs(out[0]) + 1.0
+    assert np.max(np.abs(out[1] - out[0]) / scale) <= 1e-12
+    assert set(chosen) == set(int(n) for n in np.unique(N))
+    ref = np.where(np.isin(N, [n for n, c in chosen.items() if c == 1]), out[1], out[0])
+    assert np.max(np.abs(tuned - ref) / scale) <= 1e-12
+
+
 @pytest.mark.parametrize("strategy", ["buckets", "persistent"])
 def test_graph_captured_loop_replays_match_eager(strategy):
     """Captured loop of jump + batched likelihood: every replay equals the eager loop."""
