@@ -76,3 +76,17 @@ def test_fill_global_template_validation():
     with pytest.raises(TypeError):
         gb.fill_global_template(np.zeros(2, np.int32), np.zeros((1, 2, 10), complex), np.zeros((2, 4)),
                                 np.zeros((2, 4)), np.zeros(2, int))
+
+
+def test_bench_refuses_more_than_one_gpu():
+    """bench.py measures one GPU; --gpus N > 1 exits with a message, before touching a device."""
+    import json
+    import subprocess
+    import sys
+    from pathlib import Path
+
+    root = Path(__file__).resolve().parent.parent
+    r = subprocess.run([sys.executable, str(root / "bench.py"), "--gpus", "2"], capture_output=True, text=True,
+                       timeout=120)
+    assert r.returncode == 2
+    assert "one GPU" in json.loads(r.stdout.strip().splitlines()[-1])["error"]
