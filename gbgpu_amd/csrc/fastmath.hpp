@@ -54,7 +54,8 @@ __device__ __forceinline__ void sincos_f64(double x, double &s, double &c) {
     const double k = rint(x * TWO_OVER_PI);
     double y = fma(-k, P1, x);
     y = fma(-k, P2, y);
-    sincos_poly_f64(y, (int)(long long)k & 3, s, c);
+    // |k| < 2^31 for every argument here, so a 32-bit conversion is exact
+    sincos_poly_f64(y, (int)k & 3, s, c);
 }
 
 // 1/x to ~1 ulp: hardware estimate refined by two Newton steps.
@@ -100,7 +101,7 @@ __device__ __forceinline__ void sincos_f32(float x, float &s, float &c) {
 __device__ __forceinline__ void sincos_rev(double r, float &s, float &c) {
     const double k = rint(4.0 * r);
     const float y = (float)(r - 0.25 * k) * 6.28318530717958647692f;
-    sincos_poly(y, ((long long)k) & 3, s, c);
+    sincos_poly(y, ((int)k) & 3, s, c);  // |k| < 2^31: exact 32-bit conversion
 }
 
 // ------------------------------------------------- reduction-free series ---
@@ -172,14 +173,15 @@ __device__ __forceinline__ void transfer_series(R u, R &re, R &im) {
 
 // Series lengths per frequency tier.  Tier t applies when
 // u_max = 2 max|f| / f* (light-time and Doppler margin included) is at most
-// U_MAX[t]; the first omitted term is then below 1e-17 (FP64) or 3e-9
-// (FP32) for the transfer factor (|u| <= U_MAX), omL = f/f* (<= U_MAX/2) and
-// the mixed path's spacecraft phase offsets (<= 0.29 U_MAX rad).  Tier 3
-// falls back to range-reduced sine/cosine.
+// U_MAX[t]; the first omitted term is then below 7e-17 (FP64) or 3e-9
+// (FP32) for the transfer factor (|u| <= U_MAX; FP64 real part
+// v^KR/(2KR+1)! with v = U_MAX^2: 2.8e-17, 6.5e-17, 1.8e-17), omL = f/f*
+// (<= U_MAX/2) and the mixed path's spacecraft phase offsets
+// (<= 0.29 U_MAX rad).  Tier 3 falls back to range-reduced sine/cosine.
 constexpr double TIER_UMAX[3] = {0.75, 1.5, 2.25};
 template <int TIER>
 struct TierF64 {
-    static constexpr int KR = TIER == 0 ? 9 : TIER == 1 ? 11 : 13;
+    static constexpr int KR = TIER == 0 ? 8 : TIER == 1 ? 10 : 12;
     static constexpr int KI = TIER == 0 ? 8 : TIER == 1 ? 10 : 12;
     static constexpr int KS = TIER == 0 ? 7 : TIER == 1 ? 9 : 10;  // sincos of omL
 };
