@@ -67,6 +67,7 @@ __global__ void __launch_bounds__(WG_SIZE) k_grad(WaveArgs a, double *__restrict
     constexpr int TPB = G::TPB, BPW = G::BPW, WPB = G::WPB, SPT = G::SPT;
     __shared__ CT sbuf[BPW * 2 * N];
     __shared__ Pack<P> spack[BPW];
+    __shared__ SkyDeriv ssd[BPW];
     __shared__ double stot[BPW * WPB];
 
     const int tid = threadIdx.x;
@@ -127,21 +128,29 @@ __global__ void __launch_bounds__(WG_SIZE) k_grad(WaveArgs a, double *__restrict
     __syncthreads();
     fft_lds<N, 2, TPB>(buf, lane, tw);  // buf[n] = adjoint weight of sample n
 
+    // the backward pass reads the per-binary constants in place from LDS
+    // (register copies of them would cost occupancy)
+    if (valid && lane == 0) ssd[slot] = make_sky_deriv(a.params[7 * a.B + b], a.params[8 * a.B + b]);
+    __syncthreads();
     GradAcc acc;
     acc.SP = acc.SC = make_double2(0.0, 0.0);
 #pragma unroll
     for (int k = 0; k < NDIR; k++) acc.g[k] = 0.0;
     if (valid) {
-        const SkyDeriv sd = make_sky_deriv(a.params[7 * a.B + b], a.params[8 * a.B + b]);
         auto run = [&](auto tier) {
             constexpr int TIER = decltype(tier)::value;
 #pragma unroll 1
             for (int q = 0; q < SPT; q++) {
+                // an opaque copy of the slot index keeps the loop-invariant
+                // LDS reads of the constants inside the loop
+                int si = slot;
+                asm volatile("" : "+v"(si));
                 const int n = lane + q * TPB;
-                adjoint_sample<TIER>(s, sd, a.tab, N, n, a.fstar_inv, a.scale, buf[n], buf[N + n], acc);
+                adjoint_sample<TIER>(spack[si].s, ssd[si], a.tab, N, n, a.fstar_inv, a.scale, buf[n], buf[N + n],
+                                     acc);
             }
         };
-        switch (s.tier) {
+        switch (spack[slot].s.tier) {
             case 0: run(std::integral_constant<int, 0>{}); break;
             case 1: run(std::integral_constant<int, 1>{}); break;
             case 2: run(std::integral_constant<int, 2>{}); break;

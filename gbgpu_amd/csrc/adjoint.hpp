@@ -210,72 +210,131 @@ struct GradAcc {
 
 // Adds this sample's contribution.  yA, yE: adjoint weights of the sample's
 // A and E values (J = Re(yA A + yE E)); DP, DC: polarisation amplitudes.
+// The spacecraft are visited one at a time (their two links each), so only
+// the arm quantities, the TDI factors and the accumulators stay live.
 template <int TIER>
 __device__ __forceinline__ void adjoint_sample(const Src &s, const SkyDeriv &sd, const double *__restrict__ tab, int N,
                                                int n, double fsi, double scale, double2 yA, double2 yE,
                                                GradAcc &acc) {
-    Leaves lv;
-    make_leaves<TIER>(s, sd, tab, N, n, fsi, scale, lv);
+    constexpr double INV_C = 1.0 / C_SI;
+    const double t = tab[ROW_T * N + n];
     const double2 DP = make_double2(s.DPr, s.DPi), DC = make_double2(s.DCr, s.DCi);
+    // TDI factors F1 = e^{-i omL}, F2 = 4 omL sin(omL) scale e^{-i omL}, dF2/d omL
+    double2 F1, F2, dF2;
+    {
+        const double f = s.f0 + s.fdot * t + 0.5 * s.fddot * t * t;
+        const double omL = f * fsi;
+        double sL, cL;
+        if constexpr (TIER < 3)
+            sincos_series<TierF64<TIER>::KS>(omL, sL, cL);
+        else
+            sincos_f64(omL, sL, cL);
+        F1 = make_double2(cL, -sL);
+        const double m = 4.0 * omL * sL * scale;
+        const double dm = 4.0 * scale * (sL + omL * cL);
+        F2 = make_double2(m * cL, -m * sL);
+        dF2 = make_double2(dm * cL - m * sL, -dm * sL - m * cL);
+    }
     // channel weights: J = Re(wX X + wY Y + wZ Z), A = (Z - X)/sqrt2, E = (X - 2Y + Z)/sqrt6
     double2 w[3];
     w[0] = csub(cscale(yE, INV_SQRT6), cscale(yA, INV_SQRT2));
     w[1] = cscale(yE, -2.0 * INV_SQRT6);
     w[2] = cadd(cscale(yA, INV_SQRT2), cscale(yE, INV_SQRT6));
-    double2 Q1 = make_double2(0.0, 0.0), Q2 = Q1, SP = Q1, SC = Q1;
-    double U[3] = {0.0, 0.0, 0.0}, V[3] = {0.0, 0.0, 0.0};
-    double skr[2] = {0.0, 0.0};  // sum_links su * fonfs * sign * dkr
-    double sep[3] = {0.0, 0.0, 0.0}, sex[3] = {0.0, 0.0, 0.0};
-#pragma unroll
-    for (int L = 0; L < 6; L++) {
-        const int sc = LINK_SC[L], arm = LINK_ARM[L];
-        const double sg = LINK_SGN[L];
-        double2 T, dT;
-        transfer_and_deriv<TIER>(lv.fonfs[sc] * (1.0 + sg * lv.kr[arm]), T, dT);
-        const double2 e = cscale(cmul(T, lv.E[sc]), 0.25);
-        const double2 ed = cscale(cmul(dT, lv.E[sc]), 0.25);
-        const double2 Aarm = make_double2(lv.ep[arm] * DP.x + lv.ex[arm] * DC.x, lv.ep[arm] * DP.y + lv.ex[arm] * DC.y);
-        const double2 G = cmul(e, Aarm);
-        const double2 c1 = cscale(w[LINK_C1[L]], LINK_S1[L]);
-        const double2 c2 = cscale(w[LINK_C2[L]], LINK_S2[L]);
-        Q1 = cadd(Q1, cmul(c1, G));
-        Q2 = cadd(Q2, cmul(c2, G));
-        // Omega = F2 (c1 + c2 F1): coefficient of G_L in sum_ch w_ch X_ch
-        const double2 Om = cmul(lv.F2, cadd(c1, cmul(c2, lv.F1)));
-        const double2 Oe = cmul(Om, e);
-        SP = cadd(SP, cscale(Oe, lv.ep[arm]));
-        SC = cadd(SC, cscale(Oe, lv.ex[arm]));
-        const double2 OA = cmul(Om, Aarm);
-        const double su = cmul(OA, ed).x;   // dJ/du_L
-        const double sphi = cmul(OA, e).y;  // dJ/dPhi_s (de/dPhi = -i e)
-        U[sc] += su * (1.0 + sg * lv.kr[arm]);
-        V[sc] += sphi;
-        skr[0] += su * lv.fonfs[sc] * sg * lv.krd[0][arm];
-        skr[1] += su * lv.fonfs[sc] * sg * lv.krd[1][arm];
-        sep[arm] += Oe.x * DP.x - Oe.y * DP.y;  // Re(Oe DP)
-        sex[arm] += Oe.x * DC.x - Oe.y * DC.y;
-    }
-    // dJ/d omL = Re[dF2 (Q1 + F1 Q2) + F2 Q2 (-i F1)]
-    const double somL = cmul(lv.dF2, cadd(Q1, cmul(lv.F1, Q2))).x + cmul(cmul(lv.F2, Q2), cmul_mi(lv.F1)).x;
-    const double t = lv.t;
-    double g0 = somL * fsi, g1 = somL * t * fsi, g2 = somL * 0.5 * t * t * fsi;
-    double gl = skr[0], gs = skr[1];
-#pragma unroll
-    for (int sc = 0; sc < 3; sc++) {
-        const double xi = lv.xi[sc];
-        const double Uf = U[sc] * fsi;
-        g0 += Uf - TWO_PI * xi * V[sc];
-        g1 += xi * (Uf - PI * xi * V[sc]);
-        g2 += 0.5 * xi * xi * (Uf - (2.0 * PI / 3.0) * xi * V[sc]);
-        // d/d(k.P_s/c): fonfs through xi, Phi = om kP - argS(t - kP) -> 2 pi f(xi)
-        const double W = -(s.fdot + s.fddot * xi) * Uf + TWO_PI * lv.fi[sc] * V[sc];
-        gl += W * lv.kPd[0][sc];
-        gs += W * lv.kPd[1][sc];
-    }
+    // arms r12, r13, r23: k.r and antenna factors with their lambda / sin beta derivatives
+    double kr[3], krl[3], krs[3], ep[3], ex[3];
 #pragma unroll
     for (int a = 0; a < 3; a++) {
-        gl += sep[a] * lv.epd[0][a] + sex[a] * lv.exd[0][a];
-        gs += sep[a] * lv.epd[1][a] + sex[a] * lv.exd[1][a];
+        double r[3];
+#pragma unroll
+        for (int i = 0; i < 3; i++) r[i] = tab[(ROW_R + 3 * a + i) * N + n];
+        kr[a] = dot3(s.k, r);
+        krl[a] = dot3(sd.kl, r);
+        krs[a] = dot3(sd.ks, r);
+        const double ur = dot3(s.u, r), vr = dot3(s.v, r);
+        ep[a] = vr * vr - ur * ur;
+        ex[a] = 2.0 * ur * vr;
+    }
+    double2 Q1 = make_double2(0.0, 0.0), Q2 = Q1, SP = Q1, SC = Q1;
+    double sep[3] = {0.0, 0.0, 0.0}, sex[3] = {0.0, 0.0, 0.0};
+    double g0 = 0.0, g1 = 0.0, g2 = 0.0, gl = 0.0, gs = 0.0;
+    double Pc[3];
+#pragma unroll
+    for (int i = 0; i < 3; i++) Pc[i] = tab[(ROW_PC + i) * N + n];
+    const double kPc = dot3(s.k, Pc), kPcl = dot3(sd.kl, Pc), kPcs = dot3(sd.ks, Pc);
+#pragma unroll
+    for (int sc = 0; sc < 3; sc++) {
+        double dP[3];
+#pragma unroll
+        for (int i = 0; i < 3; i++) dP[i] = tab[(ROW_DP + 3 * sc + i) * N + n];
+        const double kP = (kPc + dot3(s.k, dP)) * INV_C;
+        const double kPl = (kPcl + dot3(sd.kl, dP)) * INV_C;
+        const double kPs = (kPcs + dot3(sd.ks, dP)) * INV_C;
+        const double xi = t - kP;
+        const double fi = s.f0 + s.fdot * xi + 0.5 * s.fddot * xi * xi;
+        const double fonfs = fi * fsi;
+        double2 E;
+        {
+            const double argS =
+                s.phi0 + s.omq * t + PI * s.fdot * (xi * xi) + (1.0 / 3.0) * PI * s.fddot * (xi * xi * xi);
+            double sp, cp;
+            sincos_f64(s.om * kP - argS, sp, cp);
+            E = make_double2(cp, -sp);
+        }
+        double U = 0.0, V = 0.0;
+        // the two links that use this spacecraft's phase and frequency
+#pragma unroll
+        for (int h = 0; h < 2; h++) {
+            const int L = h == 0 ? sc : (sc == 0 ? 5 : sc + 2);  // sc0: 12, 13; sc1: 23, 21; sc2: 31, 32
+            const int arm = LINK_ARM[L];
+            const double sg = LINK_SGN[L];
+            double2 T, dT;
+            transfer_and_deriv<TIER>(fonfs * (1.0 + sg * kr[arm]), T, dT);
+            const double2 e = cscale(cmul(T, E), 0.25);
+            const double2 ed = cscale(cmul(dT, E), 0.25);
+            const double2 Aarm = make_double2(ep[arm] * DP.x + ex[arm] * DC.x, ep[arm] * DP.y + ex[arm] * DC.y);
+            const double2 G = cmul(e, Aarm);
+            const double2 c1 = cscale(w[LINK_C1[L]], LINK_S1[L]);
+            const double2 c2 = cscale(w[LINK_C2[L]], LINK_S2[L]);
+            Q1 = cadd(Q1, cmul(c1, G));
+            Q2 = cadd(Q2, cmul(c2, G));
+            // Omega = F2 (c1 + c2 F1): coefficient of G_L in sum_ch w_ch X_ch
+            const double2 Om = cmul(F2, cadd(c1, cmul(c2, F1)));
+            const double2 Oe = cmul(Om, e);
+            SP = cadd(SP, cscale(Oe, ep[arm]));
+            SC = cadd(SC, cscale(Oe, ex[arm]));
+            const double2 OA = cmul(Om, Aarm);
+            const double su = cmul(OA, ed).x;  // dJ/du_L
+            U += su * (1.0 + sg * kr[arm]);
+            V += cmul(OA, e).y;  // dJ/dPhi_s (de/dPhi = -i e)
+            gl += su * fonfs * sg * krl[arm];
+            gs += su * fonfs * sg * krs[arm];
+            sep[arm] += Oe.x * DP.x - Oe.y * DP.y;  // Re(Oe DP)
+            sex[arm] += Oe.x * DC.x - Oe.y * DC.y;
+        }
+        const double Uf = U * fsi;
+        g0 += Uf - TWO_PI * xi * V;
+        g1 += xi * (Uf - PI * xi * V);
+        g2 += 0.5 * xi * xi * (Uf - (2.0 * PI / 3.0) * xi * V);
+        // d/d(k.P_s/c): fonfs through xi, Phi = om kP - argS(t - kP) -> 2 pi f(xi)
+        const double W = -(s.fdot + s.fddot * xi) * Uf + TWO_PI * fi * V;
+        gl += W * kPl;
+        gs += W * kPs;
+    }
+    // dJ/d omL = Re[dF2 (Q1 + F1 Q2) + F2 Q2 (-i F1)]
+    const double somL = cmul(dF2, cadd(Q1, cmul(F1, Q2))).x + cmul(cmul(F2, Q2), cmul_mi(F1)).x;
+    g0 += somL * fsi;
+    g1 += somL * t * fsi;
+    g2 += somL * 0.5 * t * t * fsi;
+    // antenna factors: recompute their lambda / sin beta derivatives per arm
+#pragma unroll
+    for (int a = 0; a < 3; a++) {
+        double r[3];
+#pragma unroll
+        for (int i = 0; i < 3; i++) r[i] = tab[(ROW_R + 3 * a + i) * N + n];
+        const double ur = dot3(s.u, r), vr = dot3(s.v, r);
+        const double url = dot3(sd.ul, r), urs = dot3(sd.us, r), vrl = dot3(sd.vl, r);
+        gl += sep[a] * 2.0 * (vr * vrl - ur * url) + sex[a] * 2.0 * (url * vr + ur * vrl);
+        gs += sep[a] * (-2.0 * ur * urs) + sex[a] * 2.0 * urs * vr;  // dv/ds = 0
     }
     acc.SP = cadd(acc.SP, SP);
     acc.SC = cadd(acc.SC, SC);
