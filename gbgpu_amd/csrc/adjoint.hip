@@ -501,6 +501,10 @@ extern "C" int gbgpu_fisher(double *fisher, double *grad, double *loglike, const
         set_error("no output requested");
         return GBGPU_E_ARG;
     }
+    if (cfg->N > 1024) {
+        set_error("Fisher matrices support N in [32, 1024] (LDS limit)");
+        return GBGPU_E_ARG;
+    }
     if (num_bin == 0) return GBGPU_OK;
     WaveArgs a = base_args(params, num_params, num_bin, cfg);
     set_data(a, data);

@@ -53,6 +53,39 @@ def test_invalid_arguments_return_errors_without_gpu_work():
     assert lib.gbgpu_get_ll(None, None, None, None, None, None, None, None, 1.0, None, 4, 0, None, None, 10, None) == 0
 
 
+def test_empty_batches_are_noops_on_every_entry_point():
+    """num_bin = 0 returns OK after argument checks and launches nothing
+    (dummy non-null pointers are never dereferenced)."""
+    import ctypes
+
+    from gbgpu_amd import _capi
+
+    lib = _capi.load()
+    P = 8  # any non-null address
+    cfg = _capi.WaveCfg(N=256, T=1e7, armlength=2.5e9, orbit_table=P, tdi2=0, precision=0)
+    dat = _capi.Data(A=P, E=P, psdA=P, psdE=P, data_index=None, noise_index=None, data_length=10,
+                     start_freq_ind=0, df=1e-7)
+    assert lib.gbgpu_run_wave(None, P, P, P, 9, 0, cfg, None) == 0
+    assert lib.gbgpu_wave_ll(P, None, None, P, 9, 0, cfg, dat, 0.0, 0, None) == 0
+    assert lib.gbgpu_swap_ll_diff(P, None, P, 9, 0, cfg, dat, None) == 0
+    assert lib.gbgpu_wave_ll_grad(P, P, None, None, P, 9, 0, cfg, dat, 0.0, 0, None) == 0
+    assert lib.gbgpu_fisher(P, None, None, P, 9, 0, cfg, dat, 0.0, None) == 0
+    assert lib.gbgpu_fill_global(P, P, P, P, P, 4, 0, P, 10, 1, None) == 0
+    bcfg = _capi.BatchCfg(T=1e7, armlength=2.5e9, tdi2=0, precision=1)
+    assert lib.gbgpu_wave_ll_batched(P, None, None, P, 9, 0, P, bcfg, dat, 0.0, 0, None, None) == 0
+    assert lib.gbgpu_perturb_params(P, P, 9, 0, 1e-3, 1, 0, None, 0, None, None, None) == 0
+    # derivatives reject what they do not support, before any GPU work
+    cfg.precision = 1
+    assert lib.gbgpu_wave_ll_grad(P, P, None, None, P, 9, 4, cfg, dat, 0.0, 0, None) == -1
+    assert b"FP64" in lib.gbgpu_last_error()
+    cfg.precision = 0
+    assert lib.gbgpu_wave_ll_grad(P, P, None, None, P, 14, 4, cfg, dat, 0.0, 0, None) == -1
+    cfg.N = 2048
+    assert lib.gbgpu_fisher(P, None, None, P, 9, 4, cfg, dat, 0.0, None) == -1
+    assert b"1024" in lib.gbgpu_last_error()
+    del ctypes
+
+
 def test_missing_library_fails_loudly(tmp_path):
     from gbgpu_amd import _capi
 

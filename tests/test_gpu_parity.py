@@ -132,6 +132,45 @@ def test_get_ll_fused_fp64_matches_oracle(N):
     assert rel_err(gb.A.cpu().numpy(), o["AETf"][:, 0], axis=1).max() <= 1e-9
 
 
+def test_windows_straddling_the_data_edges():
+    """Data restricted to bins [k0, k0 + L) (start_freq_ind = k0): templates
+    cut by either end, and one whose start bin is negative, contribute only
+    their in-range bins to the fused likelihood, and are clipped by the fill."""
+    import torch
+
+    rng = np.random.default_rng(222)
+    N, T = 256, YR
+    Tadj = F.adjust_T(T, DT)
+    B = 12
+    p = draw_params(B, rng)
+    k0, L = 60_000, 40_000
+    # starts near both ends of the window (and beyond them), one at ~0 Hz
+    edge_bins = np.array([k0 - 200, k0 - 10, k0 + 5, k0 + L - 250, k0 + L - 100, k0 + L + 50,
+                          k0 + 1000, k0 + 20_000, k0 - 128, k0 + L - 128, 3, 90])
+    p[1] = (edge_bins + N // 2 + 0.25) / Tadj
+    p[2] = p[3] = 0.0
+    A_full, E_full, SA_full, SE_full, df, Lf = make_data(p, N, T, DT, rng)
+    A, E, SA, SE = (x[k0:k0 + L].copy() for x in (A_full, E_full, SA_full, SE_full))
+    gb = _gb()
+    gb.d_d = 1.0
+    ll = gb.get_ll(p, [A, E], [SA, SE], N=N, T=T, dt=DT, start_freq_ind=k0)
+    o = F.run_wave(*p, N=N, T=T, dt=DT)
+    d_h, h_h = F.inner_terms(o["AETf"][:, 0], o["AETf"][:, 1], o["start_inds"] - k0, A, E, SA, SE, df)
+    ll_o = F.loglike(1.0, d_h, h_h)
+    assert np.all(np.abs(ll - ll_o) <= 1e-8 * np.maximum(1.0, np.abs(ll_o))), np.abs(ll - ll_o)
+    # fill into the restricted window: only in-range bins land.  The HIP
+    # waveforms carry the FP64 waveform tolerance against the oracle's, hence 1e-9.
+    tm = torch.zeros((1, 2, L), dtype=torch.complex128, device="cuda")
+    gb.generate_global_template(p.T, np.zeros(B, np.int32), tm, N=N, T=T, dt=DT, start_freq_ind=k0)
+    ref = F.fill_global(1, L, o["AETf"][:, 0], o["AETf"][:, 1], o["start_inds"] - k0, np.zeros(B, int))
+    got = tm.cpu().numpy()
+    assert np.abs(got - ref).max() <= 1e-9 * np.abs(ref).max()
+    # and exactly the clipped placement of the HIP templates themselves
+    ref_hip = F.fill_global(1, L, gb.AETf[:, 0].cpu().numpy(), gb.AETf[:, 1].cpu().numpy(),
+                            gb.start_inds.cpu().numpy() - k0, np.zeros(B, int))
+    assert np.abs(got - ref_hip).max() <= 1e-15 * np.abs(ref_hip).max()
+
+
 def test_get_ll_multi_stream_indices():
     import torch
 
