@@ -23,6 +23,8 @@
 // replayed (the per-N orbit tables and twiddles must have been created by an
 // earlier, uncaptured call).  Which workgroup runs a binary depends on
 // scheduling; each binary's result does not, so results are deterministic.
+#include <atomic>
+
 #include "common.hpp"
 #include "wave_common.hpp"
 
@@ -296,22 +298,18 @@ size_t batch_ws_bytes(long long B) {
 template <class P>
 static int launch_persistent(BatchArgs &ba, hipStream_t st) {
     // persistent grid: as many workgroups as can be resident at once
-    static int blocks_per_cu[2] = {0, 0};
-    static int num_cu = 0;
-    const int pi = sizeof(typename P::CT) == 16 ? 0 : 1;
+    // (initialised once per policy, thread-safe)
     const size_t lds = batch_lds_bytes<P>();
-    if (!blocks_per_cu[pi]) {
-        int dev = 0;
+    static const long long grid = [lds] {
+        int dev = 0, num_cu = 0, nb = 0;
         (void)hipGetDevice(&dev);
         (void)hipDeviceGetAttribute(&num_cu, hipDeviceAttributeMultiprocessorCount, dev);
         (void)hipFuncSetAttribute(BatchKernel<P>::fn(), hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
-        int nb = 0;
         if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, BatchKernel<P>::fn(), WG_SIZE, lds) != hipSuccess ||
             nb < 1)
             nb = 1;
-        blocks_per_cu[pi] = nb;
-    }
-    const long long grid = (long long)blocks_per_cu[pi] * num_cu;
+        return (long long)nb * (num_cu > 0 ? num_cu : 256);
+    }();
     void *kargs[] = {&ba};
     hipError_t e;
     {
@@ -325,7 +323,7 @@ static int launch_persistent(BatchArgs &ba, hipStream_t st) {
 
 // Tile shape of the per-bucket kernel, per (precision, bucket): 0 = Geo<N>,
 // 1 = Geo<N, 2>; set by gbgpu_batch_autotune.
-static int g_variant[2][NBUCKET] = {};
+static std::atomic<int> g_variant[2][NBUCKET] = {};
 
 // One launch of the uniform fused kernel per populated bucket.  With the
 // accuracy guard (mixed precision, guard_tol > 0), each mixed launch lists
@@ -342,7 +340,7 @@ static int launch_buckets(const BatchArgs &ba, int precision, hipStream_t st, co
     for (int k = NBUCKET - 1; k >= 0; k--) {
         if (!ba.tabs[k]) continue;
         const int N = 32 << k;
-        const int var = variants ? variants[k] : g_variant[precision == GBGPU_MIXED ? 1 : 0][k];
+        const int var = variants ? variants[k] : g_variant[precision == GBGPU_MIXED ? 1 : 0][k].load();
         if (bucket_events) (void)hipEventRecord(bucket_events[2 * k], st);
         WaveArgs a = ba.a;
         a.tab = ba.tabs[k];
@@ -571,7 +569,7 @@ extern "C" int gbgpu_batch_autotune(double *loglike, const double *params, int n
         int pick = -1;
         if (ba.tabs[k]) {
             pick = best[1][k] < best[0][k] ? 1 : 0;
-            g_variant[pi][k] = pick;
+            g_variant[pi][k].store(pick);
         }
         if (chosen) chosen[k] = pick;
     }
@@ -588,7 +586,7 @@ extern "C" int gbgpu_batch_set_variants(int precision, const int *variants) {
             set_error("gbgpu_batch_set_variants: variants are 0 or 1");
             return GBGPU_E_ARG;
         }
-    for (int k = 0; k < NBUCKET; k++) g_variant[precision == GBGPU_MIXED ? 1 : 0][k] = variants[k];
+    for (int k = 0; k < NBUCKET; k++) g_variant[precision == GBGPU_MIXED ? 1 : 0][k].store(variants[k]);
     return GBGPU_OK;
 }
 

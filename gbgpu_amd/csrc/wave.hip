@@ -205,13 +205,13 @@ static int launch_wave(int N, const WaveArgs &a, hipStream_t st) {
 // device, so the grid covers a few passes of resident workgroups and strides.
 template <class P, class PK, int NN, int M>
 static int launch_segment_n(const WaveArgs &a, hipStream_t st) {
-    static int grid = 0;
-    if (!grid) {
+    // several passes' worth of resident workgroups (initialised once, thread-safe)
+    static const int grid = [] {
         int dev = 0, ncu = 0;
         (void)hipGetDevice(&dev);
         (void)hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev);
-        grid = 16 * (ncu > 0 ? ncu : 256);  // several passes' worth of resident workgroups
-    }
+        return 16 * (ncu > 0 ? ncu : 256);
+    }();
     long long g = (a.B + Geo<NN, M>::BPW - 1) / Geo<NN, M>::BPW;
     if (g > grid) g = grid;
     KernelTimer kt(st);
