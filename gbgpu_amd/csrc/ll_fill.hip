@@ -144,8 +144,13 @@ __global__ void __launch_bounds__(64 * GATHER_WAVES) k_gather_sum(
     const unsigned long long KS = (unsigned long long)(L + M);
     const unsigned long long base = (unsigned long long)g * KS;
     // binaries with start in [j0 - M + 1, j0 + 63], i.e. key in [base + j0 + 1, base + j0 + 63 + M]
-    const long long lo = lower_bound_u64(keys, B, base + (unsigned long long)j0 + 1ULL);
-    const long long hi = lower_bound_u64(keys, B, base + (unsigned long long)(j0 + 64 + M));
+    const unsigned long long key_lo = base + (unsigned long long)j0 + 1ULL;
+    const unsigned long long key_hi = base + (unsigned long long)(j0 + 64 + M);
+    // tiles outside the sorted key range (most of them when the binaries
+    // cover a narrow band) leave before the binary searches
+    if (keys[0] >= key_hi || keys[B - 1] < key_lo) return;  // uniform over the workgroup
+    const long long lo = lower_bound_u64(keys, B, key_lo);
+    const long long hi = lower_bound_u64(keys, B, key_hi);
     if (hi <= lo) return;  // uniform over the workgroup
     const long long per = (hi - lo + GATHER_WAVES - 1) / GATHER_WAVES;
     const long long r0 = lo + w * per;
