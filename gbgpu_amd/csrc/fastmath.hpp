@@ -179,11 +179,14 @@ __device__ __forceinline__ void transfer_series(R u, R &re, R &im) {
 // (<= U_MAX/2) and the mixed path's spacecraft phase offsets
 // (<= 0.29 U_MAX rad).  Tier 3 falls back to range-reduced sine/cosine.
 constexpr double TIER_UMAX[3] = {0.75, 1.5, 2.25};
+// FP64 phase offsets (|x| <= 0.29 U_MAX: 0.22, 0.44, 0.65): the first
+// omitted sine term x^(2K+1)/(2K+1)! is 3.7e-19, 2.7e-18, 1.8e-18.
 template <int TIER>
 struct TierF64 {
     static constexpr int KR = TIER == 0 ? 8 : TIER == 1 ? 10 : 12;
     static constexpr int KI = TIER == 0 ? 8 : TIER == 1 ? 10 : 12;
     static constexpr int KS = TIER == 0 ? 7 : TIER == 1 ? 9 : 10;  // sincos of omL
+    static constexpr int KD = TIER == 0 ? 6 : TIER == 1 ? 7 : 8;   // sincos of phase offsets
 };
 template <int TIER>
 struct TierF32 {

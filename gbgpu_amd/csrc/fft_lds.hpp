@@ -45,16 +45,21 @@ __device__ __forceinline__ void fft_lds(CT *__restrict__ buf, int lane, const CT
         for (int rr = 0; rr < R; rr++) {
             const int j = lane + rr * TPB;
             const int k = j & (Ns - 1);
-            const CT w1 = tw[k * step];
-            const CT w2 = tw[2 * k * step];
-            const CT w3 = tw[3 * k * step];
+            CT w1{}, w2{}, w3{};
+            if (Ns > 1) {
+                w1 = tw[k * step];
+                w2 = tw[2 * k * step];
+                w3 = tw[3 * k * step];
+            }
             const int base = ((j - k) << 2) + k;
 #pragma unroll
             for (int c = 0; c < NCH; c++) {
-                CT x0 = v[rr][c][0];
-                CT x1 = cmul(v[rr][c][1], w1);
-                CT x2 = cmul(v[rr][c][2], w2);
-                CT x3 = cmul(v[rr][c][3], w3);
+                CT x0 = v[rr][c][0], x1 = v[rr][c][1], x2 = v[rr][c][2], x3 = v[rr][c][3];
+                if (Ns > 1) {  // the first stage's twiddles are all w^0 = 1
+                    x1 = cmul(x1, w1);
+                    x2 = cmul(x2, w2);
+                    x3 = cmul(x3, w3);
+                }
                 bfly4(x0, x1, x2, x3);
                 buf[c * N + base] = x0;
                 buf[c * N + base + Ns] = x1;

@@ -167,6 +167,8 @@ __device__ inline double third_vlos(const Src &s, double t) {
 struct SampleGeom {
     double t;
     double kP[3];   // k . P_s / c
+    double kPc;     // k . Pc / c (guiding centre)
+    double d[3];    // k . dP_s / c (offsets from the guiding centre)
     double kr12, kr13, kr23;
     double ur12, vr12, ur13, vr13, ur23, vr23;
 };
@@ -194,7 +196,20 @@ __device__ __forceinline__ SampleGeom load_geom(const Src &s, const double *__re
     double r[9];
 #pragma unroll
     for (int i = 0; i < 9; i++) r[i] = tab[(ROW_R + i) * N + n];
-    load_kP(s, tab, N, n, g.kP);
+    {
+        constexpr double INV_C = 1.0 / C_SI;
+        const double kc =
+            s.k[0] * tab[ROW_PC * N + n] + s.k[1] * tab[(ROW_PC + 1) * N + n] + s.k[2] * tab[(ROW_PC + 2) * N + n];
+        g.kPc = kc * INV_C;
+#pragma unroll
+        for (int sc = 0; sc < 3; sc++) {
+            const int r0 = ROW_DP + 3 * sc;
+            const double kd =
+                s.k[0] * tab[r0 * N + n] + s.k[1] * tab[(r0 + 1) * N + n] + s.k[2] * tab[(r0 + 2) * N + n];
+            g.kP[sc] = (kc + kd) * INV_C;
+            g.d[sc] = kd * INV_C;
+        }
+    }
     g.kr12 = s.k[0] * r[0] + s.k[1] * r[1] + s.k[2] * r[2];
     g.kr13 = s.k[0] * r[3] + s.k[1] * r[4] + s.k[2] * r[5];
     g.kr23 = s.k[0] * r[6] + s.k[1] * r[7] + s.k[2] * r[8];
@@ -207,8 +222,9 @@ __device__ __forceinline__ SampleGeom load_geom(const Src &s, const double *__re
     return g;
 }
 
-// One transfer-function link: 0.25 sin(a)/a e^{-i(a + Phi_s)} A_arm
-// (gbgpu.py:508-525), with e^{-i Phi_s} = (cphi, -sphi) precomputed.
+// One transfer-function link: sin(a)/a e^{-i(a + Phi_s)} A_arm, i.e. 4 G_ij of
+// gbgpu.py:508-525 (the 1/4 is applied with the TDI factor, an exact power
+// of two), with e^{-i Phi_s} = (cphi, -sphi) precomputed.
 // sinc(a) e^{-ia} comes from the reduction-free series in u = 2a when the
 // binary's tier allows, else from a reduced sine/cosine.
 template <int TIER>
@@ -224,8 +240,8 @@ __device__ __forceinline__ double2 link_f64(double fonfs, double kr, double cphi
         tr = sinc * ca;
         ti = -sinc * sa;
     }
-    // 0.25 (tr + i ti)(cphi - i sphi)
-    const double2 e = make_double2(0.25 * (tr * cphi + ti * sphi), 0.25 * (ti * cphi - tr * sphi));
+    // (tr + i ti)(cphi - i sphi)
+    const double2 e = make_double2(tr * cphi + ti * sphi, ti * cphi - tr * sphi);
     return cmul(e, Aarm);
 }
 
@@ -238,16 +254,43 @@ __device__ __forceinline__ void slow_xyz_f64(const Src &s, const SampleGeom &g, 
                                              const double *v3, const double *term3, double2 &X, double2 &Y,
                                              double2 &Z) {
     double fonfs[3], cphi[3], sphi[3];
+    if constexpr (!THIRD && TIER < 3) {
+        // Guiding-centre split: with xi_c = t - k.Pc/c and d_s = k.dP_s/c,
+        //   Phi_s = Phi_c + 2 pi f(xi_c) d_s - pi f'(xi_c) d_s^2 + (pi/3) fddot d_s^3,
+        // exact for the cubic argS.  Phi_c takes the one range-reduced
+        // sine/cosine; the offsets (|.| <= 0.29 U_MAX of the tier) take short
+        // series and a rotation.
+        const double xic = g.t - g.kPc;
+        const double fc = s.f0 + s.fdot * xic + 0.5 * s.fddot * xic * xic;
+        const double fdc = s.fdot + s.fddot * xic;
+        const double argSc =
+            s.phi0 + s.omq * g.t + PI * s.fdot * (xic * xic) + (1.0 / 3.0) * PI * s.fddot * (xic * xic * xic);
+        double sC, cC;
+        sincos_f64(s.om * g.kPc - argSc, sC, cC);
+        const double w1 = 2.0 * PI * fc, w2 = PI * fdc, w3 = (PI / 3.0) * s.fddot;
 #pragma unroll
-    for (int sc = 0; sc < 3; sc++) {
-        const double xi = g.t - g.kP[sc];
-        double fi = s.f0 + s.fdot * xi + 0.5 * s.fddot * xi * xi;
-        if (THIRD) fi *= 1.0 + v3[sc] / C_SI;
-        fonfs[sc] = fi * fstar_inv;
-        double argS = s.phi0 + s.omq * g.t + PI * s.fdot * (xi * xi) + (1.0 / 3.0) * PI * s.fddot * (xi * xi * xi);
-        if (THIRD) argS += term3[sc];
-        const double Phi = s.om * g.kP[sc] - argS;
-        sincos_f64(Phi, sphi[sc], cphi[sc]);
+        for (int sc = 0; sc < 3; sc++) {
+            const double dd = g.d[sc];
+            const double dphi = dd * (w1 - dd * (w2 - dd * w3));
+            double sd, cd;
+            sincos_series<TierF64<TIER>::KD>(dphi, sd, cd);
+            sphi[sc] = sC * cd + cC * sd;
+            cphi[sc] = cC * cd - sC * sd;
+            // f(xi_c - d_s), exact for the quadratic f
+            fonfs[sc] = (fc - dd * (fdc - 0.5 * s.fddot * dd)) * fstar_inv;
+        }
+    } else {
+#pragma unroll
+        for (int sc = 0; sc < 3; sc++) {
+            const double xi = g.t - g.kP[sc];
+            double fi = s.f0 + s.fdot * xi + 0.5 * s.fddot * xi * xi;
+            if (THIRD) fi *= 1.0 + v3[sc] / C_SI;
+            fonfs[sc] = fi * fstar_inv;
+            double argS = s.phi0 + s.omq * g.t + PI * s.fdot * (xi * xi) + (1.0 / 3.0) * PI * s.fddot * (xi * xi * xi);
+            if (THIRD) argS += term3[sc];
+            const double Phi = s.om * g.kP[sc] - argS;
+            sincos_f64(Phi, sphi[sc], cphi[sc]);
+        }
     }
     // arm antenna factors A_ij = (r e+ r) DP + (r ex r) DC  (gbgpu.py:467-473)
     // with r e+ r = (v.r)^2 - (u.r)^2 and r ex r = 2 (u.r)(v.r); r31 = -r13.
@@ -273,7 +316,7 @@ __device__ __forceinline__ void slow_xyz_f64(const Src &s, const SampleGeom &g, 
     else
         sincos_f64(omL, sL, cL);
     const double2 fctr = make_double2(cL, -sL);
-    const double m = 4.0 * omL * sL * scale;
+    const double m = omL * sL * scale;  // 4 omL sL scale, times the links' 1/4
     const double2 fctr2 = make_double2(m * cL, -m * sL);
     X = cmul(fctr2, cadd(csub(G21, G31), cmul(csub(G12, G13), fctr)));
     Y = cmul(fctr2, cadd(csub(G32, G12), cmul(csub(G23, G21), fctr)));

@@ -70,7 +70,7 @@ __device__ __forceinline__ void wave_pass(const WaveArgs &a, WaveLds<N, MODE, TH
     float *s_v = L.s_v;
     double *s_g = L.s_g;
     double *s_xi = L.s_xi;
-    const int slot = tid / TPB;
+    const int slot = per_group<TPB>(tid / TPB);
     const int lane = tid % TPB;
     const long long item = blk * BPW + slot;
     const bool valid = item < count;
@@ -78,13 +78,13 @@ __device__ __forceinline__ void wave_pass(const WaveArgs &a, WaveLds<N, MODE, TH
 
     if (valid) load_pack<PK, TPB>(a, b, lane, spack + slot);
     __syncthreads();
-    const Src s = spack[slot].s;
+    const Src s = per_group<TPB>(spack[slot].s);
     CT *buf = sbuf + slot * NCH * N;
     double *tot = stot + slot * WPB;
     ThirdScratch ts{XI_IN_BUF ? reinterpret_cast<double *>(buf) : s_xi + slot * 3 * NXI, s_v + slot * 3 * NT,
                     s_g + slot * 3 * NT};
 
-    binary_spectrum<N, NCH, THIRD, P, M>(s, aux_of<P, PK>(spack[slot]), valid, lane, tab, a.fstar_inv, a.scale,
+    binary_spectrum<N, NCH, THIRD, P, M>(s, per_group<TPB>(aux_of<P, PK>(spack[slot])), valid, lane, tab, a.fstar_inv, a.scale,
                                          buf, static_cast<const CT *>(twp), ts, tot);
 
     const double2 fac = P::fac(s);

@@ -97,6 +97,77 @@ __global__ void __launch_bounds__(256) k_setup(WaveArgs a, int N, Pack<P> *__res
     }
 }
 
+// Values shared by a whole wavefront, moved into scalar registers: once a
+// wavefront owns one binary (TPB >= 64) its per-binary constants are
+// wave-uniform, and keeping them in SGPRs frees the vector registers they
+// would otherwise hold for the whole sample loop.
+__device__ __forceinline__ int to_sgpr(int x) { return __builtin_amdgcn_readfirstlane(x); }
+__device__ __forceinline__ float to_sgpr(float x) { return __int_as_float(to_sgpr(__float_as_int(x))); }
+__device__ __forceinline__ long long to_sgpr(long long x) {
+    const unsigned lo = (unsigned)to_sgpr((int)(x & 0xffffffffLL));
+    const unsigned hi = (unsigned)to_sgpr((int)(x >> 32));
+    return (long long)(((unsigned long long)hi << 32) | lo);
+}
+__device__ __forceinline__ double to_sgpr(double x) { return __longlong_as_double(to_sgpr(__double_as_longlong(x))); }
+
+__device__ __forceinline__ Src to_sgpr(const Src &v) {
+    Src s;
+    s.amp = to_sgpr(v.amp);
+    s.f0 = to_sgpr(v.f0);
+    s.fdot = to_sgpr(v.fdot);
+    s.fddot = to_sgpr(v.fddot);
+    s.phi0 = to_sgpr(v.phi0);
+    s.om = to_sgpr(v.om);
+    s.omq = to_sgpr(v.omq);
+#pragma unroll
+    for (int i = 0; i < 3; i++) {
+        s.k[i] = to_sgpr(v.k[i]);
+        s.u[i] = to_sgpr(v.u[i]);
+        s.v[i] = to_sgpr(v.v[i]);
+    }
+    s.DPr = to_sgpr(v.DPr);
+    s.DPi = to_sgpr(v.DPi);
+    s.DCr = to_sgpr(v.DCr);
+    s.DCi = to_sgpr(v.DCi);
+    s.A2 = to_sgpr(v.A2);
+    s.varpi = to_sgpr(v.varpi);
+    s.e2 = to_sgpr(v.e2);
+    s.n2 = to_sgpr(v.n2);
+    s.T2 = to_sgpr(v.T2);
+    s.tdr = to_sgpr(v.tdr);
+    s.tdi = to_sgpr(v.tdi);
+    s.start = to_sgpr(v.start);
+    s.tier = to_sgpr(v.tier);
+    return s;
+}
+
+__device__ __forceinline__ F64::Aux to_sgpr(const F64::Aux &v) { return v; }
+__device__ __forceinline__ SrcF to_sgpr(const SrcF &v) {
+    SrcF f;
+#pragma unroll
+    for (int i = 0; i < 3; i++) {
+        f.k[i] = to_sgpr(v.k[i]);
+        f.u[i] = to_sgpr(v.u[i]);
+        f.v[i] = to_sgpr(v.v[i]);
+    }
+    f.DP = make_float2(to_sgpr(v.DP.x), to_sgpr(v.DP.y));
+    f.DC = make_float2(to_sgpr(v.DC.x), to_sgpr(v.DC.y));
+    f.f0 = to_sgpr(v.f0);
+    f.fdot = to_sgpr(v.fdot);
+    f.fddot = to_sgpr(v.fddot);
+    return f;
+}
+
+// v itself when a group of TPB lanes spans less than a wavefront (several
+// binaries per wavefront), else its scalar-register copy.
+template <int TPB, class T>
+__device__ __forceinline__ T per_group(const T &v) {
+    if constexpr (TPB >= 64)
+        return to_sgpr(v);
+    else
+        return v;
+}
+
 // Cooperative copy of binary b's Pack into LDS by the TPB lanes owning it.
 template <class P, int TPB>
 __device__ __forceinline__ void load_pack(const WaveArgs &a, long long b, int lane, Pack<P> *dst) {
@@ -246,27 +317,32 @@ __device__ __forceinline__ void binary_spectrum(const Src &s, const typename P::
 }
 
 // Accumulates this thread's share of <d|h> (complex) and <h|h> for a binary
-// whose A/E spectra sit in buf (natural order), over output bins
-// m = lane + q TPB (fftshifted), data bin j = j0 + m.
+// whose A/E spectra h = fac x have x in buf (natural order), over output bins
+// m = lane + q TPB (fftshifted), data bin j = j0 + m.  The sums are formed
+// for x and scaled once: <d|h> = fac <d|x>, <h|h> = |fac|^2 <x|x>.
 template <int N, class P, int M = 1>
 __device__ __forceinline__ void ll_partial(const typename P::CT *buf, double2 fac, int lane, long long j0,
                                            long long doff, long long noff, const WaveArgs &a, double &dhr,
                                            double &dhi, double &hh) {
     using G = Geo<N, M>;
+    double xr = 0.0, xi = 0.0, xx = 0.0;
 #pragma unroll
     for (int q = 0; q < G::SPT; q++) {
         const int m = lane + q * G::TPB;
         const int idx = (m + N / 2) & (N - 1);
         const long long j = j0 + m;
         if (j < 0 || j >= a.L) continue;
-        const double2 hA = cmul(P::up(buf[idx]), fac), hE = cmul(P::up(buf[N + idx]), fac);
+        const double2 hA = P::up(buf[idx]), hE = P::up(buf[N + idx]);
         const double2 dA = a.dA[doff + j], dE = a.dE[doff + j];
         const double iA = rcp_f64(a.SA[noff + j]), iE = rcp_f64(a.SE[noff + j]);
-        // conj(d) h / S
-        dhr += (dA.x * hA.x + dA.y * hA.y) * iA + (dE.x * hE.x + dE.y * hE.y) * iE;
-        dhi += (dA.x * hA.y - dA.y * hA.x) * iA + (dE.x * hE.y - dE.y * hE.x) * iE;
-        hh += (hA.x * hA.x + hA.y * hA.y) * iA + (hE.x * hE.x + hE.y * hE.y) * iE;
+        // conj(d) x / S
+        xr += (dA.x * hA.x + dA.y * hA.y) * iA + (dE.x * hE.x + dE.y * hE.y) * iE;
+        xi += (dA.x * hA.y - dA.y * hA.x) * iA + (dE.x * hE.y - dE.y * hE.x) * iE;
+        xx += (hA.x * hA.x + hA.y * hA.y) * iA + (hE.x * hE.x + hE.y * hE.y) * iE;
     }
+    dhr += fac.x * xr - fac.y * xi;
+    dhi += fac.x * xi + fac.y * xr;
+    hh += (fac.x * fac.x + fac.y * fac.y) * xx;
 }
 
 }  // namespace gbgpu

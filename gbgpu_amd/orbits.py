@@ -112,6 +112,14 @@ def orbit_table(orbits: Orbits, T: float, N: int) -> np.ndarray:
     r13 = (P[2] - P[0]) / L
     r23 = (P[2] - P[1]) / L
     Pc = (P[0] + P[1] + P[2]) / 3.0
+    # The kernels' short series for the per-spacecraft phase offsets assume
+    # |P_s - Pc| <= L/sqrt(3) (an equilateral constellation); their error
+    # stays below 1e-16 up to 30% beyond that.
+    dmax = max(float(np.max(np.linalg.norm(p - Pc, axis=1))) for p in P)
+    if dmax > 1.3 * L / np.sqrt(3.0):
+        raise ValueError(
+            f"spacecraft sit up to {dmax:.4g} m from the constellation centre, beyond 1.3 armlength/sqrt(3)"
+        )
     rows = [tm, Pc[:, 0], Pc[:, 1], Pc[:, 2]]
     for p in P:
         d = p - Pc
