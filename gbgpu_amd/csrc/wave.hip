@@ -27,18 +27,23 @@ constexpr bool fits_lds() {
 // LDS of one workgroup of k_wave.  PK is the precision policy whose Pack
 // layout the per-binary constants were written in (normally P; the FP64
 // re-evaluation of guarded mixed binaries reads the mixed packs).
+// Arrays a variant does not use have length zero, so that the transform
+// buffers alone set the footprint (32 KB for the FP64 likelihood at N = 256:
+// five workgroups per CU).
 template <int N, int MODE, bool THIRD, class P, class PK = P, int M = 1>
 struct WaveLds {
     using G = Geo<N, M>;
     using CT = typename P::CT;
     static constexpr int NCH = (MODE == MODE_WAVE) ? 3 : 2;
-    static constexpr int NT = THIRD ? N : 1;
+    static constexpr int NT = THIRD ? N : 0;
     // xi shares the transform buffer when that is large enough
     static constexpr bool XI_IN_BUF = NCH * sizeof(CT) * N >= 3 * sizeof(double) * N;
-    static constexpr int NXI = (THIRD && !XI_IN_BUF) ? N : 1;
+    static constexpr int NXI = (THIRD && !XI_IN_BUF) ? N : 0;
+    // a wavefront that owns its binary reads the Pack straight into scalar registers
+    static constexpr bool PACK_IN_LDS = G::TPB < 64;
     CT sbuf[G::BPW * NCH * N];
-    Pack<PK> spack[G::BPW];
-    double stot[G::BPW * G::WPB];
+    Pack<PK> spack[PACK_IN_LDS ? G::BPW : 0];
+    double stot[G::WPB > 1 ? G::BPW * G::WPB : 0];
     float s_v[G::BPW * 3 * NT];
     double s_g[G::BPW * 3 * NT];
     double s_xi[G::BPW * 3 * NXI];
@@ -76,16 +81,23 @@ __device__ __forceinline__ void wave_pass(const WaveArgs &a, WaveLds<N, MODE, TH
     const bool valid = item < count;
     const long long b = (valid && a.order) ? (long long)a.order[first + item] : item;
 
-    if (valid) load_pack<PK, TPB>(a, b, lane, spack + slot);
-    __syncthreads();
-    const Src s = per_group<TPB>(spack[slot].s);
+    const Pack<PK> *pk;
+    if constexpr (W::PACK_IN_LDS) {
+        if (valid) load_pack<PK, TPB>(a, b, lane, spack + slot);
+        __syncthreads();
+        pk = spack + slot;
+    } else {
+        // wave-uniform index; an empty slot reads binary 0 and discards it
+        pk = static_cast<const Pack<PK> *>(a.pack) + (valid ? b : 0);
+    }
+    const Src s = per_group<TPB>(pk->s);
     CT *buf = sbuf + slot * NCH * N;
     double *tot = stot + slot * WPB;
     ThirdScratch ts{XI_IN_BUF ? reinterpret_cast<double *>(buf) : s_xi + slot * 3 * NXI, s_v + slot * 3 * NT,
                     s_g + slot * 3 * NT};
 
-    binary_spectrum<N, NCH, THIRD, P, M>(s, per_group<TPB>(aux_of<P, PK>(spack[slot])), valid, lane, tab, a.fstar_inv, a.scale,
-                                         buf, static_cast<const CT *>(twp), ts, tot);
+    binary_spectrum<N, NCH, THIRD, P, M>(s, per_group<TPB>(aux_of<P, PK>(*pk)), valid, lane, tab, a.fstar_inv,
+                                         a.scale, buf, static_cast<const CT *>(twp), ts, tot);
 
     const double2 fac = P::fac(s);
     if constexpr (MODE == MODE_WAVE) {
