@@ -321,8 +321,8 @@ static int launch_persistent(BatchArgs &ba, hipStream_t st) {
     return GBGPU_OK;
 }
 
-// Tile shape of the per-bucket kernel, per (precision, bucket): 0 = Geo<N>,
-// 1 = Geo<N, 2>; set by gbgpu_batch_autotune.
+// Tile shape of the per-bucket kernel, per (precision, bucket): 0 = N/4
+// threads per binary, 1 = half as many; set by gbgpu_batch_autotune.
 static std::atomic<int> g_variant[2][NBUCKET] = {};
 
 // One launch of the uniform fused kernel per populated bucket.  With the

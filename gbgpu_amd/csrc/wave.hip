@@ -151,9 +151,10 @@ __device__ __forceinline__ void wave_pass(const WaveArgs &a, WaveLds<N, MODE, TH
 }
 
 template <int N, int MODE, bool THIRD, class P>
-__global__ void __launch_bounds__(WG_SIZE) k_wave(WaveArgs a) {
-    __shared__ WaveLds<N, MODE, THIRD, P> lds;
-    wave_pass<N, MODE, THIRD, P>(a, lds, blockIdx.x, a.B, 0, threadIdx.x, a.tab, a.tw);
+__global__ void __launch_bounds__((Geo<N, wave_tile<N>()>::WG)) k_wave(WaveArgs a) {
+    constexpr int M = wave_tile<N>();
+    __shared__ WaveLds<N, MODE, THIRD, P, P, M> lds;
+    wave_pass<N, MODE, THIRD, P, P, M>(a, lds, blockIdx.x, a.B, 0, threadIdx.x, a.tab, a.tw);
 }
 
 // Fused likelihood over one device-side segment of a batch: item i <
@@ -163,7 +164,7 @@ __global__ void __launch_bounds__(WG_SIZE) k_wave(WaveArgs a) {
 // of the thread index keep the compiler from hoisting all of them out of the
 // loop into registers (which would cost occupancy).
 template <int N, class P, class PK, int M>
-__global__ void __launch_bounds__(WG_SIZE) k_wave_seg(WaveArgs a) {
+__global__ void __launch_bounds__((Geo<N, M>::WG)) k_wave_seg(WaveArgs a) {
     constexpr int BPW = Geo<N, M>::BPW;
     __shared__ WaveLds<N, MODE_LL, false, P, PK, M> lds;
     const long long count = *a.seg_count;
@@ -183,10 +184,11 @@ static int launch_wave(int N, const WaveArgs &a, hipStream_t st) {
 #define GBGPU_CASE(NN)                                                                              \
     case NN: {                                                                                      \
         if constexpr (fits_lds<NN, MODE, THIRD, P>()) {                                             \
-            const long long grid = (a.B + Geo<NN>::BPW - 1) / Geo<NN>::BPW;                         \
+            using GG = Geo<NN, wave_tile<NN>()>;                                                    \
+            const long long grid = (a.B + GG::BPW - 1) / GG::BPW;                                   \
             return with_pack<P>(a, NN, st, [&](const WaveArgs &aa) -> int {                         \
                 KernelTimer kt(st);                                                                 \
-                hipLaunchKernelGGL((k_wave<NN, MODE, THIRD, P>), dim3((unsigned)grid), dim3(WG_SIZE), \
+                hipLaunchKernelGGL((k_wave<NN, MODE, THIRD, P>), dim3((unsigned)grid), dim3(GG::WG),  \
                                    0, st, aa);                                                      \
                 GBGPU_CHECK_LAUNCH("k_wave");                                                       \
                 return GBGPU_OK;                                                                    \
@@ -227,18 +229,22 @@ static int launch_segment_n(const WaveArgs &a, hipStream_t st) {
     long long g = (a.B + Geo<NN, M>::BPW - 1) / Geo<NN, M>::BPW;
     if (g > grid) g = grid;
     KernelTimer kt(st);
-    hipLaunchKernelGGL((k_wave_seg<NN, P, PK, M>), dim3((unsigned)g), dim3(WG_SIZE), 0, st, a);
+    hipLaunchKernelGGL((k_wave_seg<NN, P, PK, M>), dim3((unsigned)g), dim3(Geo<NN, M>::WG), 0, st, a);
     GBGPU_CHECK_LAUNCH("k_wave_seg");
     return GBGPU_OK;
 }
 
+// variant 0: the waveform kernels' tile (wave_tile); variant 1: half the
+// threads per binary of that tile.
 template <class P, class PK, int NN>
 static int launch_segment_m(int variant, const WaveArgs &a, hipStream_t st) {
-    return variant == 1 ? launch_segment_n<P, PK, NN, 2>(a, st) : launch_segment_n<P, PK, NN, 1>(a, st);
+    constexpr int M0 = wave_tile<NN>();
+    constexpr int M1 = M0 == 0 ? 1 : 2;
+    return variant == 1 ? launch_segment_n<P, PK, NN, M1>(a, st) : launch_segment_n<P, PK, NN, M0>(a, st);
 }
 
 // precision: arithmetic of the pass; pack_precision: layout of a.pack;
-// variant: tile shape (0: Geo<N>, 1: Geo<N, 2>).
+// variant: tile shape (0: wave_tile, 1: half the threads per binary).
 int launch_ll_segment(int N, int precision, int pack_precision, int variant, const WaveArgs &a, hipStream_t st) {
     if (precision == GBGPU_MIXED && pack_precision != GBGPU_MIXED) {
         set_error("mixed segments need mixed per-binary constants");

@@ -14,19 +14,28 @@ enum { MODE_WAVE = 0, MODE_LL = 1 };
 
 constexpr int WG_SIZE = 256;
 
-// A group of TPB = min(N/4, 256) / M threads owns one binary; each thread
-// holds SPT = N/TPB samples (4M up to N = 1024, 8M at 2048).  Workgroups are
-// 256 threads holding BPW binaries.  M = 1 is the default tile; M = 2 (half
-// the threads per binary, twice the binaries per workgroup) is the other
-// shape the batch autotuner can pick.
+// A group of TPB threads owns one binary; each thread holds SPT = N/TPB
+// samples.  Tile shapes:
+//   M = 1: TPB = min(N/4, 256) in 256-thread workgroups (SPT = 4, 8 at N = 2048);
+//   M = 2: half the threads per binary, twice the binaries per workgroup;
+//   M = 0 ("wide"): TPB = N/4 (SPT = 4) in workgroups of max(N/4, 256)
+//          threads -- at N = 2048 a 512-thread workgroup per binary, twice
+//          the wavefronts per byte of LDS of M = 1.
+// Workgroups hold BPW binaries.
 template <int N, int M = 1>
 struct Geo {
-    static constexpr int TPB = ((N / 4 < WG_SIZE) ? N / 4 : WG_SIZE) / M;
-    static constexpr int WG = WG_SIZE;
+    static constexpr int TPB = M == 0 ? N / 4 : ((N / 4 < WG_SIZE) ? N / 4 : WG_SIZE) / M;
+    static constexpr int WG = TPB > WG_SIZE ? TPB : WG_SIZE;
     static constexpr int BPW = WG / TPB;
     static constexpr int SPT = N / TPB;
     static constexpr int WPB = (TPB + 63) / 64;  // wavefronts per binary
 };
+
+// Tile shape of the waveform kernels: wide where N/4 exceeds 256 threads.
+template <int N>
+constexpr int wave_tile() {
+    return N / 4 > WG_SIZE ? 0 : 1;
+}
 
 struct WaveArgs {
     const double *params;

@@ -202,8 +202,9 @@ int gbgpu_wave_ll_batched(double *loglike, gbgpu_cmplx *d_h, gbgpu_cmplx *h_h, c
                           void *stream);
 
 /* Autotuner of the per-bucket launches (GBGPU_BATCH_BUCKETS): runs the batch
- * with each tile shape (0: min(N/4, 256) threads per binary; 1: half as many
- * threads per binary, twice the binaries per workgroup) three times, times
+ * with each tile shape (0: N/4 threads per binary, in workgroups of
+ * max(N/4, 256) threads; 1: half as many threads per binary, twice the
+ * binaries per workgroup) three times, times
  * every bucket with HIP events, and keeps the faster shape per (precision,
  * bucket) for all later calls in the process.  chosen (7 ints, N = 32 << k,
  * may be NULL) receives the pick, -1 for buckets without an orbit table.

@@ -524,8 +524,11 @@ def test_perturb_params_matches_numpy_restatement():
 
 
 def test_uneven_batch_strategies_agree_at_scale():
-    """60k binaries: the per-bucket launches run several grid-stride passes;
-    both strategies give bit-identical results."""
+    """60k binaries: the per-bucket launches run several grid-stride passes.
+    The two strategies give bit-identical results where they share the tile
+    (N <= 1024); at N = 2048 the bucket launch gives each binary 512 threads
+    and the persistent kernel 256, so the sums are added in another order:
+    within 1e-12 of max(|logL|, <h|h>)."""
     import torch
 
     from gbgpu_amd.batch import BatchedLogLike
@@ -541,7 +544,12 @@ def test_uneven_batch_strategies_agree_at_scale():
     for strat in ("buckets", "persistent"):
         bl = BatchedLogLike(gb, [A, E], [SA, SE], T, DT, precision="fp64", strategy=strat)
         out[strat] = bl(p, N=N).cpu().numpy()
-    assert np.array_equal(out["buckets"], out["persistent"])
+    shared = N <= 1024
+    assert np.array_equal(out["buckets"][shared], out["persistent"][shared])
+    wide = ~shared
+    assert wide.sum() > 1000
+    scale = np.maximum(np.abs(out["persistent"][wide]), abs(gb.d_d))
+    assert np.all(np.abs(out["buckets"][wide] - out["persistent"][wide]) <= 1e-12 * scale)
     assert np.all(np.isfinite(out["buckets"]))
     # a sample against standalone per-N calls
     sel = np.flatnonzero(N == 128)[:500]
