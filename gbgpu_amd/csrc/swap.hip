@@ -33,39 +33,62 @@ struct SwapGeo {
     static constexpr int WPB = Geo<N>::WPB;
 };
 
+// LDS of one k_swap workgroup; arrays a variant does not use have length zero.
+template <int N, bool THIRD, class P>
+struct SwapLds {
+    using G = SwapGeo<N>;
+    using CT = typename P::CT;
+    static constexpr int NT = THIRD ? N : 0;
+    static constexpr bool XI_IN_BUF = 2 * sizeof(CT) * N >= 3 * sizeof(double) * N;
+    static constexpr int NXI = (THIRD && !XI_IN_BUF) ? N : 0;
+    static constexpr bool PACK_IN_LDS = G::TPB < 64;
+    CT sbuf[G::BPW * 2 * N];
+    Pack<P> spack[PACK_IN_LDS ? G::BPW : 0];
+    double stot[G::WPB > 1 ? G::BPW * G::WPB : 0];
+    float s_v[G::BPW * 3 * NT];
+    double s_g[G::BPW * 3 * NT];
+    double s_xi[G::BPW * 3 * NXI];
+};
+
 template <int N, bool THIRD, class P>
 __global__ void __launch_bounds__(SwapGeo<N>::WG) k_swap(WaveArgs a, double *dll, double *terms) {
     using G = SwapGeo<N>;
     using CT = typename P::CT;
     constexpr int TPB = G::TPB, BPW = G::BPW, WPB = G::WPB;
     constexpr int SPT = Geo<N>::SPT;
-    constexpr int NT = THIRD ? N : 1;
-    constexpr bool XI_IN_BUF = 2 * sizeof(CT) * N >= 3 * sizeof(double) * N;
-    constexpr int NXI = (THIRD && !XI_IN_BUF) ? N : 1;
-    __shared__ CT sbuf[BPW * 2 * N];
-    __shared__ Pack<P> spack[BPW];
-    __shared__ double stot[BPW * WPB];
-    __shared__ double sred[BPW][5];
-    __shared__ float s_v[BPW * 3 * NT];
-    __shared__ double s_g[BPW * 3 * NT];
-    __shared__ double s_xi[BPW * 3 * NXI];
+    __shared__ SwapLds<N, THIRD, P> L;
+    constexpr bool XI_IN_BUF = SwapLds<N, THIRD, P>::XI_IN_BUF;
+    constexpr int NT = SwapLds<N, THIRD, P>::NT, NXI = SwapLds<N, THIRD, P>::NXI;
+    constexpr bool PACK_IN_LDS = SwapLds<N, THIRD, P>::PACK_IN_LDS;
+    CT *sbuf = L.sbuf;
 
     const int tid = threadIdx.x;
-    const int slot = tid / TPB;
+    const int slot = per_group<TPB>(tid / TPB);
     const int lane = tid % TPB;
     const long long b = (long long)blockIdx.x * BPW + slot;  // 2p = remove, 2p+1 = add
     const long long prop = b >> 1;
     const bool valid = b < a.B;
     const bool is_add = slot & 1;
 
-    if (valid) load_pack<P, TPB>(a, b, lane, spack + slot);
-    __syncthreads();
-    const Src s = spack[slot].s;
+    // per-binary constants: scalar loads when a wavefront owns the binary,
+    // else shared through LDS; the partner's (slot | 1) likewise
+    const Pack<P> *packs = static_cast<const Pack<P> *>(a.pack);
+    const Pack<P> *pk, *ppk;
+    if constexpr (PACK_IN_LDS) {
+        if (valid) load_pack<P, TPB>(a, b, lane, L.spack + slot);
+        __syncthreads();
+        pk = L.spack + slot;
+        ppk = L.spack + (slot | 1);
+    } else {
+        pk = packs + (valid ? b : 0);
+        ppk = packs + ((b | 1) < a.B ? (b | 1) : 0);
+    }
+    const Src s = per_group<TPB>(pk->s);
     CT *buf = sbuf + slot * 2 * N;
-    double *tot = stot + slot * WPB;
-    ThirdScratch ts{XI_IN_BUF ? reinterpret_cast<double *>(buf) : s_xi + slot * 3 * NXI, s_v + slot * 3 * NT,
-                    s_g + slot * 3 * NT};
-    binary_spectrum<N, 2, THIRD, P>(s, spack[slot].aux, valid, lane, a.tab, a.fstar_inv, a.scale, buf,
+    double *tot = L.stot + slot * WPB;
+    ThirdScratch ts{XI_IN_BUF ? reinterpret_cast<double *>(buf) : L.s_xi + slot * 3 * NXI, L.s_v + slot * 3 * NT,
+                    L.s_g + slot * 3 * NT};
+    binary_spectrum<N, 2, THIRD, P>(s, per_group<TPB>(pk->aux), valid, lane, a.tab, a.fstar_inv, a.scale, buf,
                                     static_cast<const CT *>(a.tw), ts, tot);
 
     // partial sums: 0 Re<r|h_r>, 1 Re<r|h_a>, 2 <h_r|h_r>, 3 <h_a|h_a>, 4 Re<h_r|h_a>
@@ -76,7 +99,7 @@ __global__ void __launch_bounds__(SwapGeo<N>::WG) k_swap(WaveArgs a, double *dll
         const long long noff = (a.ni ? (long long)a.ni[prop] : 0LL) * a.L;
         const long long j0 = s.start - a.start_freq_ind;
         // the partner (add binary) of a remove slot
-        const Src &sp = spack[slot | 1].s;
+        const Src sp = per_group<TPB>(ppk->s);
         const CT *pbuf = sbuf + (slot | 1) * 2 * N;
         const double2 pfac = P::fac(sp);
         const long long pj0 = sp.start - a.start_freq_ind;
@@ -104,18 +127,23 @@ __global__ void __launch_bounds__(SwapGeo<N>::WG) k_swap(WaveArgs a, double *dll
     rd = group_total<TPB>(rd, lane, tot);
     hh = group_total<TPB>(hh, lane, tot);
     cross = group_total<TPB>(cross, lane, tot);
+    // the pair's sums meet in the first words of each slot's transform
+    // buffer, once every group has finished reading the spectra
+    __syncthreads();
+    double *sred = reinterpret_cast<double *>(buf);
     if (lane == 0) {
-        sred[slot][0] = rd;
-        sred[slot][1] = hh;
-        sred[slot][2] = cross;
+        sred[0] = rd;
+        sred[1] = hh;
+        sred[2] = cross;
     }
     __syncthreads();
     if (valid && !is_add && lane == 0 && b + 1 < a.B) {
-        const double r_hr = a.df4 * sred[slot][0];
-        const double hr_hr = a.df4 * sred[slot][1];
-        const double hr_ha = a.df4 * sred[slot][2];
-        const double r_ha = a.df4 * sred[slot + 1][0];
-        const double ha_ha = a.df4 * sred[slot + 1][1];
+        const double *pred = reinterpret_cast<const double *>(sbuf + (slot + 1) * 2 * N);
+        const double r_hr = a.df4 * sred[0];
+        const double hr_hr = a.df4 * sred[1];
+        const double hr_ha = a.df4 * sred[2];
+        const double r_ha = a.df4 * pred[0];
+        const double ha_ha = a.df4 * pred[1];
         dll[prop] = r_ha - r_hr - 0.5 * (ha_ha + hr_hr - 2.0 * hr_ha);
         if (terms) {
             terms[5 * prop + 0] = r_hr;
