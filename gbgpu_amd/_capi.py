@@ -139,7 +139,8 @@ def load(path: os.PathLike | None = None):
     global _lib
     if _lib is not None and path is None:
         return _lib
-    p = Path(path) if path is not None else LIB_PATH
+    # GBGPU_HIP_LIB selects another build of the same ABI (e.g. for A/B timing)
+    p = Path(path) if path is not None else Path(os.environ.get("GBGPU_HIP_LIB", LIB_PATH))
     if not p.exists():
         raise RuntimeError(
             f"HIP library {p} is missing: build it with `python -m gbgpu_amd.build` "

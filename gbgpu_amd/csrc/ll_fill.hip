@@ -129,7 +129,7 @@ __device__ __forceinline__ long long lower_bound_u64(const unsigned long long *_
 // Every output bin's sum therefore has a fixed order: results are
 // bit-identical from run to run.
 constexpr int GATHER_WAVES = 4;
-constexpr int GATHER_UNROLL = 4;
+constexpr int GATHER_UNROLL = 8;
 
 __global__ void __launch_bounds__(64 * GATHER_WAVES) k_gather_sum(
     double2 *__restrict__ Aglob, double2 *__restrict__ Eglob, const double2 *__restrict__ rowsA,
@@ -153,35 +153,49 @@ __global__ void __launch_bounds__(64 * GATHER_WAVES) k_gather_sum(
     const long long hi = lower_bound_u64(keys, B, key_hi);
     if (hi <= lo) return;  // uniform over the workgroup
     const long long per = (hi - lo + GATHER_WAVES - 1) / GATHER_WAVES;
-    const long long r0 = lo + w * per;
+    // the wavefront's row range is wave-uniform: the sorted keys and the
+    // permutation are read with scalar loads, which do not wait behind the
+    // template rows in flight
+    const long long r0 = lo + (long long)__builtin_amdgcn_readfirstlane(w) * per;
     const long long r1 = r0 + per < hi ? r0 + per : hi;
     const long long j = j0 + lane;
     double ar = 0.0, ai = 0.0, er = 0.0, ei = 0.0;
+    // row metadata of one group of GATHER_UNROLL rows: first template bin
+    // relative to the tile origin, and the template row; rows past r1 get an
+    // empty window
+    long long st[GATHER_UNROLL];
+    long long rowoff[GATHER_UNROLL];
+    auto meta = [&](long long r) {
+#pragma unroll
+        for (int u = 0; u < GATHER_UNROLL; u++) {
+            const bool in = r + u < r1;
+            st[u] = in ? (long long)(keys[r + u] - base) - M : j0 + 64;
+            rowoff[u] = in ? (long long)perm[r + u] * row_stride : 0;
+        }
+    };
+    if (r0 < r1) meta(r0);
     for (long long r = r0; r < r1; r += GATHER_UNROLL) {
         double2 va[GATHER_UNROLL], ve[GATHER_UNROLL];
         bool ok[GATHER_UNROLL];
+        // branch-free loads (a divergent branch around each load makes the
+        // compiler drain the loads in flight): lanes outside the row's window
+        // read its clamped end element, one cache line, and add nothing
 #pragma unroll
         for (int u = 0; u < GATHER_UNROLL; u++) {
-            const long long rr = r + u;
-            ok[u] = false;
-            if (rr < r1) {
-                const long long i = j - ((long long)(keys[rr] - base) - M);
-                if (i >= 0 && i < M) {
-                    const long long off = (long long)perm[rr] * row_stride + i;
-                    va[u] = rowsA[off];
-                    ve[u] = rowsE[off];
-                    ok[u] = true;
-                }
-            }
+            const long long i = j - st[u];
+            ok[u] = i >= 0 && i < M;
+            const long long ic = i < 0 ? 0 : (i >= M ? M - 1 : i);
+            va[u] = rowsA[rowoff[u] + ic];
+            ve[u] = rowsE[rowoff[u] + ic];
         }
+        // the next group's metadata while this group's rows are in flight
+        if (r + GATHER_UNROLL < r1) meta(r + GATHER_UNROLL);
 #pragma unroll
         for (int u = 0; u < GATHER_UNROLL; u++) {
-            if (ok[u]) {
-                ar += va[u].x;
-                ai += va[u].y;
-                er += ve[u].x;
-                ei += ve[u].y;
-            }
+            ar += ok[u] ? va[u].x : 0.0;
+            ai += ok[u] ? va[u].y : 0.0;
+            er += ok[u] ? ve[u].x : 0.0;
+            ei += ok[u] ? ve[u].y : 0.0;
         }
     }
     part[w][lane] = make_double4(ar, ai, er, ei);
