@@ -35,9 +35,9 @@ HBM_PEAK_GBS = 8000.0
 # HBM bytes per launch of the fused FP64 kernel on the default workload
 # (1e6 binaries, N = 256), from rocprofv3 FETCH_SIZE and WRITE_SIZE passes of
 # this script (profiles/k_wave_256_fp64_uniform_1e6_counters.md):
-# 305,745 KiB x 2.0 + 32,015 KiB x 1.0, the factors being what the counters
+# 415,203 KiB x 2.0 + 32,018 KiB x 1.0, the factors being what the counters
 # read for a 1 GiB device copy (0.5 GiB fetched, 1 GiB written).
-MEASURED_TRAFFIC_BYTES_1E6_N256 = (305745 * 2.0 + 32015 * 1.0) * 1024
+MEASURED_TRAFFIC_BYTES_1E6_N256 = (415203 * 2.0 + 32018 * 1.0) * 1024
 
 
 def flop_per_binary(N: int) -> float:
@@ -299,6 +299,59 @@ def differentiation(gb, A, E, SA, SE, d_d, T, dt, steps, warmup, B=100_000, seed
     return res
 
 
+def dropin_get_ll(params, A, E, SA, SE, d_d, df, L, N, T, dt, steps, warmup, ll_fused):
+    """The reference's two-step path on the GPU: run_wave materialises the A/E
+    templates in its (N, B)-major layout, then the drop-in native get_ll
+    (gbgpu_get_ll; src/gbgpu/cutils/gbgpu_utils.cu:125-210) reduces them
+    against the data and PSD windows.  HBM-bound: 80 N + 44 bytes per binary
+    (templates 32 N, data 32 N, PSD 16 N; start index, outputs)."""
+    import torch
+
+    from gbgpu_amd import GBGPU, _capi
+
+    print("[bench] drop-in get_ll", file=sys.stderr, flush=True)
+    B = params.shape[1]
+    gbw = GBGPU()
+    gbw.run_wave(*params, N=N, T=T, dt=dt)
+    A_t, E_t = gbw.A_out, gbw.E_out
+    start = torch.as_tensor(gbw.start_inds, device=A.device).to(torch.int32)
+    d_h = torch.zeros(B, dtype=torch.complex128, device=A.device)
+    h_h = torch.zeros_like(d_h)
+    lib = _capi.load()
+    st = _capi.stream_handle()
+
+    def call():
+        rc = lib.gbgpu_get_ll(d_h.data_ptr(), h_h.data_ptr(), A_t.data_ptr(), E_t.data_ptr(), A.data_ptr(),
+                              E.data_ptr(), SA.data_ptr(), SE.data_ptr(), df, start.data_ptr(), N, B, None, None,
+                              L, st)
+        _capi.check(rc, "gbgpu_get_ll")
+
+    for _ in range(warmup):
+        call()
+    torch.cuda.synchronize()
+    with _capi.KernelTimer() as kt:
+        sec, _ = time_calls(call, steps, 0)
+    ll = -0.5 * (d_d + h_h.real - 2.0 * d_h.real)
+    err = float((ll - ll_fused).abs().max())
+    kernel_s = kt.mean_ms / 1e3
+    nbytes = (80.0 * N + 44.0) * B
+    out = dict(
+        workload=f"gbgpu_get_ll on run_wave templates: {B} binaries, N={N}, (N, B)-major A/E",
+        binaries_per_s=B / sec,
+        ms_per_call=sec * 1e3,
+        kernel="k_get_ll",
+        kernel_ms=kernel_s * 1e3,
+        bytes_per_binary=80.0 * N + 44.0,
+        algorithmic_GB_per_s=nbytes / kernel_s / 1e9,
+        hbm_peak_GB_per_s=HBM_PEAK_GBS,
+        frac=nbytes / kernel_s / 1e9 / HBM_PEAK_GBS,
+        max_abs_logL_diff_vs_fused=err,
+    )
+    del gbw, A_t, E_t, d_h, h_h
+    torch.cuda.empty_cache()
+    return out
+
+
 def global_and_swap(gb, SA, SE, T, dt, B=10_000_000, P=100_000, chunk_samples=1 << 28, seed=11):
     """Global template of 1e7 binaries (f0 log-uniform 0.1-10 mHz, N by the
     frequency rule) summed deterministically into one 4-yr A/E pair, twice
@@ -435,6 +488,7 @@ def main():
     ap.add_argument("--uneven-calls", type=int, default=100, help="calls in the captured uneven-N loop")
     ap.add_argument("--no-deriv", action="store_true", help="skip the differentiation workload")
     ap.add_argument("--no-global", action="store_true", help="skip the global-template and swap workload")
+    ap.add_argument("--no-dropin", action="store_true", help="skip the drop-in run_wave + native get_ll path")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build measures one GPU only; --gpus must be 1"}))
@@ -506,6 +560,12 @@ def main():
       except RuntimeError as e:
         mixed = dict(error=str(e)[:200])
 
+    dropin = None
+    if not args.no_dropin and args.only is None:
+        call64()
+        torch.cuda.synchronize()
+        dropin = dropin_get_ll(params, A, E, SA, SE, d_d, df, L, N, T, dt, args.steps, args.warmup, ll.clone())
+
     uneven = None
     if not args.no_uneven:
         uneven = uneven_batch(gb, A, E, SA, SE, d_d, args.num_bin, T, dt, args.uneven_calls)
@@ -563,6 +623,7 @@ def main():
         "parity": par64,
         "mixed": mixed,
         "uneven_batch": uneven,
+        "dropin_get_ll": dropin,
         "differentiation": deriv,
         "global_template_and_swap": glob,
         "wall_ms_per_step": wall64 * 1e3,
