@@ -77,6 +77,30 @@ def test_orbit_table_layout():
     assert np.allclose(tab[16:19].T, r13, rtol=0, atol=1e-15)
 
 
+def test_orbit_table_refuses_a_stretched_constellation():
+    """The kernels' phase-offset series assume |P_s - Pc| <= L/sqrt(3).
+    Moving spacecraft 1 to Pc + f (P_1 - Pc) puts it (2f + 1)/3 L/sqrt(3) from
+    the new centroid: f = 2.5 (2.0) is refused, f = 1.1 (1.07) is accepted
+    within the 1.3 margin."""
+
+    class Stretched(EqualArmlengthOrbits):
+        def __init__(self, factor):
+            super().__init__()
+            self.factor = factor
+
+        def get_pos(self, t, sc):
+            P = [super(Stretched, self).get_pos(t, k) for k in (1, 2, 3)]
+            Pc = (P[0] + P[1] + P[2]) / 3
+            s = int(np.atleast_1d(sc)[0])
+            if s == 1:
+                return Pc + self.factor * (P[0] - Pc)
+            return P[s - 1]
+
+    orbit_table(Stretched(1.1), YR, 32)
+    with pytest.raises(ValueError, match="constellation centre"):
+        orbit_table(Stretched(2.5), YR, 32)
+
+
 def test_get_N_bands_without_sensitivity():
     f0 = np.array([5e-4, 2e-3, 0.02, 0.05, 0.2])
     amp = np.full(5, 1e-30)
