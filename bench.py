@@ -380,7 +380,7 @@ def global_and_swap(gb, SA, SE, T, dt, B=10_000_000, P=100_000, chunk_samples=1 
     buckets = [(int(n), np.flatnonzero(Nb == n)) for n in np.unique(Nb)]
     maxN = max(n for n, _ in buckets)
     C = max(1, chunk_samples // maxN)
-    aet = torch.empty((C, 3, maxN), dtype=torch.complex128, device=dev)
+    aet = torch.empty((C, 2, maxN), dtype=torch.complex128, device=dev)
     starts = torch.empty(C, dtype=torch.int64, device=dev)
     gidx = torch.zeros(C, dtype=torch.int32, device=dev)
     pd_all = torch.from_numpy(np.ascontiguousarray(p)).to(dev)
@@ -394,14 +394,14 @@ def global_and_swap(gb, SA, SE, T, dt, B=10_000_000, P=100_000, chunk_samples=1 
                 sel = torch.from_numpy(idx[c0:c0 + C]).to(dev)
                 pc = pd_all[:, sel].contiguous()
                 m = pc.shape[1]
-                a = aet[:m].view(-1)[: m * 3 * n].view(m, 3, n)
-                _capi.check(lib.gbgpu_run_wave(None, a.data_ptr(), starts.data_ptr(), pc.data_ptr(), 9, m, cfg, st),
-                            "run_wave")
+                a = aet[:m].view(-1)[: m * 2 * n].view(m, 2, n)
+                _capi.check(lib.gbgpu_run_wave_ae(a.data_ptr(), starts.data_ptr(), pc.data_ptr(), 9, m, cfg, st),
+                            "run_wave_ae")
                 # the gather-sum adds into the (zeroed) global arrays; chunks and
                 # buckets always come in the same order, so the sum is reproducible
                 with _capi.KernelTimer() as kt:
                     _capi.check(lib.gbgpu_fill_global_rows(total[0].data_ptr(), total[1].data_ptr(),
-                                                           a[:, 0].data_ptr(), a[:, 1].data_ptr(), 3 * n,
+                                                           a[:, 0].data_ptr(), a[:, 1].data_ptr(), 2 * n,
                                                            starts.data_ptr(), 0, n, m, gidx.data_ptr(), L, 1, st),
                                 "fill_global_rows")
                 fill_ms += kt.total_ms

@@ -88,6 +88,10 @@ SIGNATURES = {
         c_int,
         [c_void_p, c_void_p, c_void_p, c_void_p, c_int, c_int, ctypes.POINTER(WaveCfg), c_void_p],
     ),
+    "gbgpu_run_wave_ae": (
+        c_int,
+        [c_void_p, c_void_p, c_void_p, c_int, c_int, ctypes.POINTER(WaveCfg), c_void_p],
+    ),
     "gbgpu_wave_ll": (
         c_int,
         [c_void_p, c_void_p, c_void_p, c_void_p, c_int, c_int, ctypes.POINTER(WaveCfg), ctypes.POINTER(Data),

@@ -124,6 +124,18 @@ __device__ __forceinline__ void wave_pass(const WaveArgs &a, WaveLds<N, MODE, TH
             }
             if (lane == 0) a.start_inds[b] = s.start;
         }
+    } else if constexpr (MODE == MODE_AE) {
+        if (valid) {
+            double2 *ae = a.AETf + b * 2 * N;
+#pragma unroll
+            for (int q = 0; q < SPT; q++) {
+                const int m = lane + q * TPB;
+                const int idx = (m + N / 2) & (N - 1);  // fftshift
+                ae[m] = cmul(P::up(buf[idx]), fac);
+                ae[N + m] = cmul(P::up(buf[N + idx]), fac);
+            }
+            if (lane == 0) a.start_inds[b] = s.start;
+        }
     } else {
         double dhr = 0.0, dhi = 0.0, hh = 0.0;
         if (valid) {
@@ -360,6 +372,24 @@ extern "C" int gbgpu_run_wave(gbgpu_cmplx *XYZf, gbgpu_cmplx *AETf, long long *s
     if (!a.tw) return hip_fail(hipErrorOutOfMemory, "twiddles");
     return launch_any<MODE_WAVE>(cfg->N, num_params == GBGPU_NPARAMS_THIRD, cfg->precision, a,
                                  static_cast<hipStream_t>(stream));
+}
+
+extern "C" int gbgpu_run_wave_ae(gbgpu_cmplx *AEf, long long *start_inds, const double *params, int num_params,
+                                 int num_bin, const gbgpu_wave_cfg *cfg, void *stream) {
+    int rc = check_cfg(params, num_params, num_bin, cfg);
+    if (rc) return rc;
+    if (!AEf || !start_inds) {
+        set_error("AEf and start_inds must not be null");
+        return GBGPU_E_ARG;
+    }
+    if (num_bin == 0) return GBGPU_OK;
+    WaveArgs a = base_args(params, num_params, num_bin, cfg);
+    a.AETf = reinterpret_cast<double2 *>(AEf);
+    a.start_inds = start_inds;
+    a.tw = twiddles_for(cfg->N, cfg->precision);
+    if (!a.tw) return hip_fail(hipErrorOutOfMemory, "twiddles");
+    return launch_any<MODE_AE>(cfg->N, num_params == GBGPU_NPARAMS_THIRD, cfg->precision, a,
+                               static_cast<hipStream_t>(stream));
 }
 
 extern "C" int gbgpu_wave_ll(double *loglike, gbgpu_cmplx *d_h, gbgpu_cmplx *h_h, const double *params,

@@ -10,7 +10,9 @@
 
 namespace gbgpu {
 
-enum { MODE_WAVE = 0, MODE_LL = 1 };
+// MODE_WAVE: X, Y, Z (and A, E, T) spectra to HBM; MODE_LL: fused likelihood;
+// MODE_AE: the A and E spectra only (two channels through the transform).
+enum { MODE_WAVE = 0, MODE_LL = 1, MODE_AE = 2 };
 
 constexpr int WG_SIZE = 256;
 

@@ -498,11 +498,36 @@ class GBGPUBase(abc.ABC):
         templates[:, 1] = tE
 
     def generate_global_template(self, params, group_index, templates, start_freq_ind=0, **kwargs):
-        """run_wave then fill_global_template (src/gbgpu/gbgpu.py:839-878)."""
-        params = np.asarray(params)
-        self.run_wave(*params.T, **kwargs)
-        self.fill_global_template(group_index, templates, self.AETf[:, 0], self.AETf[:, 1], self.start_inds,
-                                  self.N, start_freq_ind=start_freq_ind)
+        """run_wave then fill_global_template (src/gbgpu/gbgpu.py:839-878).
+
+        Only the A and E spectra are computed for the fill
+        (``gbgpu_run_wave_ae``); the other run_wave outputs (``XYZf``,
+        ``AETf``, ``A``, ...) are computed on first access, as after get_ll.
+        """
+        torch = _torch()
+        self._check_hooks()
+        params = np.asarray(params).T
+        N = kwargs.pop("N", None)
+        T = kwargs.pop("T", 4 * YRSID_SI)
+        dt = kwargs.pop("dt", 10.0)
+        oversample = kwargs.pop("oversample", 1)
+        tdi2 = kwargs.pop("tdi2", False)
+        precision = kwargs.pop("precision", None)
+        if kwargs:
+            raise TypeError(f"unexpected keyword arguments {list(kwargs)}")
+        p, N, T = self._prepare(*params[:9], params[9:], N, T, dt, oversample)
+        B = p.shape[1]
+        self.num_bin, self.N, self.df = B, N, 1 / T
+        cfg, _tab = self._cfg(N, T, tdi2, precision)
+        AE = torch.empty((B, 2, N), dtype=torch.complex128, device=p.device)
+        start = torch.empty((B,), dtype=torch.int64, device=p.device)
+        rc = _capi.load().gbgpu_run_wave_ae(AE.data_ptr(), start.data_ptr(), p.data_ptr(), p.shape[0], B, cfg,
+                                            _capi.stream_handle())
+        _capi.check(rc, "gbgpu_run_wave_ae")
+        self.start_inds = self.kmin = start
+        self._wave_state = (p, N, T, tdi2, precision)
+        self.fill_global_template(group_index, templates, AE[:, 0], AE[:, 1], start, N,
+                                  start_freq_ind=start_freq_ind)
 
     def inject_signal(self, *args, fmax=None, T=4.0 * YRSID_SI, dt=10.0, **kwargs):
         """Place binary 0 into zero A/E arrays on [0, fmax] (src/gbgpu/gbgpu.py:880-946)."""

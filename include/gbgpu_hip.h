@@ -113,6 +113,15 @@ int gbgpu_fill_global_rows(gbgpu_cmplx *A_glob, gbgpu_cmplx *E_glob, const gbgpu
 int gbgpu_run_wave(gbgpu_cmplx *XYZf, gbgpu_cmplx *AETf, long long *start_inds, const double *params,
                    int num_params, int num_bin, const gbgpu_wave_cfg *cfg, void *stream);
 
+/* The A and E outputs of run_wave only, for callers that consume nothing
+ * else -- generate_global_template (src/gbgpu/gbgpu.py:839-878), which runs
+ * run_wave (:108-316) and fills A and E: AEf (num_bin, 2, N) complex128,
+ * fftshifted and scaled as gbgpu_run_wave's AETf[:, 0:2], and start_inds.
+ * Two channels go through the transform instead of three, and a third less
+ * is written. */
+int gbgpu_run_wave_ae(gbgpu_cmplx *AEf, long long *start_inds, const double *params, int num_params,
+                      int num_bin, const gbgpu_wave_cfg *cfg, void *stream);
+
 /* Fused run_wave + get_ll (src/gbgpu/gbgpu.py:580-735): per binary computes
  * d_h, h_h (complex, as get_ll_wrap) and loglike = -1/2 Re(d_d + h_h - 2 d_h),
  * with |d_h| in place of d_h when phase_marginalize (gbgpu.py:714-728).
@@ -237,7 +246,7 @@ int gbgpu_perturb_params(double *out, const double *in, int num_params, int num_
 /* Message describing the last failure on the calling thread. */
 const char *gbgpu_last_error(void);
 
-/* ABI version of this header (2). */
+/* ABI version of this header (3). */
 int gbgpu_abi_version(void);
 
 /* Measurement hook with no counterpart in the reference: while enabled,

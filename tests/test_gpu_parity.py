@@ -165,10 +165,51 @@ def test_windows_straddling_the_data_edges():
     ref = F.fill_global(1, L, o["AETf"][:, 0], o["AETf"][:, 1], o["start_inds"] - k0, np.zeros(B, int))
     got = tm.cpu().numpy()
     assert np.abs(got - ref).max() <= 1e-9 * np.abs(ref).max()
-    # and exactly the clipped placement of the HIP templates themselves
-    ref_hip = F.fill_global(1, L, gb.AETf[:, 0].cpu().numpy(), gb.AETf[:, 1].cpu().numpy(),
-                            gb.start_inds.cpu().numpy() - k0, np.zeros(B, int))
+    # and exactly the clipped placement of the HIP templates themselves (the
+    # A/E-only spectra generate_global_template fills)
+    ae, st = _run_wave_ae(p, N, T)
+    ref_hip = F.fill_global(1, L, ae[:, 0], ae[:, 1], st - k0, np.zeros(B, int))
     assert np.abs(got - ref_hip).max() <= 1e-15 * np.abs(ref_hip).max()
+
+
+def _run_wave_ae(p, N, T, gb=None):
+    """gbgpu_run_wave_ae through the C ABI: (B, 2, N) A/E and start bins
+    (host arguments prepared as run_wave prepares them)."""
+    import torch
+
+    from gbgpu_amd import _capi
+
+    gb = gb if gb is not None else _gb()
+    pt, N, Tadj = gb._prepare(*p[:9], p[9:], N, T, DT, 1)
+    B = pt.shape[1]
+    cfg, _tab = gb._cfg(N, Tadj, False, "fp64")
+    ae = torch.empty((B, 2, N), dtype=torch.complex128, device="cuda")
+    st = torch.empty(B, dtype=torch.int64, device="cuda")
+    _capi.check(_capi.load().gbgpu_run_wave_ae(ae.data_ptr(), st.data_ptr(), pt.data_ptr(), pt.shape[0], B, cfg,
+                                               _capi.stream_handle()), "gbgpu_run_wave_ae")
+    return ae.cpu().numpy(), st.cpu().numpy()
+
+
+@pytest.mark.parametrize("N,third", [(128, False), (256, False), (2048, False), (256, True), (1024, True)])
+def test_run_wave_ae_matches_run_wave(N, third):
+    """The A/E-only path (A, E formed before the transform, two channels)
+    equals run_wave's A and E to rounding (1e-13 of the largest bin) and
+    gives the same start bins; both match the oracle."""
+    from gbgpu_amd import GBGPUThirdBody
+
+    rng = np.random.default_rng(40 + N)
+    B = 48
+    p = draw_params(B, rng, f_lo=1e-3, f_hi=2e-2 if N == 2048 else 6e-3)
+    if third:  # A2, varpi, e2, P2 (yr), T2 (yr)
+        p = np.vstack([p, rng.uniform(0, 100, B), rng.uniform(0, 2 * np.pi, B), np.full(B, 0.3),
+                       10 ** rng.uniform(0, 1, B), rng.uniform(0, 1, B)])
+    T = YR
+    gb = GBGPUThirdBody() if third else _gb()
+    gb.run_wave(*p, N=N, T=T, dt=DT)
+    ae, st = _run_wave_ae(p, N, T, gb)
+    ref = gb.AETf[:, :2].cpu().numpy()
+    assert np.array_equal(st, gb.start_inds.cpu().numpy())
+    assert np.abs(ae - ref).max() <= 1e-13 * np.abs(ref).max()
 
 
 def test_get_ll_multi_stream_indices():
