@@ -378,11 +378,12 @@ def global_and_swap(gb, SA, SE, T, dt, B=10_000_000, P=100_000, chunk_samples=1 
                        "into one 4-yr A/E pair; swap of 1e5 proposals, circular and third-body",
            "num_bin": B, "N_fractions": {int(n): float(np.mean(Nb == n)) for n in np.unique(Nb)}}
     buckets = [(int(n), np.flatnonzero(Nb == n)) for n in np.unique(Nb)]
-    maxN = max(n for n, _ in buckets)
-    C = max(1, chunk_samples // maxN)
-    aet = torch.empty((C, 2, maxN), dtype=torch.complex128, device=dev)
-    starts = torch.empty(C, dtype=torch.int64, device=dev)
-    gidx = torch.zeros(C, dtype=torch.int32, device=dev)
+    # chunks of chunk_samples template samples per N bucket (C_n = chunk_samples / n binaries)
+    minN = min(n for n, _ in buckets)
+    Cmax = max(1, chunk_samples // minN)
+    aet = torch.empty(2 * chunk_samples, dtype=torch.complex128, device=dev)
+    starts = torch.empty(Cmax, dtype=torch.int64, device=dev)
+    gidx = torch.zeros(Cmax, dtype=torch.int32, device=dev)
     pd_all = torch.from_numpy(np.ascontiguousarray(p)).to(dev)
 
     def generate():
@@ -390,11 +391,12 @@ def global_and_swap(gb, SA, SE, T, dt, B=10_000_000, P=100_000, chunk_samples=1 
         fill_ms, fill_n, tmpl_bins = 0.0, 0, 0
         for n, idx in buckets:
             cfg, _tab = gb._cfg(n, Tadj, False, "fp64")
+            C = max(1, chunk_samples // n)
             for c0 in range(0, len(idx), C):
                 sel = torch.from_numpy(idx[c0:c0 + C]).to(dev)
                 pc = pd_all[:, sel].contiguous()
                 m = pc.shape[1]
-                a = aet[:m].view(-1)[: m * 2 * n].view(m, 2, n)
+                a = aet[: m * 2 * n].view(m, 2, n)
                 _capi.check(lib.gbgpu_run_wave_ae(a.data_ptr(), starts.data_ptr(), pc.data_ptr(), 9, m, cfg, st),
                             "run_wave_ae")
                 # the gather-sum adds into the (zeroed) global arrays; chunks and
