@@ -151,8 +151,15 @@ __device__ inline double kepler_u(double l, double e) {
     return u;
 }
 
-// Line-of-sight velocity (m/s), src/gbgpu/thirdbody.py:322-380.
+// Line-of-sight velocity (m/s), src/gbgpu/thirdbody.py:322-380.  A circular
+// orbit (e2 = 0) needs no Kepler solve: the eccentric anomaly is the mean
+// anomaly, which the solver would return up to rounding.
 __device__ inline double third_vlos(const Src &s, double t) {
+    if (s.e2 == 0.0) {
+        double sn, cs;
+        sincos_f64(s.n2 * (t - s.T2) + s.varpi, sn, cs);
+        return s.A2 * sn;
+    }
     double u = kepler_u(s.n2 * (t - s.T2), s.e2);
     if (s.e2 > 1e-6) {
         const double bb = (1.0 - sqrt(1.0 - s.e2 * s.e2)) / s.e2;
