@@ -87,7 +87,7 @@ __global__ void __launch_bounds__(WG_SIZE) __attribute__((amdgpu_waves_per_eu(2)
     CT *buf = sbuf + slot * 2 * N;
     double *tot = stot + slot * WPB;
     const CT *tw = static_cast<const CT *>(a.tw);
-    ThirdScratch ts{nullptr, nullptr, nullptr};
+    ThirdScratch ts{nullptr};
     binary_spectrum<N, 2, false, P>(s, spack[slot].aux, valid, lane, a.tab, a.fstar_inv, a.scale, buf, tw, ts, tot);
 
     // forward likelihood sums

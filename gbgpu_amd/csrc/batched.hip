@@ -208,7 +208,7 @@ __device__ __noinline__ void batch_item(const BatchArgs &ba, int k, int local, i
     const Src &s = spack[slot].s;  // read in place from LDS
     CT *buf = sbuf + slot * 2 * N;
     double *tot = stot + slot * WPB;
-    ThirdScratch ts{nullptr, nullptr, nullptr};
+    ThirdScratch ts{nullptr};
     binary_spectrum<N, 2, false, P>(s, spack[slot].aux, valid, lane, ba.tabs[k], ba.a.fstar_inv, 0.5 * ba.a.T / N,
                                     buf, static_cast<const CT *>(ba.tws[k]), ts, tot);
     const double2 fac = P::fac(s);

@@ -38,16 +38,12 @@ template <int N, bool THIRD, class P>
 struct SwapLds {
     using G = SwapGeo<N>;
     using CT = typename P::CT;
-    static constexpr int NT = THIRD ? N : 0;
-    static constexpr bool XI_IN_BUF = 2 * sizeof(CT) * N >= 3 * sizeof(double) * N;
-    static constexpr int NXI = (THIRD && !XI_IN_BUF) ? N : 0;
     static constexpr bool PACK_IN_LDS = G::TPB < 64;
+    static constexpr int NEDGE = (THIRD && G::WPB > 1) ? 6 * G::WPB : 0;
     CT sbuf[G::BPW * 2 * N];
     Pack<P> spack[PACK_IN_LDS ? G::BPW : 0];
     double stot[G::WPB > 1 ? G::BPW * G::WPB : 0];
-    float s_v[G::BPW * 3 * NT];
-    double s_g[G::BPW * 3 * NT];
-    double s_xi[G::BPW * 3 * NXI];
+    double s_edge[G::BPW * NEDGE];
 };
 
 template <int N, bool THIRD, class P>
@@ -57,8 +53,6 @@ __global__ void __launch_bounds__(SwapGeo<N>::WG) k_swap(WaveArgs a, double *dll
     constexpr int TPB = G::TPB, BPW = G::BPW, WPB = G::WPB;
     constexpr int SPT = Geo<N>::SPT;
     __shared__ SwapLds<N, THIRD, P> L;
-    constexpr bool XI_IN_BUF = SwapLds<N, THIRD, P>::XI_IN_BUF;
-    constexpr int NT = SwapLds<N, THIRD, P>::NT, NXI = SwapLds<N, THIRD, P>::NXI;
     constexpr bool PACK_IN_LDS = SwapLds<N, THIRD, P>::PACK_IN_LDS;
     CT *sbuf = L.sbuf;
 
@@ -86,8 +80,7 @@ __global__ void __launch_bounds__(SwapGeo<N>::WG) k_swap(WaveArgs a, double *dll
     const Src s = per_group<TPB>(pk->s);
     CT *buf = sbuf + slot * 2 * N;
     double *tot = L.stot + slot * WPB;
-    ThirdScratch ts{XI_IN_BUF ? reinterpret_cast<double *>(buf) : L.s_xi + slot * 3 * NXI, L.s_v + slot * 3 * NT,
-                    L.s_g + slot * 3 * NT};
+    ThirdScratch ts{L.s_edge + slot * SwapLds<N, THIRD, P>::NEDGE};
     binary_spectrum<N, 2, THIRD, P>(s, per_group<TPB>(pk->aux), valid, lane, a.tab, a.fstar_inv, a.scale, buf,
                                     static_cast<const CT *>(a.tw), ts, tot);
 

@@ -17,12 +17,6 @@
 
 namespace gbgpu {
 
-// LDS budget (160 KB per workgroup on gfx950): the FP64 third-body run_wave
-// variant at N = 2048 would need 168 KB.
-template <int N, int MODE, bool THIRD, class P>
-constexpr bool fits_lds() {
-    return !(THIRD && MODE == MODE_WAVE && N > 1024 && sizeof(typename P::CT) == 16);
-}
 
 // LDS of one workgroup of k_wave.  PK is the precision policy whose Pack
 // layout the per-binary constants were written in (normally P; the FP64
@@ -35,18 +29,13 @@ struct WaveLds {
     using G = Geo<N, M>;
     using CT = typename P::CT;
     static constexpr int NCH = (MODE == MODE_WAVE) ? 3 : 2;
-    static constexpr int NT = THIRD ? N : 0;
-    // xi shares the transform buffer when that is large enough
-    static constexpr bool XI_IN_BUF = NCH * sizeof(CT) * N >= 3 * sizeof(double) * N;
-    static constexpr int NXI = (THIRD && !XI_IN_BUF) ? N : 0;
     // a wavefront that owns its binary reads the Pack straight into scalar registers
     static constexpr bool PACK_IN_LDS = G::TPB < 64;
+    static constexpr int NEDGE = (THIRD && G::WPB > 1) ? 6 * G::WPB : 0;
     CT sbuf[G::BPW * NCH * N];
     Pack<PK> spack[PACK_IN_LDS ? G::BPW : 0];
     double stot[G::WPB > 1 ? G::BPW * G::WPB : 0];
-    float s_v[G::BPW * 3 * NT];
-    double s_g[G::BPW * 3 * NT];
-    double s_xi[G::BPW * 3 * NXI];
+    double s_edge[G::BPW * NEDGE];
 };
 
 // One pass of a workgroup: items blk*BPW .. blk*BPW + BPW-1 of the
@@ -67,14 +56,10 @@ __device__ __forceinline__ void wave_pass(const WaveArgs &a, WaveLds<N, MODE, TH
     using W = WaveLds<N, MODE, THIRD, P, PK, M>;
     using CT = typename P::CT;
     constexpr int TPB = G::TPB, BPW = G::BPW, WPB = G::WPB, SPT = G::SPT;
-    constexpr int NCH = W::NCH, NT = W::NT, NXI = W::NXI;
-    constexpr bool XI_IN_BUF = W::XI_IN_BUF;
+    constexpr int NCH = W::NCH;
     CT *sbuf = L.sbuf;
     Pack<PK> *spack = L.spack;
     double *stot = L.stot;
-    float *s_v = L.s_v;
-    double *s_g = L.s_g;
-    double *s_xi = L.s_xi;
     const int slot = per_group<TPB>(tid / TPB);
     const int lane = tid % TPB;
     const long long item = blk * BPW + slot;
@@ -93,8 +78,7 @@ __device__ __forceinline__ void wave_pass(const WaveArgs &a, WaveLds<N, MODE, TH
     const Src s = per_group<TPB>(pk->s);
     CT *buf = sbuf + slot * NCH * N;
     double *tot = stot + slot * WPB;
-    ThirdScratch ts{XI_IN_BUF ? reinterpret_cast<double *>(buf) : s_xi + slot * 3 * NXI, s_v + slot * 3 * NT,
-                    s_g + slot * 3 * NT};
+    ThirdScratch ts{L.s_edge + slot * W::NEDGE};
 
     binary_spectrum<N, NCH, THIRD, P, M>(s, per_group<TPB>(aux_of<P, PK>(*pk)), valid, lane, tab, a.fstar_inv,
                                          a.scale, buf, static_cast<const CT *>(twp), ts, tot);
@@ -195,20 +179,15 @@ template <int MODE, bool THIRD, class P>
 static int launch_wave(int N, const WaveArgs &a, hipStream_t st) {
 #define GBGPU_CASE(NN)                                                                              \
     case NN: {                                                                                      \
-        if constexpr (fits_lds<NN, MODE, THIRD, P>()) {                                             \
-            using GG = Geo<NN, wave_tile<NN>()>;                                                    \
-            const long long grid = (a.B + GG::BPW - 1) / GG::BPW;                                   \
-            return with_pack<P>(a, NN, st, [&](const WaveArgs &aa) -> int {                         \
-                KernelTimer kt(st);                                                                 \
-                hipLaunchKernelGGL((k_wave<NN, MODE, THIRD, P>), dim3((unsigned)grid), dim3(GG::WG),  \
-                                   0, st, aa);                                                      \
-                GBGPU_CHECK_LAUNCH("k_wave");                                                       \
-                return GBGPU_OK;                                                                    \
-            });                                                                                     \
-        } else {                                                                                    \
-            set_error("FP64 third-body run_wave supports N <= 1024 (LDS limit)");                   \
-            return GBGPU_E_ARG;                                                                     \
-        }                                                                                           \
+        using GG = Geo<NN, wave_tile<NN>()>;                                                        \
+        const long long grid = (a.B + GG::BPW - 1) / GG::BPW;                                       \
+        return with_pack<P>(a, NN, st, [&](const WaveArgs &aa) -> int {                             \
+            KernelTimer kt(st);                                                                     \
+            hipLaunchKernelGGL((k_wave<NN, MODE, THIRD, P>), dim3((unsigned)grid), dim3(GG::WG), 0, \
+                               st, aa);                                                             \
+            GBGPU_CHECK_LAUNCH("k_wave");                                                           \
+            return GBGPU_OK;                                                                        \
+        });                                                                                         \
     }
     switch (N) {
         GBGPU_CASE(32)

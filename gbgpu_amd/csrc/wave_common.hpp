@@ -228,17 +228,23 @@ __device__ __forceinline__ double group_total(double x, int lane, double *tot) {
     return x;
 }
 
-// LDS scratch of one binary for the third-body pre-pass (3N each).
+// LDS of one binary for the third-body pre-pass: when the binary spans
+// several wavefronts, the last sample's xi and g of each wavefront (per
+// spacecraft) for the next one (6 WPB doubles); unused otherwise.
 struct ThirdScratch {
-    double *xi;
-    float *v;
-    double *g;  // g = v f_GW, then the cumulative phase term
+    double *edge;
 };
 
 // Computes the NCH spectra of this group's binary into `buf` (NCH x N,
 // natural FFT order, before fftshift and before P::fac()).  NCH == 3 gives
 // X, Y, Z; NCH == 2 gives A, E.  Invalid slots (beyond the batch) write
 // zeros and still take part in every barrier.
+//
+// Each thread evaluates SPT samples: n = lane + q TPB, or with a third body
+// n = SPT lane + q, so that the pre-pass values of a thread's own consecutive
+// samples stay in its registers; only the predecessor of its first sample
+// comes from the previous lane (a shuffle) or the previous wavefront
+// (ThirdScratch).
 template <int N, int NCH, bool THIRD, class P, int M = 1>
 __device__ __forceinline__ void binary_spectrum(const Src &s, const typename P::Aux &aux, bool valid, int lane,
                                                 const double *__restrict__ tab, double fstar_inv, double scale,
@@ -247,13 +253,18 @@ __device__ __forceinline__ void binary_spectrum(const Src &s, const typename P::
     using G = Geo<N, M>;
     using CT = typename P::CT;
     constexpr int TPB = G::TPB, SPT = G::SPT;
+    constexpr int NT = THIRD ? SPT : 1;
+    float vf[3][NT];   // v_LOS at xi_sc(t_n)
+    double gc[3][NT];  // the cumulative phase term at t_n
     if constexpr (THIRD) {
         // Third-body pre-pass (thirdbody.py:155-226, :382-428): v_LOS and
         // g = v_LOS f_GW at xi_sc(t_n), then the cumulative trapezoid
         // term_n = sum_{m=1..n} (xi_m - xi_{m-1}) (g_{m-1} + g_m)/2 * 2 pi / c.
+        constexpr int W = TPB < 64 ? TPB : 64;
+        double xs[3][SPT];
 #pragma unroll
-        for (int q = 0; q < SPT; q++) {
-            const int n = lane + q * TPB;
+        for (int i = 0; i < SPT; i++) {
+            const int n = SPT * lane + i;
             double kP[3] = {0.0, 0.0, 0.0};
             if (valid) load_kP(s, tab, N, n, kP);
 #pragma unroll
@@ -266,12 +277,36 @@ __device__ __forceinline__ void binary_spectrum(const Src &s, const typename P::
                     v = (float)vd;
                     gg = vd * (s.f0 + s.fdot * xi + 0.5 * s.fddot * xi * xi);
                 }
-                ts.xi[sc * N + n] = xi;
-                ts.v[sc * N + n] = v;
-                ts.g[sc * N + n] = gg;
+                xs[sc][i] = xi;
+                vf[sc][i] = v;
+                gc[sc][i] = gg;
             }
         }
-        __syncthreads();
+        // sample SPT lane - 1 is the previous lane's last
+        double xprev[3], gprev[3];
+#pragma unroll
+        for (int sc = 0; sc < 3; sc++) {
+            xprev[sc] = __shfl_up(xs[sc][SPT - 1], 1, W);
+            gprev[sc] = __shfl_up(gc[sc][SPT - 1], 1, W);
+        }
+        if constexpr (TPB > 64) {
+            const int w = lane / 64, wl = lane % 64;
+            if (wl == 63) {
+#pragma unroll
+                for (int sc = 0; sc < 3; sc++) {
+                    ts.edge[6 * w + 2 * sc] = xs[sc][SPT - 1];
+                    ts.edge[6 * w + 2 * sc + 1] = gc[sc][SPT - 1];
+                }
+            }
+            __syncthreads();
+            if (wl == 0 && w > 0) {
+#pragma unroll
+                for (int sc = 0; sc < 3; sc++) {
+                    xprev[sc] = ts.edge[6 * (w - 1) + 2 * sc];
+                    gprev[sc] = ts.edge[6 * (w - 1) + 2 * sc + 1];
+                }
+            }
+        }
         double loc[3][SPT];
 #pragma unroll
         for (int sc = 0; sc < 3; sc++) {
@@ -280,25 +315,27 @@ __device__ __forceinline__ void binary_spectrum(const Src &s, const typename P::
             for (int i = 0; i < SPT; i++) {
                 const int n = SPT * lane + i;
                 if (n > 0) {
-                    const double dtt = ts.xi[sc * N + n] - ts.xi[sc * N + n - 1];
-                    acc += dtt * (ts.g[sc * N + n - 1] + ts.g[sc * N + n]) / 2.0 * PI * 2 / C_SI;
+                    const double xp = i > 0 ? xs[sc][i - 1] : xprev[sc];
+                    const double gp = i > 0 ? gc[sc][i - 1] : gprev[sc];
+                    const double dtt = xs[sc][i] - xp;
+                    acc += dtt * (gp + gc[sc][i]) / 2.0 * PI * 2 / C_SI;
                 }
                 loc[sc][i] = acc;
             }
         }
-        __syncthreads();
 #pragma unroll
         for (int sc = 0; sc < 3; sc++) {
             const double pre = group_excl_scan<TPB>(loc[sc][SPT - 1], lane, tot);
 #pragma unroll
-            for (int i = 0; i < SPT; i++) ts.g[sc * N + SPT * lane + i] = pre + loc[sc][i];
+            for (int i = 0; i < SPT; i++) gc[sc][i] = pre + loc[sc][i];
         }
-        __syncthreads();
     }
     using R = decltype(CT{}.x);
 #pragma unroll
     for (int q = 0; q < SPT; q++) {
-        const int n = lane + q * TPB;
+        // third body: the thread's own consecutive samples (pre-pass values in
+        // registers); otherwise strided, so orbit-table loads coalesce
+        const int n = THIRD ? SPT * lane + q : lane + q * TPB;
         CT X{}, Y{}, Z{};
         if (valid) {
             float v3[3] = {0.f, 0.f, 0.f};
@@ -306,8 +343,8 @@ __device__ __forceinline__ void binary_spectrum(const Src &s, const typename P::
             if constexpr (THIRD) {
 #pragma unroll
                 for (int sc = 0; sc < 3; sc++) {
-                    v3[sc] = ts.v[sc * N + n];
-                    t3[sc] = ts.g[sc * N + n];
+                    v3[sc] = vf[sc][q];
+                    t3[sc] = gc[sc][q];
                 }
             }
             P::template sample<THIRD>(s, aux, tab, N, n, fstar_inv, scale, v3, t3, X, Y, Z);
