@@ -374,6 +374,28 @@ def test_third_body_matches_oracle(e2):
     assert rel_err(gb.XYZf.cpu().numpy(), o["XYZf"], axis=(1, 2)).max() <= 1e-8
 
 
+@pytest.mark.parametrize("N", [128, 1024, 2048])
+def test_third_body_matches_oracle_across_tiles(N):
+    """The third-body pre-pass hands each thread's first sample its predecessor
+    by a shuffle within a wavefront and through LDS across wavefronts: at
+    N = 128 several binaries share a wavefront (shuffle width 32), at 1024 a
+    binary spans 4 wavefronts and at 2048 (512-thread tile) 8.  FP64 run_wave
+    against the oracle, eccentric orbits."""
+    from gbgpu_amd import GBGPUThirdBody
+
+    rng = np.random.default_rng(32 + N)
+    B = 6
+    p = draw_params(B, rng, f_lo=1e-3, f_hi=2e-2 if N == 2048 else 6e-3)
+    third = [rng.uniform(0, 100, B), rng.uniform(0, 2 * np.pi, B), np.full(B, 0.3), 10 ** rng.uniform(0, 1, B),
+             rng.uniform(0, 1, B)]
+    print(f"third body N={N}", flush=True)
+    gb = GBGPUThirdBody()
+    gb.run_wave(*p, *third, N=N, T=YR, dt=DT)
+    o = F.run_wave(*p, *third, N=N, T=YR, dt=DT)
+    assert np.array_equal(gb.start_inds.cpu().numpy(), o["start_inds"])
+    assert rel_err(gb.XYZf.cpu().numpy(), o["XYZf"], axis=(1, 2)).max() <= 1e-8
+
+
 # ------------------------------------------------------------------ mixed ---
 # Mixed precision bars: per-binary relative L2 waveform error <= 1e-6 against
 # the FP64 oracle; logL error <= 1e-6 * (|<h|h>| + |<d|h>|) + 1e-6, i.e. the
