@@ -9,10 +9,11 @@
 //
 // fill_global: deterministic replacement of the reference's atomic scatter
 // (gbgpu_utils.cu:55-101).  Binaries are stably radix-sorted by
-// key = group * (L + M) + (start + M); a wavefront then owns a 64-bin output
+// key = group * (L + M) + (start + M); a workgroup then owns a 64-bin output
 // tile, finds the contiguous key range of binaries overlapping it by binary
-// search, and sums their samples in sorted order -- one thread per output
-// bin, fixed summation order, so results are bit-identical run to run.
+// search, splits it over its wavefronts and sums their samples in sorted
+// order -- one lane per output bin, fixed summation order, so results are
+// bit-identical run to run.
 #include <hipcub/hipcub.hpp>
 
 #include "common.hpp"
