@@ -24,7 +24,7 @@ inline int with_pack(WaveArgs a, int N, hipStream_t st, Launch launch) {
     if (sort) {
         hipError_t e = hipcub::DeviceRadixSort::SortPairs(nullptr, sort_bytes, (unsigned *)nullptr,
                                                           (unsigned *)nullptr, (int *)nullptr, (int *)nullptr,
-                                                          (int)a.B, 0, 32, st);
+                                                          (int)a.B, 0, SORT_KEY_BITS, st);
         if (e != hipSuccess) return hip_fail(e, "SortPairs(size)");
     }
     auto al = [](size_t n) { return (n + 255) & ~(size_t)255; };
@@ -54,7 +54,8 @@ inline int with_pack(WaveArgs a, int N, hipStream_t st, Launch launch) {
     hipLaunchKernelGGL((k_setup<P>), dim3((unsigned)((a.B + 255) / 256)), dim3(256), 0, st, a, N, pk);
     int rc = GBGPU_OK;
     if (sort) {
-        e = hipcub::DeviceRadixSort::SortPairs(tmp, sort_bytes, keys, keys_out, vals, order, (int)a.B, 0, 32, st);
+        e = hipcub::DeviceRadixSort::SortPairs(tmp, sort_bytes, keys, keys_out, vals, order, (int)a.B, 0,
+                                               SORT_KEY_BITS, st);
         if (e != hipSuccess) rc = hip_fail(e, "SortPairs");
     }
     a.keys = nullptr;

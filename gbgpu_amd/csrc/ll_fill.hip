@@ -260,9 +260,15 @@ static int fill_rows_impl(double2 *Aglob, double2 *Eglob, const double2 *rowsA, 
     void *tmp = nullptr;
     size_t tmp_bytes = 0;
     hipError_t e;
-    // end_bit: keys are < G * (L + M) <= 2^62 or KEY_NONE; sorting the full
-    // 64 bits keeps KEY_NONE last.
-    e = hipcub::DeviceRadixSort::SortPairs(nullptr, tmp_bytes, keys, keys_sorted, vals, perm, (int)B, 0, 64, st);
+    // Keys are < G (L + M), or KEY_NONE, whose low bits are all ones.  Sorting
+    // the low end_bit bits with 2^end_bit > G (L + M) therefore keeps the full
+    // 64-bit order (KEY_NONE last) and the sort stable, in ceil(end_bit / 8)
+    // radix passes (3 for one 4-yr stream) instead of 8.
+    const unsigned long long span = (unsigned long long)G * (unsigned long long)(L + M);
+    int end_bit = 1;
+    while (end_bit < 64 && (1ULL << end_bit) <= span) end_bit++;
+    e = hipcub::DeviceRadixSort::SortPairs(nullptr, tmp_bytes, keys, keys_sorted, vals, perm, (int)B, 0, end_bit,
+                                           st);
     if (e != hipSuccess) return hip_fail(e, "SortPairs(size)");
     const size_t kb = sizeof(unsigned long long) * (size_t)B, vb = sizeof(int) * (size_t)B;
     char *ws = nullptr;
@@ -282,7 +288,7 @@ static int fill_rows_impl(double2 *Aglob, double2 *Eglob, const double2 *rowsA, 
     tmp = carve(tmp_bytes);
     hipLaunchKernelGGL(k_fill_keys, dim3((unsigned)((B + 255) / 256)), dim3(256), 0, st, keys, vals, start, start64,
                        group, B, M, L, G, start_shift);
-    e = hipcub::DeviceRadixSort::SortPairs(tmp, tmp_bytes, keys, keys_sorted, vals, perm, (int)B, 0, 64, st);
+    e = hipcub::DeviceRadixSort::SortPairs(tmp, tmp_bytes, keys, keys_sorted, vals, perm, (int)B, 0, end_bit, st);
     if (e != hipSuccess) {
         (void)hipFreeAsync(ws, st);
         return hip_fail(e, "SortPairs");

@@ -81,6 +81,9 @@ struct WaveArgs {
 // with a 30% margin.
 constexpr double GUARD_REL = 2.5e-6;
 
+// Bits of the start-bin sort keys written by k_setup.
+constexpr int SORT_KEY_BITS = 24;
+
 // Per-binary constants for one precision policy, computed once per binary
 // by k_setup (64 binaries per wavefront) and copied into LDS by the
 // waveform kernels.
@@ -100,9 +103,11 @@ __global__ void __launch_bounds__(256) k_setup(WaveArgs a, int N, Pack<P> *__res
     out[b] = pk;
     if (a.keys) {
         // order by start bin: neighbouring workgroups then read overlapping
-        // data/PSD windows, which stay resident in L2
-        long long k = pk.s.start + (1LL << 31);
-        k = k < 0 ? 0 : (k > 0xffffffffLL ? 0xffffffffLL : k);
+        // data/PSD windows, which stay resident in L2.  24-bit keys (start
+        // bins within +-2^23, f0 up to 66 mHz at 4 yr; others clamped, which
+        // only affects the order) let the radix sort run three 8-bit passes.
+        long long k = pk.s.start + (1LL << (SORT_KEY_BITS - 1));
+        k = k < 0 ? 0 : (k >= (1LL << SORT_KEY_BITS) ? (1LL << SORT_KEY_BITS) - 1 : k);
         a.keys[b] = (unsigned)k;
         a.vals[b] = (int)b;
     }
