@@ -66,6 +66,10 @@ def test_empty_batches_are_noops_on_every_entry_point():
     dat = _capi.Data(A=P, E=P, psdA=P, psdE=P, data_index=None, noise_index=None, data_length=10,
                      start_freq_ind=0, df=1e-7)
     assert lib.gbgpu_run_wave(None, P, P, P, 9, 0, cfg, None) == 0
+    assert lib.gbgpu_run_wave_ae(P, P, P, 9, 0, cfg, None) == 0
+    # the A/E-only entry needs both outputs
+    assert lib.gbgpu_run_wave_ae(None, P, P, 9, 4, cfg, None) == -1
+    assert b"AEf" in lib.gbgpu_last_error()
     assert lib.gbgpu_wave_ll(P, None, None, P, 9, 0, cfg, dat, 0.0, 0, None) == 0
     assert lib.gbgpu_swap_ll_diff(P, None, P, 9, 0, cfg, dat, None) == 0
     assert lib.gbgpu_wave_ll_grad(P, P, None, None, P, 9, 0, cfg, dat, 0.0, 0, None) == 0

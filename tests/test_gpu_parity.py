@@ -172,7 +172,7 @@ def test_windows_straddling_the_data_edges():
     assert np.abs(got - ref_hip).max() <= 1e-15 * np.abs(ref_hip).max()
 
 
-def _run_wave_ae(p, N, T, gb=None):
+def _run_wave_ae(p, N, T, gb=None, precision="fp64"):
     """gbgpu_run_wave_ae through the C ABI: (B, 2, N) A/E and start bins
     (host arguments prepared as run_wave prepares them)."""
     import torch
@@ -182,7 +182,7 @@ def _run_wave_ae(p, N, T, gb=None):
     gb = gb if gb is not None else _gb()
     pt, N, Tadj = gb._prepare(*p[:9], p[9:], N, T, DT, 1)
     B = pt.shape[1]
-    cfg, _tab = gb._cfg(N, Tadj, False, "fp64")
+    cfg, _tab = gb._cfg(N, Tadj, False, precision)
     ae = torch.empty((B, 2, N), dtype=torch.complex128, device="cuda")
     st = torch.empty(B, dtype=torch.int64, device="cuda")
     _capi.check(_capi.load().gbgpu_run_wave_ae(ae.data_ptr(), st.data_ptr(), pt.data_ptr(), pt.shape[0], B, cfg,
@@ -210,6 +210,17 @@ def test_run_wave_ae_matches_run_wave(N, third):
     ref = gb.AETf[:, :2].cpu().numpy()
     assert np.array_equal(st, gb.start_inds.cpu().numpy())
     assert np.abs(ae - ref).max() <= 1e-13 * np.abs(ref).max()
+
+
+def test_run_wave_ae_mixed_precision():
+    """Mixed-precision A/E-only spectra: per-binary relative L2 error within
+    the mixed waveform bar (1e-6) of the FP64 ones, same start bins."""
+    rng = np.random.default_rng(47)
+    p = draw_params(64, rng, f_lo=1e-3, f_hi=6e-3)
+    ae64, st64 = _run_wave_ae(p, 256, YR)
+    aem, stm = _run_wave_ae(p, 256, YR, precision="mixed")
+    assert np.array_equal(st64, stm)
+    assert rel_err(aem, ae64, axis=(1, 2)).max() <= 1e-6
 
 
 def test_get_ll_multi_stream_indices():
