@@ -364,7 +364,9 @@ static int launch_buckets(const BatchArgs &ba, int precision, hipStream_t st, co
             r.seg_count = ba.ctl + CTL_REDO + k;
             r.tw = twiddles_f64(N);
             if (!r.tw) return hip_fail(hipErrorOutOfMemory, "twiddles");
-            rc = launch_ll_segment(N, GBGPU_FP64, GBGPU_MIXED, var, r, st);
+            // the FP64 pass takes the FP64 tile of the bucket (tuned or default),
+            // as the plain FP64 batch does: same speed, same results
+            rc = launch_ll_segment(N, GBGPU_FP64, GBGPU_MIXED, g_variant[0][k].load(), r, st);
             if (rc) return rc;
         }
         if (bucket_events) (void)hipEventRecord(bucket_events[2 * k + 1], st);
