@@ -35,9 +35,9 @@ HBM_PEAK_GBS = 8000.0
 # HBM bytes per launch of the fused FP64 kernel on the default workload
 # (1e6 binaries, N = 256), from rocprofv3 FETCH_SIZE and WRITE_SIZE passes of
 # this script (profiles/k_wave_256_fp64_uniform_1e6_counters.md):
-# 415,212 KiB x 2.0 + 32,008 KiB x 1.0, the factors being what the counters
+# 415,216 KiB x 2.0 + 32,002 KiB x 1.0, the factors being what the counters
 # read for a 1 GiB device copy (0.5 GiB fetched, 1 GiB written).
-MEASURED_TRAFFIC_BYTES_1E6_N256 = (415212 * 2.0 + 32008 * 1.0) * 1024
+MEASURED_TRAFFIC_BYTES_1E6_N256 = (415216 * 2.0 + 32002 * 1.0) * 1024
 
 
 def flop_per_binary(N: int) -> float:
