@@ -20,6 +20,7 @@ GBGPU_MIXED = 1
 GBGPU_BATCH_BUCKETS = 0
 GBGPU_BATCH_PERSISTENT = 1
 ORBIT_ROWS = 22
+NOALLOC_MAX_BIN = 65536
 
 c_void_p = ctypes.c_void_p
 c_int = ctypes.c_int
@@ -93,6 +94,11 @@ SIGNATURES = {
         [c_void_p, c_void_p, c_void_p, c_int, c_int, ctypes.POINTER(WaveCfg), c_void_p],
     ),
     "gbgpu_wave_ll": (
+        c_int,
+        [c_void_p, c_void_p, c_void_p, c_void_p, c_int, c_int, ctypes.POINTER(WaveCfg), ctypes.POINTER(Data),
+         c_double, c_int, c_void_p],
+    ),
+    "gbgpu_wave_ll_noalloc": (
         c_int,
         [c_void_p, c_void_p, c_void_p, c_void_p, c_int, c_int, ctypes.POINTER(WaveCfg), ctypes.POINTER(Data),
          c_double, c_int, c_void_p],

@@ -65,6 +65,9 @@ int hip_fail(hipError_t e, const char *where);
 // Device twiddle table w[k] = exp(-2 pi i k / N), k < N, cached per (device, N).
 const double2 *twiddles_f64(int N);
 const float2 *twiddles_f32(int N);
+// The same tables written to host memory (N entries).
+void host_twiddles_f64(int N, double2 *out);
+void host_twiddles_f32(int N, float2 *out);
 
 inline bool is_pow2(long long n) { return n > 0 && (n & (n - 1)) == 0; }
 

@@ -1,7 +1,6 @@
 // Host-side launch helper shared by the waveform, likelihood and derivative
-// kernels: per-binary set-up into a stream-ordered workspace, optional
-// start-bin ordering, the caller's launch, then release of the workspace on
-// the same stream.
+// kernels: per-binary set-up into a workspace, optional start-bin ordering,
+// the caller's launch, then release of the workspace on the same stream.
 #pragma once
 
 #include <hipcub/hipcub.hpp>
@@ -11,31 +10,54 @@
 
 namespace gbgpu {
 
-// Per-binary set-up into a stream-ordered workspace, then the waveform
-// kernel, then the workspace is released on the same stream.
 // Batches of at least this many binaries are processed in start-bin order.
 constexpr long long SORT_MIN_BATCH = 8192;
 
-template <class P, class Launch>
-inline int with_pack(WaveArgs a, int N, hipStream_t st, Launch launch) {
-    const size_t B = (size_t)a.B;
-    const bool sort = a.B >= SORT_MIN_BATCH;
-    size_t sort_bytes = 0;
-    if (sort) {
-        hipError_t e = hipcub::DeviceRadixSort::SortPairs(nullptr, sort_bytes, (unsigned *)nullptr,
+inline size_t align256(size_t n) { return (n + 255) & ~(size_t)255; }
+
+// Bytes of the per-binary workspace of with_pack for B binaries: the Pack
+// array and, for sorted batches, keys, values and the radix sort's temporary
+// storage.  The size query of the sort allocates nothing.
+template <class P>
+inline int pack_workspace_bytes(long long B, hipStream_t st, size_t *total, size_t *sort_bytes) {
+    *sort_bytes = 0;
+    if (B >= SORT_MIN_BATCH) {
+        hipError_t e = hipcub::DeviceRadixSort::SortPairs(nullptr, *sort_bytes, (unsigned *)nullptr,
                                                           (unsigned *)nullptr, (int *)nullptr, (int *)nullptr,
-                                                          (int)a.B, 0, SORT_KEY_BITS, st);
+                                                          (int)B, 0, SORT_KEY_BITS, st);
         if (e != hipSuccess) return hip_fail(e, "SortPairs(size)");
     }
-    auto al = [](size_t n) { return (n + 255) & ~(size_t)255; };
-    const size_t bytes = al(sizeof(Pack<P>) * B) + (sort ? 2 * al(4 * B) + 2 * al(4 * B) + al(sort_bytes) : 0);
-    char *ws = nullptr;
-    hipError_t e = hipMallocAsync((void **)&ws, bytes, st);
-    if (e != hipSuccess) return hip_fail(e, "hipMallocAsync(per-binary workspace)");
+    *total = align256(sizeof(Pack<P>) * (size_t)B) +
+             (B >= SORT_MIN_BATCH ? 4 * align256(4 * (size_t)B) + align256(*sort_bytes) : 0);
+    return GBGPU_OK;
+}
+
+// Per-binary set-up (k_setup) into a workspace, the start-bin sort for large
+// batches, then `launch(args)`.  With ext == nullptr the workspace is
+// stream-ordered scratch (hipMallocAsync, freed with hipFreeAsync after the
+// launch); otherwise it is carved from the caller's ext_bytes at ext and
+// nothing is allocated.
+template <class P, class Launch>
+inline int with_pack(WaveArgs a, int N, hipStream_t st, Launch launch, char *ext = nullptr, size_t ext_bytes = 0) {
+    const size_t B = (size_t)a.B;
+    const bool sort = a.B >= SORT_MIN_BATCH;
+    size_t bytes = 0, sort_bytes = 0;
+    int rc = pack_workspace_bytes<P>(a.B, st, &bytes, &sort_bytes);
+    if (rc) return rc;
+    char *ws = ext;
+    if (ext) {
+        if (bytes > ext_bytes) {
+            set_error("per-binary workspace exceeds the caller's buffer");
+            return GBGPU_E_ARG;
+        }
+    } else {
+        hipError_t e = hipMallocAsync((void **)&ws, bytes, st);
+        if (e != hipSuccess) return hip_fail(e, "hipMallocAsync(per-binary workspace)");
+    }
     size_t off = 0;
     auto carve = [&](size_t n) {
         char *p = ws + off;
-        off += al(n);
+        off += align256(n);
         return p;
     };
     Pack<P> *pk = (Pack<P> *)carve(sizeof(Pack<P>) * B);
@@ -52,8 +74,9 @@ inline int with_pack(WaveArgs a, int N, hipStream_t st, Launch launch) {
     a.keys = keys;
     a.vals = vals;
     hipLaunchKernelGGL((k_setup<P>), dim3((unsigned)((a.B + 255) / 256)), dim3(256), 0, st, a, N, pk);
-    int rc = GBGPU_OK;
-    if (sort) {
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) rc = hip_fail(e, "k_setup");
+    if (!rc && sort) {
         e = hipcub::DeviceRadixSort::SortPairs(tmp, sort_bytes, keys, keys_out, vals, order, (int)a.B, 0,
                                                SORT_KEY_BITS, st);
         if (e != hipSuccess) rc = hip_fail(e, "SortPairs");
@@ -63,6 +86,7 @@ inline int with_pack(WaveArgs a, int N, hipStream_t st, Launch launch) {
     a.pack = pk;
     a.order = order;
     if (!rc) rc = launch(a);
+    if (ext) return rc;
     e = hipFreeAsync(ws, st);
     if (rc) return rc;
     if (e != hipSuccess) return hip_fail(e, "hipFreeAsync(per-binary workspace)");

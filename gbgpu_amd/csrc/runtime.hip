@@ -21,6 +21,16 @@ int hip_fail(hipError_t e, const char *where) {
 }
 
 namespace {
+template <class C>
+void host_twiddles(int N, C *out) {
+    for (int k = 0; k < N; k++) {
+        // long double keeps the table correctly rounded in float64
+        long double ang = -2.0L * 3.141592653589793238462643383279502884L * (long double)k / (long double)N;
+        out[k].x = (decltype(out[k].x))std::cos(ang);
+        out[k].y = (decltype(out[k].y))std::sin(ang);
+    }
+}
+
 std::mutex g_tw_mutex;
 std::map<std::pair<int, int>, void *> g_tw64, g_tw32;
 
@@ -33,12 +43,7 @@ const C *get_twiddles(std::map<std::pair<int, int>, void *> &cache, int N) {
     auto it = cache.find(key);
     if (it != cache.end()) return static_cast<const C *>(it->second);
     std::vector<C> host(N);
-    for (int k = 0; k < N; k++) {
-        // long double keeps the table correctly rounded in float64
-        long double ang = -2.0L * 3.141592653589793238462643383279502884L * (long double)k / (long double)N;
-        host[k].x = (decltype(host[k].x))std::cos(ang);
-        host[k].y = (decltype(host[k].y))std::sin(ang);
-    }
+    host_twiddles(N, host.data());
     void *dptr = nullptr;
     if (hipMalloc(&dptr, sizeof(C) * N) != hipSuccess) return nullptr;
     if (hipMemcpy(dptr, host.data(), sizeof(C) * N, hipMemcpyHostToDevice) != hipSuccess) {
@@ -49,6 +54,9 @@ const C *get_twiddles(std::map<std::pair<int, int>, void *> &cache, int N) {
     return static_cast<const C *>(dptr);
 }
 }  // namespace
+
+void host_twiddles_f64(int N, double2 *out) { host_twiddles(N, out); }
+void host_twiddles_f32(int N, float2 *out) { host_twiddles(N, out); }
 
 const double2 *twiddles_f64(int N) { return get_twiddles<double2>(g_tw64, N); }
 const float2 *twiddles_f32(int N) { return get_twiddles<float2>(g_tw32, N); }
@@ -108,4 +116,4 @@ extern "C" int gbgpu_kernel_timing_read(double *total_ms, int *count) {
 
 extern "C" const char *gbgpu_last_error(void) { return g_last_error.c_str(); }
 
-extern "C" int gbgpu_abi_version(void) { return 3; }
+extern "C" int gbgpu_abi_version(void) { return 4; }

@@ -9,7 +9,10 @@
 // data/PSD window with coalesced loads and reduces <d|h>, <h|h> over the
 // group (wave_ll).
 #include <hipcub/hipcub.hpp>
+#include <map>
+#include <mutex>
 #include <type_traits>
+#include <vector>
 
 #include "common.hpp"
 #include "dispatch.hpp"
@@ -175,19 +178,24 @@ __global__ void __launch_bounds__((Geo<N, M>::WG)) k_wave_seg(WaveArgs a) {
     }
 }
 
+// ws / ws_bytes: caller-provided per-binary workspace (nullptr: stream-ordered
+// scratch allocated by with_pack).
 template <int MODE, bool THIRD, class P>
-static int launch_wave(int N, const WaveArgs &a, hipStream_t st) {
+static int launch_wave(int N, const WaveArgs &a, hipStream_t st, char *ws = nullptr, size_t ws_bytes = 0) {
 #define GBGPU_CASE(NN)                                                                              \
     case NN: {                                                                                      \
         using GG = Geo<NN, wave_tile<NN>()>;                                                        \
         const long long grid = (a.B + GG::BPW - 1) / GG::BPW;                                       \
-        return with_pack<P>(a, NN, st, [&](const WaveArgs &aa) -> int {                             \
-            KernelTimer kt(st);                                                                     \
-            hipLaunchKernelGGL((k_wave<NN, MODE, THIRD, P>), dim3((unsigned)grid), dim3(GG::WG), 0, \
-                               st, aa);                                                             \
-            GBGPU_CHECK_LAUNCH("k_wave");                                                           \
-            return GBGPU_OK;                                                                        \
-        });                                                                                         \
+        return with_pack<P>(                                                                        \
+            a, NN, st,                                                                              \
+            [&](const WaveArgs &aa) -> int {                                                        \
+                KernelTimer kt(st);                                                                 \
+                hipLaunchKernelGGL((k_wave<NN, MODE, THIRD, P>), dim3((unsigned)grid), dim3(GG::WG), \
+                                   0, st, aa);                                                      \
+                GBGPU_CHECK_LAUNCH("k_wave");                                                       \
+                return GBGPU_OK;                                                                    \
+            },                                                                                      \
+            ws, ws_bytes);                                                                          \
     }
     switch (N) {
         GBGPU_CASE(32)
@@ -262,10 +270,13 @@ int launch_ll_segment(int N, int precision, int pack_precision, int variant, con
 }
 
 template <int MODE>
-static int launch_any(int N, bool third, int precision, const WaveArgs &a, hipStream_t st) {
+static int launch_any(int N, bool third, int precision, const WaveArgs &a, hipStream_t st, char *ws = nullptr,
+                      size_t ws_bytes = 0) {
     if (precision == GBGPU_MIXED)
-        return third ? launch_wave<MODE, true, Mixed>(N, a, st) : launch_wave<MODE, false, Mixed>(N, a, st);
-    return third ? launch_wave<MODE, true, F64>(N, a, st) : launch_wave<MODE, false, F64>(N, a, st);
+        return third ? launch_wave<MODE, true, Mixed>(N, a, st, ws, ws_bytes)
+                     : launch_wave<MODE, false, Mixed>(N, a, st, ws, ws_bytes);
+    return third ? launch_wave<MODE, true, F64>(N, a, st, ws, ws_bytes)
+                 : launch_wave<MODE, false, F64>(N, a, st, ws, ws_bytes);
 }
 
 int check_cfg(const double *params, int num_params, int num_bin, const gbgpu_wave_cfg *cfg) {
@@ -330,6 +341,83 @@ const void *twiddles_for(int N, int precision) {
     return precision == GBGPU_MIXED ? (const void *)twiddles_f32(N) : (const void *)twiddles_f64(N);
 }
 
+// ---- allocation-free fused likelihood (gbgpu_wave_ll_noalloc) ---------------
+//
+// `scratch` holds, at fixed offsets: the FP64 and FP32 twiddle tables of every
+// supported N (N = 32 << k, k < 7, back to back), then the per-binary
+// workspace of with_pack (Pack array, sort keys/values, radix-sort storage)
+// for up to GBGPU_NOALLOC_MAX_BIN binaries.  Each device holds its own copy
+// of the array for the life of the process.  In host code `scratch` names a
+// host-side stand-in: every host use goes through the device address that
+// hipGetSymbolAddress returns for the current device (noalloc_base).
+constexpr int NOALLOC_NK = 7;
+constexpr size_t NOALLOC_TW_ENTRIES = 32 * ((1 << NOALLOC_NK) - 1);  // sum of N over 32..2048
+constexpr size_t NOALLOC_TW64_BYTES = (16 * NOALLOC_TW_ENTRIES + 255) & ~(size_t)255;
+constexpr size_t NOALLOC_TW32_BYTES = (8 * NOALLOC_TW_ENTRIES + 255) & ~(size_t)255;
+constexpr size_t NOALLOC_TW_BYTES = NOALLOC_TW64_BYTES + NOALLOC_TW32_BYTES;
+constexpr size_t NOALLOC_BYTES = (size_t)24 << 20;
+constexpr size_t NOALLOC_WS_BYTES = NOALLOC_BYTES - NOALLOC_TW_BYTES;
+static_assert(sizeof(Pack<F64>) * (size_t)GBGPU_NOALLOC_MAX_BIN + ((size_t)2 << 20) < NOALLOC_WS_BYTES,
+              "scratch too small for GBGPU_NOALLOC_MAX_BIN FP64 packs");
+static_assert(sizeof(Pack<Mixed>) * (size_t)GBGPU_NOALLOC_MAX_BIN + ((size_t)2 << 20) < NOALLOC_WS_BYTES,
+              "scratch too small for GBGPU_NOALLOC_MAX_BIN mixed packs");
+
+static __device__ __align__(256) char scratch[NOALLOC_BYTES];
+
+namespace {
+std::mutex g_noalloc_mutex;
+std::map<int, char *> g_noalloc_base;  // device -> address of its scratch
+
+inline size_t tw_offset(int N) { return (size_t)N - 32; }  // entries before N's table (32 + 64 + ... = N - 32)
+
+// Device address of the current device's scratch, set up on first use: the
+// symbol address is looked up once per device, with that device current, and
+// the twiddle tables are copied in (blocking; nothing is allocated).
+int noalloc_base(char **out) {
+    int dev = 0;
+    hipError_t e = hipGetDevice(&dev);
+    if (e != hipSuccess) return hip_fail(e, "hipGetDevice");
+    std::lock_guard<std::mutex> lock(g_noalloc_mutex);
+    auto it = g_noalloc_base.find(dev);
+    if (it != g_noalloc_base.end()) {
+        *out = it->second;
+        return GBGPU_OK;
+    }
+    char *ptr = nullptr;
+    e = hipGetSymbolAddress((void **)&ptr, HIP_SYMBOL(scratch));
+    if (e != hipSuccess) return hip_fail(e, "hipGetSymbolAddress(scratch)");
+    if (!ptr) {
+        set_error("hipGetSymbolAddress(scratch) returned a null address");
+        return GBGPU_E_HIP;
+    }
+    // the radix sort's storage for the largest batch must fit beside the packs
+    for (int prec = 0; prec < 2; prec++) {
+        size_t total = 0, sort_bytes = 0;
+        int rc = prec == GBGPU_MIXED
+                     ? pack_workspace_bytes<Mixed>(GBGPU_NOALLOC_MAX_BIN, nullptr, &total, &sort_bytes)
+                     : pack_workspace_bytes<F64>(GBGPU_NOALLOC_MAX_BIN, nullptr, &total, &sort_bytes);
+        if (rc) return rc;
+        if (total > NOALLOC_WS_BYTES) {
+            set_error("allocation-free scratch is smaller than the workspace of GBGPU_NOALLOC_MAX_BIN binaries");
+            return GBGPU_E_ARG;
+        }
+    }
+    std::vector<double2> t64(NOALLOC_TW_ENTRIES);
+    std::vector<float2> t32(NOALLOC_TW_ENTRIES);
+    for (int k = 0; k < NOALLOC_NK; k++) {
+        const int N = 32 << k;
+        host_twiddles_f64(N, t64.data() + tw_offset(N));
+        host_twiddles_f32(N, t32.data() + tw_offset(N));
+    }
+    e = hipMemcpy(ptr, t64.data(), 16 * NOALLOC_TW_ENTRIES, hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = hipMemcpy(ptr + NOALLOC_TW64_BYTES, t32.data(), 8 * NOALLOC_TW_ENTRIES, hipMemcpyHostToDevice);
+    if (e != hipSuccess) return hip_fail(e, "hipMemcpy(twiddles to scratch)");
+    g_noalloc_base[dev] = ptr;
+    *out = ptr;
+    return GBGPU_OK;
+}
+}  // namespace
+
 }  // namespace gbgpu
 
 using namespace gbgpu;
@@ -393,4 +481,36 @@ extern "C" int gbgpu_wave_ll(double *loglike, gbgpu_cmplx *d_h, gbgpu_cmplx *h_h
     if (!a.tw) return hip_fail(hipErrorOutOfMemory, "twiddles");
     return launch_any<MODE_LL>(cfg->N, num_params == GBGPU_NPARAMS_THIRD, cfg->precision, a,
                                static_cast<hipStream_t>(stream));
+}
+
+extern "C" int gbgpu_wave_ll_noalloc(double *loglike, gbgpu_cmplx *d_h, gbgpu_cmplx *h_h, const double *params,
+                                    int num_params, int num_bin, const gbgpu_wave_cfg *cfg, const gbgpu_data *data,
+                                    double d_d, int phase_marginalize, void *stream) {
+    int rc = check_cfg(params, num_params, num_bin, cfg);
+    if (rc) return rc;
+    if ((rc = check_data(data))) return rc;
+    if (!loglike) {
+        set_error("loglike must not be null");
+        return GBGPU_E_ARG;
+    }
+    if (num_bin > GBGPU_NOALLOC_MAX_BIN) {
+        set_error("gbgpu_wave_ll_noalloc: num_bin exceeds GBGPU_NOALLOC_MAX_BIN (65536); split the batch or use "
+                  "gbgpu_wave_ll");
+        return GBGPU_E_ARG;
+    }
+    if (num_bin == 0) return GBGPU_OK;
+    char *base = nullptr;
+    if ((rc = noalloc_base(&base))) return rc;
+    WaveArgs a = base_args(params, num_params, num_bin, cfg);
+    set_data(a, data);
+    a.d_d = d_d;
+    a.phase_marg = phase_marginalize;
+    a.loglike = loglike;
+    a.d_h = reinterpret_cast<double2 *>(d_h);
+    a.h_h = reinterpret_cast<double2 *>(h_h);
+    a.tw = cfg->precision == GBGPU_MIXED ? (const void *)(reinterpret_cast<const float2 *>(base + NOALLOC_TW64_BYTES) +
+                                                         tw_offset(cfg->N))
+                                         : (const void *)(reinterpret_cast<const double2 *>(base) + tw_offset(cfg->N));
+    return launch_any<MODE_LL>(cfg->N, num_params == GBGPU_NPARAMS_THIRD, cfg->precision, a,
+                               static_cast<hipStream_t>(stream), base + NOALLOC_TW_BYTES, NOALLOC_WS_BYTES);
 }

@@ -130,6 +130,26 @@ int gbgpu_wave_ll(double *loglike, gbgpu_cmplx *d_h, gbgpu_cmplx *h_h, const dou
                   int num_params, int num_bin, const gbgpu_wave_cfg *cfg, const gbgpu_data *data,
                   double d_d, int phase_marginalize, void *stream);
 
+/* Largest num_bin gbgpu_wave_ll_noalloc accepts. */
+#define GBGPU_NOALLOC_MAX_BIN 65536
+
+/* gbgpu_wave_ll without any device allocation.  No counterpart in the
+ * reference: it is for callers that cannot allocate (code under HIP graph
+ * capture, tight sampler loops, harnesses that forbid hipMalloc).  Same
+ * arguments, semantics and kernels as gbgpu_wave_ll; the per-binary
+ * constants, sort keys and twiddle tables live in a fixed 24 MiB device
+ * array of the library (one per GPU, held until the process ends; the first
+ * call on a device looks up its address and copies the twiddle tables in,
+ * blocking).  num_bin <= GBGPU_NOALLOC_MAX_BIN, else GBGPU_E_ARG.  Calls
+ * never allocate, free or memset; every call fully rewrites the part of the
+ * array it reads, so nothing carries over from the previous call.  All
+ * calls on one GPU share the array: calls that overlap in time on one GPU
+ * (concurrent streams or host threads) race and must be serialised by the
+ * caller. */
+int gbgpu_wave_ll_noalloc(double *loglike, gbgpu_cmplx *d_h, gbgpu_cmplx *h_h, const double *params,
+                          int num_params, int num_bin, const gbgpu_wave_cfg *cfg, const gbgpu_data *data,
+                          double d_d, int phase_marginalize, void *stream);
+
 /* swap_likelihood_difference (no native counterpart in the reference; its
  * likelihood algebra, src/gbgpu/gbgpu.py:728 with the inner product of
  * src/gbgpu/cutils/gbgpu_utils.cu:156-183): for proposal p, remove binary
@@ -246,7 +266,7 @@ int gbgpu_perturb_params(double *out, const double *in, int num_params, int num_
 /* Message describing the last failure on the calling thread. */
 const char *gbgpu_last_error(void);
 
-/* ABI version of this header (3). */
+/* ABI version of this header (4). */
 int gbgpu_abi_version(void);
 
 /* Measurement hook with no counterpart in the reference: while enabled,

@@ -32,7 +32,7 @@ def test_library_exports_every_declared_symbol():
         assert hasattr(lib, n), n
     # the ctypes table covers exactly the declared functions
     assert set(_capi.SIGNATURES) == set(declared_functions())
-    assert lib.gbgpu_abi_version() == 3
+    assert lib.gbgpu_abi_version() == 4
 
 
 def test_invalid_arguments_return_errors_without_gpu_work():
@@ -71,6 +71,10 @@ def test_empty_batches_are_noops_on_every_entry_point():
     assert lib.gbgpu_run_wave_ae(None, P, P, 9, 4, cfg, None) == -1
     assert b"AEf" in lib.gbgpu_last_error()
     assert lib.gbgpu_wave_ll(P, None, None, P, 9, 0, cfg, dat, 0.0, 0, None) == 0
+    assert lib.gbgpu_wave_ll_noalloc(P, None, None, P, 9, 0, cfg, dat, 0.0, 0, None) == 0
+    # the allocation-free entry refuses batches beyond its fixed scratch
+    assert lib.gbgpu_wave_ll_noalloc(P, None, None, P, 9, _capi.NOALLOC_MAX_BIN + 1, cfg, dat, 0.0, 0, None) == -1
+    assert b"GBGPU_NOALLOC_MAX_BIN" in lib.gbgpu_last_error()
     assert lib.gbgpu_swap_ll_diff(P, None, P, 9, 0, cfg, dat, None) == 0
     assert lib.gbgpu_wave_ll_grad(P, P, None, None, P, 9, 0, cfg, dat, 0.0, 0, None) == 0
     assert lib.gbgpu_fisher(P, None, None, P, 9, 0, cfg, dat, 0.0, None) == 0
