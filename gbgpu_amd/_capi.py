@@ -173,9 +173,15 @@ def check(rc: int, what: str) -> None:
 
 
 def ptr(t) -> int | None:
-    """Device pointer of a torch tensor (None passes NULL)."""
+    """Device pointer of a torch GPU tensor (None passes NULL).
+
+    Host tensors are refused: the kernels would dereference a host address
+    (a GPU page fault with XNACK off)."""
     if t is None:
         return None
+    if getattr(getattr(t, "device", None), "type", None) != "cuda":
+        raise TypeError(f"expected a GPU (cuda) tensor, got {type(t).__name__} on "
+                        f"{getattr(t, 'device', 'host')}")
     return t.data_ptr()
 
 

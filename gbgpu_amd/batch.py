@@ -59,15 +59,18 @@ class BatchedLogLike:
         if len(self.N_levels) != len(self.f_edges) + 1:
             raise ValueError("need len(N_levels) == len(f_edges) + 1")
         self.phase_marginalize = bool(phase_marginalize)
+        if strategy not in self.STRATEGIES:
+            raise ValueError(f"strategy must be one of {list(self.STRATEGIES)}")
+        self.guard_tol = float(guard_tol or 0.0) if precision == "mixed" else 0.0
+        if self.guard_tol > 0 and strategy == "persistent":
+            raise ValueError("the accuracy guard runs with the 'buckets' strategy only; pass guard_tol=None "
+                             "for an unguarded persistent launch")
         dev = gb._device()
         self.A = gb._to_dev(data[0], np.complex128).reshape(-1)
         self.E = gb._to_dev(data[1], np.complex128).reshape(-1)
         self.SA = gb._to_dev(psd[0], np.float64).reshape(-1)
         self.SE = gb._to_dev(psd[1], np.float64).reshape(-1)
         self.L = int(self.A.numel())
-        if strategy not in self.STRATEGIES:
-            raise ValueError(f"strategy must be one of {list(self.STRATEGIES)}")
-        self.guard_tol = float(guard_tol or 0.0) if precision == "mixed" else 0.0
         self.cfg = _capi.BatchCfg(T=float(self.T), armlength=float(gb.orbits.armlength), tdi2=0,
                                   precision=1 if precision == "mixed" else 0, strategy=self.STRATEGIES[strategy],
                                   guard_tol=self.guard_tol)
@@ -79,6 +82,7 @@ class BatchedLogLike:
                                start_freq_ind=int(start_freq_ind), df=float(1.0 / self.T))
         self.dev = dev
         self._ws = None
+        self._last_ws = None  # workspace of the last launch (eager or captured)
         self._last_B = 0
         self._graph = None
 
@@ -91,6 +95,9 @@ class BatchedLogLike:
             self.cfg.orbit_tables[k] = self._tabs[N].data_ptr()
 
     def _workspace(self, B):
+        """Workspace for B binaries.  A buffer whose address a captured graph
+        holds is never replaced: a larger batch after capture gets a new
+        buffer, and the graph's stays referenced by self._graph."""
         torch = _torch()
         need = _capi.load().gbgpu_batch_workspace_bytes(B, self.cfg.precision)
         if self._ws is None or self._ws.numel() < need:
@@ -132,9 +139,9 @@ class BatchedLogLike:
         """Binaries per N (dict) the accuracy guard re-evaluated in FP64 in the
         last call (synchronises the current stream)."""
         counts = np.zeros(7, dtype=np.int32)
-        if self._ws is None:
+        if self._last_ws is None:
             return {}
-        _capi.check(_capi.load().gbgpu_batch_guard_counts(self._ws.data_ptr(), self._last_B, self.cfg.precision,
+        _capi.check(_capi.load().gbgpu_batch_guard_counts(self._last_ws.data_ptr(), self._last_B, self.cfg.precision,
                                                           counts.ctypes.data, _capi.stream_handle()),
                     "gbgpu_batch_guard_counts")
         return {32 << k: int(c) for k, c in enumerate(counts) if c}
@@ -164,6 +171,7 @@ class BatchedLogLike:
     def _launch(self, ll, dh, hh, p, Nt, B):
         ws = self._workspace(B)
         self._last_B = B
+        self._last_ws = ws
         rc = _capi.load().gbgpu_wave_ll_batched(ll.data_ptr(), _capi.ptr(dh), _capi.ptr(hh), p.data_ptr(), 9, B,
                                                 Nt.data_ptr(), self.cfg, self.data, float(np.real(self.gb.d_d)),
                                                 int(self.phase_marginalize), ws.data_ptr(), _capi.stream_handle())
@@ -219,5 +227,8 @@ class BatchedLogLike:
             with torch.cuda.graph(g, stream=s):
                 self._loop_body(*bufs, n_calls, sigma, seed)
         torch.cuda.current_stream().wait_stream(s)
-        self._graph = (g, bufs)
+        # the graph holds this workspace's address: keep it alive with the
+        # graph, and give later eager calls a buffer of their own
+        self._graph = (g, bufs, self._ws)
+        self._ws = None
         return g, out

@@ -303,6 +303,8 @@ class GBGPUBase(abc.ABC):
         noise_index = self._to_dev(noise_index, np.int32)
         assert len(data_index) == num_bin
         assert len(data_index) == len(noise_index)
+        if num_bin and (int(data_index.min()) < 0 or int(noise_index.min()) < 0):
+            raise ValueError("data_index/noise_index must be non-negative")
         assert int(data_index.max()) * data_length <= len(dA)
         assert int(noise_index.max()) * data_length <= len(dA)
         if int(data_index.max()) * data_length >= len(dA) or int(noise_index.max()) * data_length >= len(SA):
@@ -431,6 +433,18 @@ class GBGPUBase(abc.ABC):
             raise NotImplementedError("per-binary N supports circular binaries, TDI1 and a single data stream")
         if len(N) != params.shape[1]:
             raise ValueError("N must have one entry per binary")
+        if T > np.max(self.orbits.t_base):
+            raise ValueError(
+                f"Observation time ({T}) is larger than length of time in orbital information "
+                f"({np.max(self.orbits.t_base)})"
+            )
+        # 2-D streams without indices: every binary uses stream 0 (data_index
+        # and noise_index default to zero, src/gbgpu/gbgpu.py:675-678), and a
+        # window reaching past data_length is clipped there
+        if data[0].ndim == 2:
+            data = [data[0][0], data[1][0]]
+        if psd[0].ndim == 2:
+            psd = [psd[0][0], psd[1][0]]
         # same arithmetic as the uniform get_ll (no FP64 re-evaluation of loud binaries)
         bl = BatchedLogLike(self, data, psd, T, dt, precision=precision or self.precision,
                             start_freq_ind=start_freq_ind, N_levels=(int(N.min()),), f_edges=(),
@@ -465,8 +479,9 @@ class GBGPUBase(abc.ABC):
         num_bin = len(group_index)
         tA = torch.zeros((total_groups, data_length), dtype=torch.complex128, device=dev)
         tE = torch.zeros((total_groups, data_length), dtype=torch.complex128, device=dev)
-        A = A if torch.is_tensor(A) else torch.from_numpy(np.asarray(A)).to(dev)
-        E = E if torch.is_tensor(E) else torch.from_numpy(np.asarray(E)).to(dev)
+        # host arrays and host tensors are copied to the templates' device
+        A = (A if torch.is_tensor(A) else torch.from_numpy(np.asarray(A))).to(dev)
+        E = (E if torch.is_tensor(E) else torch.from_numpy(np.asarray(E))).to(dev)
         if A.ndim > 2 or E.ndim > 2:
             raise ValueError("A_in, E_in have maximum allowable dimension of 2.")
         lib = _capi.load()

@@ -73,3 +73,12 @@ def rel_err(a, b, axis=None):
     num = np.sqrt(np.sum(np.abs(a - b) ** 2, axis=axis))
     den = np.sqrt(np.sum(np.abs(b) ** 2, axis=axis))
     return num / den
+
+
+def ll_bar(d_h, h_h, rel=1e-9, atol=1e-6):
+    """FP64 log-likelihood bar, absolute, on the terms a template contributes:
+    logL = -(d_d + h_h - 2 Re d_h)/2, and d_d is an input, so
+    |dlogL| <= (atol + rel (|h_h| + 2 |d_h|)) / 2 per binary -- rel is the
+    FP64 waveform bar (a relative waveform error e moves h_h by 2 e h_h and
+    d_h by e |d_h|)."""
+    return 0.5 * (atol + rel * (np.abs(h_h) + 2 * np.abs(d_h)))

@@ -90,3 +90,29 @@ def test_bench_refuses_more_than_one_gpu():
                        timeout=120)
     assert r.returncode == 2
     assert "one GPU" in json.loads(r.stdout.strip().splitlines()[-1])["error"]
+
+
+def test_host_tensors_are_refused_as_device_pointers():
+    """A host (CPU) tensor never reaches a kernel as a pointer: the ctypes
+    layer refuses it before any call."""
+    import torch
+
+    from gbgpu_amd import _capi
+
+    with pytest.raises(TypeError, match="GPU"):
+        _capi.ptr(torch.zeros(4, dtype=torch.complex128))
+    with pytest.raises(TypeError, match="GPU"):
+        _capi.ptr(np.zeros(4))
+    assert _capi.ptr(None) is None
+
+
+def test_guard_with_persistent_strategy_refused():
+    """The mixed-precision accuracy guard exists only for per-bucket launches;
+    asking for it with the persistent launch is an error, not a no-op."""
+    from gbgpu_amd.batch import BatchedLogLike
+
+    gb = GBGPU()
+    gb.d_d = 0.0
+    z = [np.zeros(10, complex)] * 2
+    with pytest.raises(ValueError, match="buckets"):
+        BatchedLogLike(gb, z, [np.ones(10)] * 2, YR, 15.0, precision="mixed", strategy="persistent", guard_tol=1e-3)

@@ -87,21 +87,6 @@ def test_aet_closed_form(base):
     assert np.allclose(T * np.sqrt(3), X + Y + Z)
 
 
-def test_transform_is_scaled_shifted_dft(base):
-    """XYZf = (T/2N) * fftshift(DFT(slow part)): check against a direct DFT
-    of the slow part rebuilt from the oracle's own pieces."""
-    p, o = base
-    T = o["T"]
-    x = np.fft.ifftshift(o["XYZf"], axes=-1) / (0.5 * T / N)
-    n = np.arange(N)
-    Wm = np.exp(2j * np.pi * np.outer(n, n) / N) / N  # inverse DFT matrix
-    slow = x @ Wm.T  # back to time domain
-    # time-domain slow part is smooth: its spectrum is band-limited around the carrier
-    assert np.all(np.isfinite(slow))
-    again = np.fft.fftshift(np.fft.fft(slow, axis=-1), axes=-1) * (0.5 * T / N)
-    assert rel_err(again, o["XYZf"]) < 1e-12
-
-
 def test_start_bins_closed_form_and_rounding():
     T = F.adjust_T(YR, DT)
     ks = np.array([40000.5, 40001.5, 40002.49, 40002.51, 99999.0])
