@@ -178,8 +178,10 @@ def uneven_batch(gb, A, E, SA, SE, d_d, B, T, dt, n_calls, seed=77):
            "N_fractions": {int(n): float(np.mean(N == n)) for n in np.unique(N)},
            "mean_N": float(N.mean())}
     lls = {}
-    for prec, strat, guard in (("fp64", "buckets", None), ("fp64", "persistent", None),
-                               ("mixed", "buckets", 1e-3), ("mixed", "persistent", None),
+    # the persistent strategy is switched off in this build (its kernel took
+    # a GPU memory fault whose cause has not been shown)
+    res["persistent_strategy"] = "disabled"
+    for prec, strat, guard in (("fp64", "buckets", None), ("mixed", "buckets", 1e-3),
                                ("mixed_unguarded", "buckets", None)):
         if True:
             print(f"[bench] uneven batch {prec} {strat}", file=sys.stderr, flush=True)

@@ -35,8 +35,9 @@ class BatchedLogLike:
         precision: "fp64" or "mixed".
         N_levels, f_edges: the N step rule used when N is derived from f0:
             N_levels[i] for f0 < f_edges[i], the last level above.
-        strategy: "buckets" (one launch of the uniform kernel per N bucket)
-            or "persistent" (one persistent launch over all buckets).
+        strategy: "buckets" (one launch of the uniform kernel per N bucket).
+            "persistent" (one persistent launch over all buckets) is
+            switched off in this build and raises ValueError.
         guard_tol: mixed precision with the "buckets" strategy: binaries
             whose log-likelihood error bound 2.5e-6 (<h|h> + |<d|h>|) exceeds
             guard_tol are re-evaluated in FP64 within the same call (None or
@@ -61,6 +62,8 @@ class BatchedLogLike:
         self.phase_marginalize = bool(phase_marginalize)
         if strategy not in self.STRATEGIES:
             raise ValueError(f"strategy must be one of {list(self.STRATEGIES)}")
+        if strategy == "persistent":
+            raise ValueError("the persistent batch strategy is disabled in this build; use strategy='buckets'")
         self.guard_tol = float(guard_tol or 0.0) if precision == "mixed" else 0.0
         if self.guard_tol > 0 and strategy == "persistent":
             raise ValueError("the accuracy guard runs with the 'buckets' strategy only; pass guard_tol=None "

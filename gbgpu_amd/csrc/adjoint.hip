@@ -154,12 +154,7 @@ __global__ void __launch_bounds__(WG_SIZE) __attribute__((amdgpu_waves_per_eu(2)
                                      acc);
             }
         };
-        switch (spack[slot].s.tier) {
-            case 0: run(std::integral_constant<int, 0>{}); break;
-            case 1: run(std::integral_constant<int, 1>{}); break;
-            case 2: run(std::integral_constant<int, 2>{}); break;
-            default: run(std::integral_constant<int, 3>{}); break;
-        }
+        with_tier(spack[slot].s.tier, run);
     }
     double r[4 + NDIR + 2] = {acc.SP.x, acc.SP.y, acc.SC.x, acc.SC.y, acc.g[0], acc.g[1],
                               acc.g[2], acc.g[3], acc.g[4], nur, nui};
@@ -270,12 +265,9 @@ k_fisher(WaveArgs a, double *__restrict__ fisher, double *__restrict__ grad) {
             const int n = tid + q * FG::WG;
             if (n >= N) break;
             TangentOut o;
-            switch (s.tier) {
-                case 0: tangent_sample<0>(s, sd, a.tab, N, n, a.fstar_inv, a.scale, o); break;
-                case 1: tangent_sample<1>(s, sd, a.tab, N, n, a.fstar_inv, a.scale, o); break;
-                case 2: tangent_sample<2>(s, sd, a.tab, N, n, a.fstar_inv, a.scale, o); break;
-                default: tangent_sample<3>(s, sd, a.tab, N, n, a.fstar_inv, a.scale, o); break;
-            }
+            with_tier(s.tier, [&](auto tier_c) {
+                tangent_sample<decltype(tier_c)::value>(s, sd, a.tab, N, n, a.fstar_inv, a.scale, o);
+            });
 #pragma unroll
             for (int cl = 0; cl < FG::NCHP; cl++) {
                 const bool chA = FG::ONE_PASS ? (cl == 0) : (pass == 0);

@@ -84,11 +84,12 @@ __device__ __forceinline__ double horner_Id(double v) {
     return p;
 }
 
-// T(u) and dT/du.  Tiers 0-2 use the series (same lengths as the waveform);
-// tier 3 uses T = sinc(a) e^{-ia}, a = u/2, dT/du = (sinc'(a) - i sinc(a)) e^{-ia} / 2.
+// T(u) and dT/du.  Series tiers use the series (same lengths as the
+// waveform); tier NTIER uses T = sinc(a) e^{-ia}, a = u/2,
+// dT/du = (sinc'(a) - i sinc(a)) e^{-ia} / 2.
 template <int TIER>
 __device__ __forceinline__ void transfer_and_deriv(double u, double2 &T, double2 &dT) {
-    if constexpr (TIER < 3) {
+    if constexpr (TIER < NTIER) {
         constexpr int KR = TierF64<TIER>::KR, KI = TierF64<TIER>::KI;
         const double v = u * u;
         T = make_double2(horner_R<KR>(v), -u * horner_I<KI>(v));
@@ -190,7 +191,7 @@ __device__ __forceinline__ void make_leaves(const Src &s, const SkyDeriv &sd, co
     const double omL = f * fsi;
     lv.omL = omL;
     double sL, cL;
-    if constexpr (TIER < 3)
+    if constexpr (TIER < NTIER)
         sincos_series<TierF64<TIER>::KS>(omL, sL, cL);
     else
         sincos_f64(omL, sL, cL);
@@ -225,7 +226,7 @@ __device__ __forceinline__ void adjoint_sample(const Src &s, const SkyDeriv &sd,
         const double f = s.f0 + s.fdot * t + 0.5 * s.fddot * t * t;
         const double omL = f * fsi;
         double sL, cL;
-        if constexpr (TIER < 3)
+        if constexpr (TIER < NTIER)
             sincos_series<TierF64<TIER>::KS>(omL, sL, cL);
         else
             sincos_f64(omL, sL, cL);

@@ -473,7 +473,13 @@ static int make_batch_args(BatchArgs &ba, double *loglike, gbgpu_cmplx *d_h, gbg
         set_error("bad num_bin, T, armlength or precision");
         return GBGPU_E_ARG;
     }
-    if (cfg->strategy != GBGPU_BATCH_BUCKETS && cfg->strategy != GBGPU_BATCH_PERSISTENT) {
+    if (cfg->strategy == GBGPU_BATCH_PERSISTENT) {
+        // switched off: its kernel took a GPU memory fault in this build whose
+        // cause has not been shown; the per-bucket launches cover every batch
+        set_error("the persistent batch strategy is disabled in this build; use GBGPU_BATCH_BUCKETS");
+        return GBGPU_E_ARG;
+    }
+    if (cfg->strategy != GBGPU_BATCH_BUCKETS) {
         set_error("unknown batch strategy");
         return GBGPU_E_ARG;
     }

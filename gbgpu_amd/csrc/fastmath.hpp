@@ -3,7 +3,8 @@
 // All sine/cosine routines reduce x = y + q*pi/2 with y in [-pi/4, pi/4] and
 // evaluate Taylor series of sin and cos in y (coefficients 1/n!).  The first
 // omitted terms bound the truncation error:
-//   FP64: sin through y^17, cos through y^18 -> |y|^19/19! < 1e-19;
+//   FP64: sin through y^15, cos through y^16 -> |y|^17/17! < 4.7e-17 (below
+//         half an ulp of |sin y| >= 0.64 |y| and of cos y >= 0.7);
 //   FP32: sin through y^9,  cos through y^10 -> |y|^11/11! < 1.8e-9.
 // Reductions:
 //   sincos_f64(x): k = rint(2x/pi), y = x - k*P1 - k*P2 with two FMAs, where
@@ -14,6 +15,8 @@
 //     r - k/4 is exact in FP64, the remainder goes to the FP32 polynomial.
 #pragma once
 
+#include <type_traits>
+
 #include "common.hpp"
 
 namespace gbgpu {
@@ -21,8 +24,7 @@ namespace gbgpu {
 // ------------------------------------------------------------------ FP64 ---
 __device__ __forceinline__ void sincos_poly_f64(double y, int q, double &s, double &c) {
     const double y2 = y * y;
-    double ps = 2.8114572543455206e-15;        // 1/17!
-    ps = fma(ps, y2, -7.647163731819816e-13);  // -1/15!
+    double ps = -7.647163731819816e-13;        // -1/15!
     ps = fma(ps, y2, 1.6059043836821613e-10);  // 1/13!
     ps = fma(ps, y2, -2.505210838544172e-08);  // -1/11!
     ps = fma(ps, y2, 2.7557319223985893e-06);  // 1/9!
@@ -30,8 +32,7 @@ __device__ __forceinline__ void sincos_poly_f64(double y, int q, double &s, doub
     ps = fma(ps, y2, 8.333333333333333e-03);   // 1/5!
     ps = fma(ps, y2, -0.16666666666666666);    // -1/3!
     const double sy = fma(y * y2, ps, y);
-    double pc = 1.5619206968586225e-16;        // 1/18!
-    pc = fma(pc, y2, -4.779477332387385e-14);  // -1/16!
+    double pc = -4.779477332387385e-14;        // -1/16!
     pc = fma(pc, y2, 1.1470745597729725e-11);  // 1/14!
     pc = fma(pc, y2, -2.08767569878681e-09);   // -1/12!
     pc = fma(pc, y2, 2.755731922398589e-07);   // 1/10!
@@ -173,27 +174,45 @@ __device__ __forceinline__ void transfer_series(R u, R &re, R &im) {
 
 // Series lengths per frequency tier.  Tier t applies when
 // u_max = 2 max|f| / f* (light-time and Doppler margin included) is at most
-// U_MAX[t]; the first omitted term is then below 7e-17 (FP64) or 3e-9
-// (FP32) for the transfer factor (|u| <= U_MAX; FP64 real part
-// v^KR/(2KR+1)! with v = U_MAX^2: 2.8e-17, 6.5e-17, 1.8e-17), omL = f/f*
-// (<= U_MAX/2) and the mixed path's spacecraft phase offsets
-// (<= 0.29 U_MAX rad).  Tier 3 falls back to range-reduced sine/cosine.
-constexpr double TIER_UMAX[3] = {0.75, 1.5, 2.25};
-// FP64 phase offsets (|x| <= 0.29 U_MAX: 0.22, 0.44, 0.65): the first
-// omitted sine term x^(2K+1)/(2K+1)! is 3.7e-19, 2.7e-18, 1.8e-18.
+// U_MAX[t]; the lengths are the shortest whose first omitted term is below
+// 1e-16 (FP64) or 3e-9 (FP32), relative to a result of order one, for the
+// transfer factor (|u| <= U_MAX: real part v^KR/(2KR+1)!, imaginary part
+// u v^KI/(2KI+2)!, v = U_MAX^2), the sine/cosine of omL = f/f*
+// (<= U_MAX/2) and of the spacecraft phase offsets (<= 0.29 U_MAX rad).
+// Tier NTIER falls back to range-reduced sine/cosine.
+constexpr int NTIER = 4;
+constexpr double TIER_UMAX[NTIER] = {0.4, 0.75, 1.5, 2.25};
+
+constexpr int tier_pick(int t, int a, int b, int c, int d) { return t == 0 ? a : t == 1 ? b : t == 2 ? c : d; }
+
 template <int TIER>
 struct TierF64 {
-    static constexpr int KR = TIER == 0 ? 8 : TIER == 1 ? 10 : 12;
-    static constexpr int KI = TIER == 0 ? 8 : TIER == 1 ? 10 : 12;
-    static constexpr int KS = TIER == 0 ? 7 : TIER == 1 ? 9 : 10;  // sincos of omL
-    static constexpr int KD = TIER == 0 ? 6 : TIER == 1 ? 7 : 8;   // sincos of phase offsets
+    static constexpr int KR = tier_pick(TIER, 7, 8, 10, 12);
+    static constexpr int KI = tier_pick(TIER, 6, 8, 10, 12);
+    static constexpr int KS = tier_pick(TIER, 6, 7, 8, 9);  // sincos of omL
+    static constexpr int KD = tier_pick(TIER, 5, 6, 7, 8);  // sincos of phase offsets
 };
 template <int TIER>
 struct TierF32 {
-    static constexpr int KR = TIER == 0 ? 5 : TIER == 1 ? 7 : 8;
-    static constexpr int KI = TIER == 0 ? 5 : TIER == 1 ? 6 : 8;
-    static constexpr int KS = TIER == 0 ? 4 : TIER == 1 ? 5 : 6;  // sincos of omL
-    static constexpr int KD = TIER == 0 ? 4 : 5;                   // sincos of phase offsets
+    static constexpr int KR = tier_pick(TIER, 4, 5, 7, 8);
+    static constexpr int KI = tier_pick(TIER, 4, 5, 6, 8);
+    static constexpr int KS = tier_pick(TIER, 3, 4, 5, 6);  // sincos of omL
+    static constexpr int KD = tier_pick(TIER, 3, 4, 5, 5);  // sincos of phase offsets
 };
+
+// Calls f(std::integral_constant<int, TIER>) for a binary's tier (0 ..
+// NTIER - 1: series; NTIER: reduced sine/cosine).  One call per binary, not
+// per sample, so that each tier's loop is its own straight-line code.
+template <class F>
+__device__ __forceinline__ void with_tier(int tier, F &&f) {
+    static_assert(NTIER == 4, "with_tier lists every tier");
+    switch (tier) {
+        case 0: f(std::integral_constant<int, 0>{}); break;
+        case 1: f(std::integral_constant<int, 1>{}); break;
+        case 2: f(std::integral_constant<int, 2>{}); break;
+        case 3: f(std::integral_constant<int, 3>{}); break;
+        default: f(std::integral_constant<int, NTIER>{}); break;
+    }
+}
 
 }  // namespace gbgpu

@@ -11,9 +11,10 @@
 //         FP32 and amp is restored in FP64 after the transform.
 //
 // Each policy provides: CT (LDS complex type), Aux (extra per-binary
-// constants), make_aux(), sample<THIRD>() (time-domain X, Y, Z at sample n,
-// already scaled by fctr2 * 0.5 T / N), fac() (complex factor applied to
-// every output bin in FP64: TDI2 factor, and amp for Mixed) and twiddles().
+// constants), make_aux(), sample<THIRD, TIER>() (time-domain X, Y, Z at
+// sample n and their common factor F = fctr2 * 0.5 T / N, applied by the
+// caller once per output channel), fac() (complex factor applied to every
+// output bin in FP64: TDI2 factor, and amp for Mixed) and twiddles().
 #pragma once
 
 #include "common.hpp"
@@ -27,10 +28,12 @@ struct F64 {
     struct Aux {};
     static __device__ __forceinline__ Aux make_aux(const Src &) { return Aux{}; }
 
-    template <bool THIRD>
+    // TIER: the binary's series tier (Src::tier), chosen once per binary by
+    // the caller so that each tier's sample loop is its own straight-line code
+    template <bool THIRD, int TIER>
     static __device__ __forceinline__ void sample(const Src &s, const Aux &, const double *__restrict__ tab, int N,
                                                   int n, double fstar_inv, double scale, const float *v3,
-                                                  const double *t3, CT &X, CT &Y, CT &Z) {
+                                                  const double *t3, CT &X, CT &Y, CT &Z, CT &F) {
         const SampleGeom g = load_geom(s, tab, N, n);
         double v3d[3] = {0.0, 0.0, 0.0};
         if (THIRD) {
@@ -38,12 +41,7 @@ struct F64 {
             v3d[1] = v3[1];
             v3d[2] = v3[2];
         }
-        switch (s.tier) {
-            case 0: slow_xyz_f64<THIRD, 0>(s, g, fstar_inv, scale, v3d, t3, X, Y, Z); break;
-            case 1: slow_xyz_f64<THIRD, 1>(s, g, fstar_inv, scale, v3d, t3, X, Y, Z); break;
-            case 2: slow_xyz_f64<THIRD, 2>(s, g, fstar_inv, scale, v3d, t3, X, Y, Z); break;
-            default: slow_xyz_f64<THIRD, 3>(s, g, fstar_inv, scale, v3d, t3, X, Y, Z); break;
-        }
+        slow_xyz_f64<THIRD, TIER>(s, g, fstar_inv, scale, v3d, t3, X, Y, Z, F);
     }
 
     static __device__ __forceinline__ double2 fac(const Src &s) { return make_double2(s.tdr, s.tdi); }
@@ -78,20 +76,20 @@ __device__ __forceinline__ float2 transfer_general(float u) {
 template <bool THIRD, int TIER>
 __device__ __forceinline__ void slow_xyz_mixed(const Src &s, const SrcF &sf, const double *__restrict__ tab, int N,
                                                int n, double fstar_inv_d, float scale, const float *v3,
-                                               const double *term3, float2 &X, float2 &Y, float2 &Z) {
+                                               const double *term3, float2 &X, float2 &Y, float2 &Z,
+                                               float2 &F) {
     const float fstar_inv = (float)fstar_inv_d;
     constexpr double INV_C = 1.0 / C_SI;
     constexpr float INV_CF = (float)(1.0 / C_SI);
     constexpr float PIF = 3.14159265358979323846f;
-    const double t = tab[ROW_T * N + n];
+    const TabRd T(tab, N);
+    const double t = T(ROW_T, n);
     float r[9], dP[9];
 #pragma unroll
-    for (int i = 0; i < 9; i++) r[i] = (float)tab[(ROW_R + i) * N + n];
+    for (int i = 0; i < 9; i++) r[i] = (float)T(ROW_R + i, n);
 #pragma unroll
-    for (int i = 0; i < 9; i++) dP[i] = (float)tab[(ROW_DP + i) * N + n];
-    const double kPc =
-        (s.k[0] * tab[ROW_PC * N + n] + s.k[1] * tab[(ROW_PC + 1) * N + n] + s.k[2] * tab[(ROW_PC + 2) * N + n]) *
-        INV_C;
+    for (int i = 0; i < 9; i++) dP[i] = (float)T(ROW_DP + i, n);
+    const double kPc = (s.k[0] * T(ROW_PC, n) + s.k[1] * T(ROW_PC + 1, n) + s.k[2] * T(ROW_PC + 2, n)) * INV_C;
     float d[3];
 #pragma unroll
     for (int sc = 0; sc < 3; sc++)
@@ -112,7 +110,7 @@ __device__ __forceinline__ void slow_xyz_mixed(const Src &s, const SrcF &sf, con
             const float dd = d[sc];
             const float dphi = dd * (w1 - dd * (w2 - dd * w3));
             float sd, cd;
-            if constexpr (TIER < 3)
+            if constexpr (TIER < NTIER)
                 sincos_series<TierF32<TIER>::KD>(dphi, sd, cd);
             else
                 sincos_f32(dphi, sd, cd);
@@ -138,11 +136,11 @@ __device__ __forceinline__ void slow_xyz_mixed(const Src &s, const SrcF &sf, con
     const float kr13 = sf.k[0] * r[3] + sf.k[1] * r[4] + sf.k[2] * r[5];
     const float kr23 = sf.k[0] * r[6] + sf.k[1] * r[7] + sf.k[2] * r[8];
     const float ur12 = sf.u[0] * r[0] + sf.u[1] * r[1] + sf.u[2] * r[2];
-    const float vr12 = sf.v[0] * r[0] + sf.v[1] * r[1] + sf.v[2] * r[2];
+    const float vr12 = sf.v[0] * r[0] + sf.v[1] * r[1];  // v_z = 0
     const float ur13 = sf.u[0] * r[3] + sf.u[1] * r[4] + sf.u[2] * r[5];
-    const float vr13 = sf.v[0] * r[3] + sf.v[1] * r[4] + sf.v[2] * r[5];
+    const float vr13 = sf.v[0] * r[3] + sf.v[1] * r[4];
     const float ur23 = sf.u[0] * r[6] + sf.u[1] * r[7] + sf.u[2] * r[8];
-    const float vr23 = sf.v[0] * r[6] + sf.v[1] * r[7] + sf.v[2] * r[8];
+    const float vr23 = sf.v[0] * r[6] + sf.v[1] * r[7];
     const float ep12 = vr12 * vr12 - ur12 * ur12, ex12 = 2.0f * ur12 * vr12;
     const float ep23 = vr23 * vr23 - ur23 * ur23, ex23 = 2.0f * ur23 * vr23;
     const float ep31 = vr13 * vr13 - ur13 * ur13, ex31 = 2.0f * ur13 * vr13;
@@ -153,7 +151,7 @@ __device__ __forceinline__ void slow_xyz_mixed(const Src &s, const SrcF &sf, con
     const float2 E0 = make_float2(cphi[0], -sphi[0]), E1 = make_float2(cphi[1], -sphi[1]),
                  E2 = make_float2(cphi[2], -sphi[2]);
     auto Tf = [](float u) {
-        if constexpr (TIER < 3) {
+        if constexpr (TIER < NTIER) {
             float re, im;
             transfer_series<TierF32<TIER>::KR, TierF32<TIER>::KI>(u, re, im);
             return make_float2(re, im);
@@ -169,7 +167,7 @@ __device__ __forceinline__ void slow_xyz_mixed(const Src &s, const SrcF &sf, con
     const float2 G32 = cmul(Tf(fonfs[2] * (1.0f - kr23)), cmul(E2, A23));
     const float2 G13 = cmul(Tf(fonfs[0] * (1.0f + kr13)), cmul(E0, A31));
     float omL, sL, cL;
-    if constexpr (TIER < 3) {
+    if constexpr (TIER < NTIER) {
         const float tf = (float)t;
         const float f = sf.f0 + sf.fdot * tf + 0.5f * sf.fddot * tf * tf;
         omL = f * fstar_inv;
@@ -184,10 +182,10 @@ __device__ __forceinline__ void slow_xyz_mixed(const Src &s, const SrcF &sf, con
     }
     const float2 fctr = make_float2(cL, -sL);
     const float m = 4.0f * omL * sL * scale;
-    const float2 fctr2 = make_float2(m * cL, -m * sL);
-    X = cmul(fctr2, cadd(csub(G21, G31), cmul(csub(G12, G13), fctr)));
-    Y = cmul(fctr2, cadd(csub(G32, G12), cmul(csub(G23, G21), fctr)));
-    Z = cmul(fctr2, cadd(csub(G13, G23), cmul(csub(G31, G32), fctr)));
+    F = make_float2(m * cL, -m * sL);  // fctr2
+    X = cadd(csub(G21, G31), cmul(csub(G12, G13), fctr));
+    Y = cadd(csub(G32, G12), cmul(csub(G23, G21), fctr));
+    Z = cadd(csub(G13, G23), cmul(csub(G31, G32), fctr));
 }
 
 struct Mixed {
@@ -210,16 +208,11 @@ struct Mixed {
         return f;
     }
 
-    template <bool THIRD>
+    template <bool THIRD, int TIER>
     static __device__ __forceinline__ void sample(const Src &s, const Aux &sf, const double *__restrict__ tab, int N,
                                                   int n, double fstar_inv, double scale, const float *v3,
-                                                  const double *t3, CT &X, CT &Y, CT &Z) {
-        switch (s.tier) {
-            case 0: slow_xyz_mixed<THIRD, 0>(s, sf, tab, N, n, fstar_inv, (float)scale, v3, t3, X, Y, Z); break;
-            case 1: slow_xyz_mixed<THIRD, 1>(s, sf, tab, N, n, fstar_inv, (float)scale, v3, t3, X, Y, Z); break;
-            case 2: slow_xyz_mixed<THIRD, 2>(s, sf, tab, N, n, fstar_inv, (float)scale, v3, t3, X, Y, Z); break;
-            default: slow_xyz_mixed<THIRD, 3>(s, sf, tab, N, n, fstar_inv, (float)scale, v3, t3, X, Y, Z); break;
-        }
+                                                  const double *t3, CT &X, CT &Y, CT &Z, CT &F) {
+        slow_xyz_mixed<THIRD, TIER>(s, sf, tab, N, n, fstar_inv, (float)scale, v3, t3, X, Y, Z, F);
     }
 
     static __device__ __forceinline__ double2 fac(const Src &s) { return make_double2(s.tdr * s.amp, s.tdi * s.amp); }

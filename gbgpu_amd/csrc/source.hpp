@@ -20,7 +20,7 @@ struct Src {
     double A2, varpi, e2, n2, T2;                 // third body (thirdbody.py:39-80)
     double tdr, tdi;                              // TDI2 factor, (1, 0) for TDI1
     long long start;                              // rint(f0 T) - N/2 (gbgpu.py:296, :387, :462)
-    int tier;   // series tier (fastmath.hpp TIER_UMAX) from u_max = 2 max_t |f(t)| / f*; 3 = none
+    int tier;   // series tier (fastmath.hpp TIER_UMAX) from u_max = 2 max_t |f(t)| / f*; NTIER = none
 };
 
 __device__ __forceinline__ double fmax_(double a, double b) { return a > b ? a : b; }
@@ -90,7 +90,9 @@ __device__ inline Src make_src(const double *__restrict__ params, int num_params
         // 1e-3 margin covers the light-time offsets (|xi - t| < 600 s) and any
         // third-body Doppler factor (|v|/c < 1e-3)
         const double umax = 2.0 * fmax * 1.001 / fstar;
-        s.tier = umax <= TIER_UMAX[0] ? 0 : umax <= TIER_UMAX[1] ? 1 : umax <= TIER_UMAX[2] ? 2 : 3;
+        s.tier = NTIER;
+        for (int t = NTIER - 1; t >= 0; t--)
+            if (umax <= TIER_UMAX[t]) s.tier = t;
     }
     if (tdi2) {
         // 2i sin(2 wL) e^{-2 i wL}, wL = 2 pi f0 L / c   (gbgpu.py:301-304)
@@ -182,37 +184,71 @@ struct SampleGeom {
 
 // Orbit-table rows (see orbits.py: orbit_table): t, guiding centre Pc,
 // offsets dP_s = P_s - Pc, arm vectors r12, r13, r23.
-enum { ROW_T = 0, ROW_PC = 1, ROW_DP = 4, ROW_R = 13 };
+enum { ROW_T = 0, ROW_PC = 1, ROW_DP = 4, ROW_R = 13, TAB_ROWS = 22 };
+
+// Word 3 of a raw buffer resource: 32-bit data format, no swizzle.
+constexpr int BUF_DWORD3 = 0x00020000;
+
+// The same pointer value taken from the first active lane, in scalar
+// registers.  For a pointer that every lane of the wavefront holds (a kernel
+// argument, or a per-launch table chosen by a wave-uniform index) this is
+// the value itself; it lets the compiler prove the uniformity it cannot see
+// through a non-inlined call.
+__device__ __forceinline__ const double *wave_uniform(const double *p) {
+    const unsigned long long v = reinterpret_cast<unsigned long long>(p);
+    const unsigned lo = __builtin_amdgcn_readfirstlane((unsigned)v);
+    const unsigned hi = __builtin_amdgcn_readfirstlane((unsigned)(v >> 32));
+    return reinterpret_cast<const double *>(((unsigned long long)hi << 32) | lo);
+}
+
+// Reads of the (22, N) orbit table through a buffer resource: the row's
+// byte offset rides in a scalar register and the sample's in one vector
+// register, so no per-row address arithmetic runs on the vector ALU.  The
+// table pointer must be the same in every lane of the wavefront (it is a
+// per-launch table everywhere); the resource is built from scalar registers.
+// Reads past the table's extent return zero instead of faulting.
+struct TabRd {
+    __amdgpu_buffer_rsrc_t r;
+    int row_bytes;
+    __device__ __forceinline__ TabRd(const double *tab, int N)
+        : r(__builtin_amdgcn_make_buffer_rsrc(const_cast<double *>(wave_uniform(tab)), (short)0, TAB_ROWS * N * 8,
+                                              BUF_DWORD3)),
+          row_bytes(N * 8) {}
+    __device__ __forceinline__ double operator()(int row, int n) const {
+        return __builtin_bit_cast(double, __builtin_amdgcn_raw_buffer_load_b64(r, n * 8, row * row_bytes, 0));
+    }
+};
 
 // k . P_s / c for the three spacecraft, as (k.Pc + k.dP_s)/c.
 __device__ __forceinline__ void load_kP(const Src &s, const double *__restrict__ tab, int N, int n, double kP[3]) {
     constexpr double INV_C = 1.0 / C_SI;
-    const double kc =
-        s.k[0] * tab[ROW_PC * N + n] + s.k[1] * tab[(ROW_PC + 1) * N + n] + s.k[2] * tab[(ROW_PC + 2) * N + n];
+    const TabRd T(tab, N);
+    const double kc = s.k[0] * T(ROW_PC, n) + s.k[1] * T(ROW_PC + 1, n) + s.k[2] * T(ROW_PC + 2, n);
 #pragma unroll
     for (int sc = 0; sc < 3; sc++) {
         const int r0 = ROW_DP + 3 * sc;
-        const double kd = s.k[0] * tab[r0 * N + n] + s.k[1] * tab[(r0 + 1) * N + n] + s.k[2] * tab[(r0 + 2) * N + n];
+        const double kd = s.k[0] * T(r0, n) + s.k[1] * T(r0 + 1, n) + s.k[2] * T(r0 + 2, n);
         kP[sc] = (kc + kd) * INV_C;
     }
 }
 
+// v = (sin lam, -cos lam, 0) (gbgpu.py:256): its z component is exactly zero
+// and is left out of v.r.
 __device__ __forceinline__ SampleGeom load_geom(const Src &s, const double *__restrict__ tab, int N, int n) {
     SampleGeom g;
-    g.t = tab[ROW_T * N + n];
+    const TabRd T(tab, N);
+    g.t = T(ROW_T, n);
     double r[9];
 #pragma unroll
-    for (int i = 0; i < 9; i++) r[i] = tab[(ROW_R + i) * N + n];
+    for (int i = 0; i < 9; i++) r[i] = T(ROW_R + i, n);
     {
         constexpr double INV_C = 1.0 / C_SI;
-        const double kc =
-            s.k[0] * tab[ROW_PC * N + n] + s.k[1] * tab[(ROW_PC + 1) * N + n] + s.k[2] * tab[(ROW_PC + 2) * N + n];
+        const double kc = s.k[0] * T(ROW_PC, n) + s.k[1] * T(ROW_PC + 1, n) + s.k[2] * T(ROW_PC + 2, n);
         g.kPc = kc * INV_C;
 #pragma unroll
         for (int sc = 0; sc < 3; sc++) {
             const int r0 = ROW_DP + 3 * sc;
-            const double kd =
-                s.k[0] * tab[r0 * N + n] + s.k[1] * tab[(r0 + 1) * N + n] + s.k[2] * tab[(r0 + 2) * N + n];
+            const double kd = s.k[0] * T(r0, n) + s.k[1] * T(r0 + 1, n) + s.k[2] * T(r0 + 2, n);
             g.kP[sc] = (kc + kd) * INV_C;
             g.d[sc] = kd * INV_C;
         }
@@ -221,11 +257,11 @@ __device__ __forceinline__ SampleGeom load_geom(const Src &s, const double *__re
     g.kr13 = s.k[0] * r[3] + s.k[1] * r[4] + s.k[2] * r[5];
     g.kr23 = s.k[0] * r[6] + s.k[1] * r[7] + s.k[2] * r[8];
     g.ur12 = s.u[0] * r[0] + s.u[1] * r[1] + s.u[2] * r[2];
-    g.vr12 = s.v[0] * r[0] + s.v[1] * r[1] + s.v[2] * r[2];
+    g.vr12 = s.v[0] * r[0] + s.v[1] * r[1];
     g.ur13 = s.u[0] * r[3] + s.u[1] * r[4] + s.u[2] * r[5];
-    g.vr13 = s.v[0] * r[3] + s.v[1] * r[4] + s.v[2] * r[5];
+    g.vr13 = s.v[0] * r[3] + s.v[1] * r[4];
     g.ur23 = s.u[0] * r[6] + s.u[1] * r[7] + s.u[2] * r[8];
-    g.vr23 = s.v[0] * r[6] + s.v[1] * r[7] + s.v[2] * r[8];
+    g.vr23 = s.v[0] * r[6] + s.v[1] * r[7];
     return g;
 }
 
@@ -237,7 +273,7 @@ __device__ __forceinline__ SampleGeom load_geom(const Src &s, const double *__re
 template <int TIER>
 __device__ __forceinline__ double2 link_f64(double fonfs, double kr, double cphi, double sphi, double2 Aarm) {
     double tr, ti;
-    if constexpr (TIER < 3) {
+    if constexpr (TIER < NTIER) {
         transfer_series<TierF64<TIER>::KR, TierF64<TIER>::KI>(fonfs * (1.0 + kr), tr, ti);
     } else {
         const double a = 0.5 * fonfs * (1.0 + kr);
@@ -252,16 +288,18 @@ __device__ __forceinline__ double2 link_f64(double fonfs, double kr, double cphi
     return cmul(e, Aarm);
 }
 
-// Time-domain X, Y, Z slow part at sample n, already multiplied by
-// fctr2 * amp * 0.5 T / N (gbgpu.py:359-375, :298; amp / amp cancels).
+// Time-domain X, Y, Z slow part at sample n, before their common factor
+// F = fctr2 * amp * 0.5 T / N (gbgpu.py:359-375, :298; amp / amp cancels),
+// which is returned separately so that the caller applies it once per
+// output channel (A and E for the likelihood).
 // v3[sc] (third-body v_LOS at xi_sc) and term3[sc] (its phase integral) are
 // used only when THIRD.
 template <bool THIRD, int TIER>
 __device__ __forceinline__ void slow_xyz_f64(const Src &s, const SampleGeom &g, double fstar_inv, double scale,
                                              const double *v3, const double *term3, double2 &X, double2 &Y,
-                                             double2 &Z) {
+                                             double2 &Z, double2 &F) {
     double fonfs[3], cphi[3], sphi[3];
-    if constexpr (!THIRD && TIER < 3) {
+    if constexpr (!THIRD && TIER < NTIER) {
         // Guiding-centre split: with xi_c = t - k.Pc/c and d_s = k.dP_s/c,
         //   Phi_s = Phi_c + 2 pi f(xi_c) d_s - pi f'(xi_c) d_s^2 + (pi/3) fddot d_s^3,
         // exact for the cubic argS.  Phi_c takes the one range-reduced
@@ -318,16 +356,16 @@ __device__ __forceinline__ void slow_xyz_f64(const Src &s, const SampleGeom &g, 
     const double f = s.f0 + s.fdot * g.t + 0.5 * s.fddot * g.t * g.t;
     const double omL = f * fstar_inv;
     double sL, cL;
-    if constexpr (TIER < 3)
+    if constexpr (TIER < NTIER)
         sincos_series<TierF64<TIER>::KS>(omL, sL, cL);
     else
         sincos_f64(omL, sL, cL);
     const double2 fctr = make_double2(cL, -sL);
     const double m = omL * sL * scale;  // 4 omL sL scale, times the links' 1/4
-    const double2 fctr2 = make_double2(m * cL, -m * sL);
-    X = cmul(fctr2, cadd(csub(G21, G31), cmul(csub(G12, G13), fctr)));
-    Y = cmul(fctr2, cadd(csub(G32, G12), cmul(csub(G23, G21), fctr)));
-    Z = cmul(fctr2, cadd(csub(G13, G23), cmul(csub(G31, G32), fctr)));
+    F = make_double2(m * cL, -m * sL);  // fctr2
+    X = cadd(csub(G21, G31), cmul(csub(G12, G13), fctr));
+    Y = cadd(csub(G32, G12), cmul(csub(G23, G21), fctr));
+    Z = cadd(csub(G13, G23), cmul(csub(G31, G32), fctr));
 }
 
 }  // namespace gbgpu

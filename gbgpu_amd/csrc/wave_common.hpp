@@ -3,6 +3,8 @@
 // (third-body pre-pass -> slow part -> LDS transform).
 #pragma once
 
+#include <type_traits>
+
 #include "common.hpp"
 #include "fft_lds.hpp"
 #include "prec.hpp"
@@ -336,35 +338,43 @@ __device__ __forceinline__ void binary_spectrum(const Src &s, const typename P::
         }
     }
     using R = decltype(CT{}.x);
+    // The series tier is fixed per binary: one branch per tier around the
+    // whole sample loop gives each tier straight-line code with its own
+    // register allocation (a branch per sample keeps every tier's values
+    // live at once).
+    auto samples = [&](auto tier_c) {
+        constexpr int TIER = decltype(tier_c)::value;
 #pragma unroll
-    for (int q = 0; q < SPT; q++) {
-        // third body: the thread's own consecutive samples (pre-pass values in
-        // registers); otherwise strided, so orbit-table loads coalesce
-        const int n = THIRD ? SPT * lane + q : lane + q * TPB;
-        CT X{}, Y{}, Z{};
-        if (valid) {
-            float v3[3] = {0.f, 0.f, 0.f};
-            double t3[3] = {0.0, 0.0, 0.0};
-            if constexpr (THIRD) {
+        for (int q = 0; q < SPT; q++) {
+            // third body: the thread's own consecutive samples (pre-pass values in
+            // registers); otherwise strided, so orbit-table loads coalesce
+            const int n = THIRD ? SPT * lane + q : lane + q * TPB;
+            CT X{}, Y{}, Z{}, F{};
+            if (valid) {
+                float v3[3] = {0.f, 0.f, 0.f};
+                double t3[3] = {0.0, 0.0, 0.0};
+                if constexpr (THIRD) {
 #pragma unroll
-                for (int sc = 0; sc < 3; sc++) {
-                    v3[sc] = vf[sc][q];
-                    t3[sc] = gc[sc][q];
+                    for (int sc = 0; sc < 3; sc++) {
+                        v3[sc] = vf[sc][q];
+                        t3[sc] = gc[sc][q];
+                    }
                 }
+                P::template sample<THIRD, TIER>(s, aux, tab, N, n, fstar_inv, scale, v3, t3, X, Y, Z, F);
             }
-            P::template sample<THIRD>(s, aux, tab, N, n, fstar_inv, scale, v3, t3, X, Y, Z);
+            if constexpr (NCH == 3) {
+                buf[n] = cmul(F, X);
+                buf[N + n] = cmul(F, Y);
+                buf[2 * N + n] = cmul(F, Z);
+            } else {
+                // A = (Z - X)/sqrt2, E = (X - 2Y + Z)/sqrt6 (utility.py:34-38); linear,
+                // so it commutes with the transform, and X, Y, Z share the factor F.
+                buf[n] = cmul(cscale(F, (R)INV_SQRT2), csub(Z, X));
+                buf[N + n] = cmul(cscale(F, (R)INV_SQRT6), cadd(csub(X, cscale(Y, (R)2)), Z));
+            }
         }
-        if constexpr (NCH == 3) {
-            buf[n] = X;
-            buf[N + n] = Y;
-            buf[2 * N + n] = Z;
-        } else {
-            // A = (Z - X)/sqrt2, E = (X - 2Y + Z)/sqrt6 (utility.py:34-38); linear,
-            // so it commutes with the transform.
-            buf[n] = cscale(csub(Z, X), (R)INV_SQRT2);
-            buf[N + n] = cscale(cadd(csub(X, cscale(Y, (R)2)), Z), (R)INV_SQRT6);
-        }
-    }
+    };
+    with_tier(s.tier, samples);
     __syncthreads();
     fft_lds<N, NCH, TPB>(buf, lane, tw);
 }

@@ -518,6 +518,10 @@ def test_swap_third_body_proposal():
 
 
 # --------------------------------------------------------- uneven N batch ---
+PERSISTENT_OFF = ("the persistent batch strategy is switched off: its kernel took a GPU memory fault "
+                  "whose cause has not been shown")
+
+
 def _uneven_batch(B, seed):
     from gbgpu_amd.utils.utility import baseline_N
 
@@ -528,8 +532,8 @@ def _uneven_batch(B, seed):
 
 @pytest.mark.parametrize("precision", ["fp64", "mixed"])
 def test_uneven_batch_matches_per_N_calls(precision):
-    """The uneven-N batch (both strategies) == one standalone call per N
-    (same arithmetic per binary) and == the oracle within the precision's bar."""
+    """The uneven-N batch == one standalone call per N (same arithmetic per
+    binary) and == the oracle within the precision's bar."""
     from gbgpu_amd.batch import BatchedLogLike
 
     B = 300
@@ -539,7 +543,7 @@ def test_uneven_batch_matches_per_N_calls(precision):
     gb = _gb(precision=precision)
     gb.d_d = float(4 * df * (np.sum(np.abs(A) ** 2 / SA) + np.sum(np.abs(E) ** 2 / SE)))
     ll = gb.get_ll(p, [A, E], [SA, SE], N=N, T=T, dt=DT)
-    bl = BatchedLogLike(gb, [A, E], [SA, SE], T, DT, precision=precision, strategy="persistent", guard_tol=None)
+    bl = BatchedLogLike(gb, [A, E], [SA, SE], T, DT, precision=precision, strategy="buckets", guard_tol=None)
     assert np.array_equal(bl(p, N=N).cpu().numpy(), ll)
     ref = np.empty(B)
     for n in np.unique(N):
@@ -600,6 +604,23 @@ def test_perturb_params_matches_numpy_restatement():
     assert np.array_equal(Nout.cpu().numpy(), levels[np.searchsorted(edges, np.abs(got[1]), side="right")])
 
 
+@pytest.mark.skip(reason=PERSISTENT_OFF)
+def test_uneven_batch_persistent_matches_per_N_calls():
+    """The persistent launch gives the per-N calls' results bit for bit."""
+    from gbgpu_amd.batch import BatchedLogLike
+
+    B = 300
+    p, N, rng = _uneven_batch(B, 600)
+    T = 4 * YR
+    A, E, SA, SE, df, L = make_data(p[:, :20], 512, T, DT, rng)
+    gb = _gb(precision="fp64")
+    gb.d_d = float(4 * df * (np.sum(np.abs(A) ** 2 / SA) + np.sum(np.abs(E) ** 2 / SE)))
+    ll = gb.get_ll(p, [A, E], [SA, SE], N=N, T=T, dt=DT)
+    bl = BatchedLogLike(gb, [A, E], [SA, SE], T, DT, precision="fp64", strategy="persistent", guard_tol=None)
+    assert np.array_equal(bl(p, N=N).cpu().numpy(), ll)
+
+
+@pytest.mark.skip(reason=PERSISTENT_OFF)
 def test_uneven_batch_strategies_agree_at_scale():
     """60k binaries: the per-bucket launches run several grid-stride passes.
     The two strategies give bit-identical results where they share the tile
@@ -698,7 +719,7 @@ def test_batch_autotune_shapes_agree(precision):
     assert np.max(np.abs(tuned - ref) / scale) <= 1e-12
 
 
-@pytest.mark.parametrize("strategy", ["buckets", "persistent"])
+@pytest.mark.parametrize("strategy", ["buckets", pytest.param("persistent", marks=pytest.mark.skip(reason=PERSISTENT_OFF))])
 def test_graph_captured_loop_replays_match_eager(strategy):
     """Captured loop of jump + batched likelihood: every replay equals the eager loop."""
     import torch

@@ -106,13 +106,29 @@ def test_host_tensors_are_refused_as_device_pointers():
     assert _capi.ptr(None) is None
 
 
-def test_guard_with_persistent_strategy_refused():
-    """The mixed-precision accuracy guard exists only for per-bucket launches;
-    asking for it with the persistent launch is an error, not a no-op."""
+def test_persistent_strategy_refused():
+    """The persistent batch strategy is switched off (its kernel faulted with
+    a cause not yet shown); asking for it is an error, with or without the
+    mixed-precision guard (which only the per-bucket launches ever had)."""
     from gbgpu_amd.batch import BatchedLogLike
 
     gb = GBGPU()
     gb.d_d = 0.0
     z = [np.zeros(10, complex)] * 2
-    with pytest.raises(ValueError, match="buckets"):
-        BatchedLogLike(gb, z, [np.ones(10)] * 2, YR, 15.0, precision="mixed", strategy="persistent", guard_tol=1e-3)
+    for guard in (1e-3, None):
+        with pytest.raises(ValueError, match="buckets"):
+            BatchedLogLike(gb, z, [np.ones(10)] * 2, YR, 15.0, precision="mixed", strategy="persistent",
+                           guard_tol=guard)
+
+
+def test_persistent_strategy_refused_by_the_library():
+    """The C ABI refuses it too, before any GPU work."""
+    from gbgpu_amd import _capi
+
+    lib = _capi.load()
+    P = 8
+    bcfg = _capi.BatchCfg(T=1e7, armlength=2.5e9, tdi2=0, precision=0, strategy=_capi.GBGPU_BATCH_PERSISTENT)
+    dat = _capi.Data(A=P, E=P, psdA=P, psdE=P, data_index=None, noise_index=None, data_length=10, start_freq_ind=0,
+                     df=1e-7)
+    assert lib.gbgpu_wave_ll_batched(P, None, None, P, 9, 4, P, bcfg, dat, 0.0, 0, None, None) == -1
+    assert b"persistent" in lib.gbgpu_last_error()
