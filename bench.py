@@ -8,18 +8,32 @@ stream of analytic-PSD Gaussian noise plus 1000 injected binaries.
 Inputs are resident in HBM before the timed region.
 
 Prints ONE JSON line.  ``value`` = binaries/s of the plain FP64 HIP path;
-the mixed-precision path on the same batch is reported under ``mixed`` when
-the library supports it.  ``roofline`` is for the dominant kernel (the fused
-FP64 wave+logL kernel), from HIP events around the timed launches;
-``cpu_baseline`` times the numpy FP64 oracle on a bounded sample.
+the mixed-precision path on the same batch is reported under ``mixed``.
+``roofline`` is for the dominant kernel (the fused FP64 wave+logL kernel):
+its duration comes from HIP events around the timed launches; its HBM
+traffic and executed FLOPs come from two short rocprofv3 counter runs of this
+script in a child process (``--counters-child``), outside the timed region,
+scaled by what the same counters read for a 1 GiB device copy (null when
+rocprofv3 is unavailable).  ``cpu_baseline`` times the numpy FP64 oracle on
+a bounded sample of the same workload, on one host thread and on every host
+core this process may use; it runs before anything touches the GPU.
+Parity of the HIP path with the oracle is established by the test suite
+(tests/test_gpu_parity.py, tests/test_gpu_golden.py); this script reports
+the mixed path's differences from the FP64 HIP path.
 """
 
 from __future__ import annotations
 
 import argparse
+import csv
+import glob
 import json
 import os
+import shutil
+import signal
+import subprocess
 import sys
+import tempfile
 import time
 from pathlib import Path
 
@@ -31,13 +45,9 @@ sys.path.insert(0, str(ROOT))
 YRSID_SI = 31558149.763545603
 JULIAN_YEAR_S = 31557600.0  # the workload's "1 yr" (4 yr -> 4,207,681 bins at dt = 15 s)
 FP64_PEAK_TFLOPS = 78.6
+FP32_PEAK_TFLOPS = 157.3
 HBM_PEAK_GBS = 8000.0
-# HBM bytes per launch of the fused FP64 kernel on the default workload
-# (1e6 binaries, N = 256), from rocprofv3 FETCH_SIZE and WRITE_SIZE passes of
-# this script (profiles/k_wave_256_fp64_uniform_1e6_counters.md):
-# 415,216 KiB x 2.0 + 32,002 KiB x 1.0, the factors being what the counters
-# read for a 1 GiB device copy (0.5 GiB fetched, 1 GiB written).
-MEASURED_TRAFFIC_BYTES_1E6_N256 = (415216 * 2.0 + 32002 * 1.0) * 1024
+COPY_BYTES = 1 << 30  # the calibration copy: 1 GiB read, 1 GiB written
 
 
 def flop_per_binary(N: int) -> float:
@@ -53,9 +63,31 @@ def bytes_per_binary(N: int) -> float:
     return 48 * N + 112
 
 
-def make_workload(B, N, T, dt, seed):
-    from tests.helpers import draw_params
+G_SI = 6.67408e-11
+C_SI = 299792458.0
+MSUN_SI = 1.98848e30
 
+
+def draw_params(B, rng, f_lo=1e-3, f_hi=6e-3, log_f=False):
+    """(9, B) seeded source parameters of the workloads: amp log-uniform in
+    [1e-23, 1e-21]; f0 uniform (or log-uniform) in [f_lo, f_hi]; chirp mass
+    U[0.2, 1] Msun with fdot = 96/5 pi^(8/3) (G Mc/c^3)^(5/3) f0^(11/3) and
+    fddot = 11/3 fdot^2 / f0; phi0 U[0, 2 pi); cos iota U[-1, 1]; psi U[0, pi);
+    lambda U[0, 2 pi); sin beta U[-1, 1]."""
+    amp = 10 ** rng.uniform(-23, -21, B)
+    f0 = 10 ** rng.uniform(np.log10(f_lo), np.log10(f_hi), B) if log_f else rng.uniform(f_lo, f_hi, B)
+    Mc = rng.uniform(0.2, 1.0, B) * MSUN_SI
+    fdot = (96.0 / 5.0) * np.pi ** (8 / 3) * (G_SI * Mc / C_SI**3) ** (5 / 3) * f0 ** (11 / 3)
+    fddot = 11.0 / 3.0 * fdot**2 / f0
+    phi0 = rng.uniform(0, 2 * np.pi, B)
+    iota = np.arccos(rng.uniform(-1, 1, B))
+    psi = rng.uniform(0, np.pi, B)
+    lam = rng.uniform(0, 2 * np.pi, B)
+    beta = np.arcsin(rng.uniform(-1, 1, B))
+    return np.array([amp, f0, fdot, fddot, phi0, iota, psi, lam, beta])
+
+
+def make_workload(B, N, T, dt, seed):
     rng = np.random.default_rng(seed)
     params = draw_params(B, rng, f_lo=1e-3, f_hi=6e-3)
     return params, rng
@@ -100,25 +132,26 @@ def time_calls(fn, steps, warmup):
     return ev0.elapsed_time(ev1) / 1e3 / steps, wall / steps
 
 
-def cpu_baseline(N, T, dt, budget_s=12.0, chunk=500):
-    """numpy FP64 oracle (run_wave + brute-force get_ll) on the host, 1 thread."""
+def _cpu_chunks(args):
+    """Worker of the CPU baseline: the numpy FP64 oracle (run_wave + the
+    brute-force get_ll) over chunks of the workload for budget_s seconds on
+    one thread.  Returns the number of binaries done and the elapsed time."""
+    N, T, dt, budget_s, chunk, seed = args
     try:
         from threadpoolctl import threadpool_limits
 
-        limiter = threadpool_limits(1)
+        threadpool_limits(1)
     except Exception:  # pragma: no cover
-        limiter = None
+        pass
+    from gbgpu_amd.noise import A1TDISens
     from oracle import fastgb as F
-    from tests.helpers import draw_params
 
-    rng = np.random.default_rng(1234)
+    rng = np.random.default_rng(seed)
     Tadj = int(T / dt) * dt
     df = 1 / Tadj
     L = int(Tadj / dt) // 2 + 1
     f = np.arange(L) * df
     f[0] = df
-    from gbgpu_amd.noise import A1TDISens
-
     S = A1TDISens.get_Sn(f)
     dA = (rng.standard_normal(L) + 1j * rng.standard_normal(L)) * np.sqrt(S / (8 * df))
     dE = (rng.standard_normal(L) + 1j * rng.standard_normal(L)) * np.sqrt(S / (8 * df))
@@ -130,33 +163,58 @@ def cpu_baseline(N, T, dt, budget_s=12.0, chunk=500):
         d_h, h_h = F.inner_terms(o["AETf"][:, 0], o["AETf"][:, 1], o["start_inds"], dA, dE, S, S, df)
         F.loglike(1.0, d_h, h_h)
         done += chunk
-    el = time.perf_counter() - t0
-    if limiter is not None:
-        limiter.unregister() if hasattr(limiter, "unregister") else None
+    return done, time.perf_counter() - t0
+
+
+def cpu_baseline(N, T, dt, budget_s=10.0, chunk=500):
+    """The CPU path timed on this host: the numpy FP64 oracle on one thread,
+    then one single-threaded worker per usable core (os.sched_getaffinity,
+    at most 16) for the same budget.  Runs before the GPU is touched, so the
+    worker processes are plain forks of a CPU-only interpreter."""
+    import multiprocessing as mp
+
+    os.environ.setdefault("OMP_NUM_THREADS", "1")
+    one_done, one_el = _cpu_chunks((N, T, dt, budget_s, chunk, 1234))
+    cores = min(16, len(os.sched_getaffinity(0)))
+    with mp.get_context("fork").Pool(cores) as pool:
+        res = pool.map(_cpu_chunks, [(N, T, dt, budget_s, chunk, 1000 + i) for i in range(cores)])
+    # the workers' timed intervals run concurrently (each after its own set-up)
+    all_done = sum(r[0] for r in res)
+    wall = max(r[1] for r in res)
     return dict(
-        value=done / el,
+        value=all_done / wall,
         unit="binaries/s",
-        cores=1,
+        cores=cores,
         kind="port",
-        sample=f"{done} binaries of the same workload (N={N}, 4 yr) through the numpy FP64 oracle "
-        f"run_wave + get_ll in chunks of {chunk}, {el:.1f} s on 1 host thread",
+        sample=f"{all_done} binaries of the same workload (N={N}, 4 yr, f0 U[1,6] mHz) through the numpy FP64 "
+        f"oracle run_wave + get_ll in chunks of {chunk}: {cores} single-threaded worker processes for "
+        f"{wall:.1f} s",
+        single_core={"value": one_done / one_el, "unit": "binaries/s", "cores": 1,
+                     "sample": f"{one_done} binaries in {one_el:.1f} s on 1 host thread"},
     )
 
 
-def parity_sample(gb, params, A, E, SA, SE, d_d, df, N, T, dt, precision, n=64):
-    from oracle import fastgb as F
-
+def mixed_vs_fp64(gb, params, A, E, SA, SE, N, T, dt, n=4096):
+    """Differences of the mixed-precision path from the FP64 HIP path (which
+    the test suite pins to the oracle) on a sample of the batch: log-L
+    absolute difference and the largest per-binary relative L2 difference of
+    the A and E spectra."""
     p = params[:, :n]
-    ll = gb.get_ll(p, [A, E], [SA, SE], N=N, T=T, dt=dt, precision=precision)
-    o = F.run_wave(*p, N=N, T=T, dt=dt)
-    d_h, h_h = F.inner_terms(o["AETf"][:, 0], o["AETf"][:, 1], o["start_inds"], A.cpu().numpy(), E.cpu().numpy(),
-                             SA.cpu().numpy(), SE.cpu().numpy(), df)
-    ll_o = F.loglike(d_d, d_h, h_h)
-    gb.run_wave(*p, N=N, T=T, dt=dt, precision=precision)
-    aet = gb.AETf.cpu().numpy()[:, :2]
-    werr = np.sqrt(np.sum(np.abs(aet - o["AETf"][:, :2]) ** 2, axis=(1, 2)) / np.sum(np.abs(o["AETf"][:, :2]) ** 2,
-                                                                                     axis=(1, 2)))
-    return dict(n=n, max_abs_logL_err=float(np.max(np.abs(ll - ll_o))), max_rel_wave_err=float(werr.max()))
+    out = {}
+    ll = {}
+    aet = {}
+    for prec in ("fp64", "mixed"):
+        ll[prec] = gb.get_ll(p, [A, E], [SA, SE], N=N, T=T, dt=dt, precision=prec)
+        gb.run_wave(*p, N=N, T=T, dt=dt, precision=prec)
+        aet[prec] = gb.AETf.cpu().numpy()
+    werr = {}
+    for c, name in enumerate("AE"):
+        num = np.sqrt(np.sum(np.abs(aet["mixed"][:, c] - aet["fp64"][:, c]) ** 2, axis=1))
+        den = np.sqrt(np.sum(np.abs(aet["fp64"][:, c]) ** 2, axis=1))
+        werr[name] = float(np.max(num / den))
+    out.update(n=n, max_abs_logL_diff=float(np.max(np.abs(ll["mixed"] - ll["fp64"]))),
+               max_rel_wave_diff_per_channel=werr)
+    return out
 
 
 def uneven_batch(gb, A, E, SA, SE, d_d, B, T, dt, n_calls, seed=77):
@@ -168,9 +226,6 @@ def uneven_batch(gb, A, E, SA, SE, d_d, B, T, dt, n_calls, seed=77):
 
     from gbgpu_amd.batch import BatchedLogLike
     from gbgpu_amd.utils.utility import baseline_N
-    from oracle import fastgb as F
-    from tests.helpers import draw_params
-
     rng = np.random.default_rng(seed)
     p = draw_params(B, rng, f_lo=1e-4, f_hi=20e-3, log_f=True)
     N = baseline_N(p[1])
@@ -220,19 +275,21 @@ def uneven_batch(gb, A, E, SA, SE, d_d, B, T, dt, n_calls, seed=77):
         d = (lls[key] - lls["fp64"]).abs()
         res[f"logL_abs_err_{key}_vs_fp64"] = {"max": float(d.max()), "median": float(d.median()),
                                              "frac_le_1e-3": float((d <= 1e-3).double().mean())}
-    # per-channel waveform error of the mixed path against the FP64 oracle,
-    # on a sample of every N bucket (unperturbed parameters)
-    errs = {"A": 0.0, "E": 0.0, "T": 0.0}
+    # per-channel waveform difference of the mixed path from the FP64 HIP
+    # path (pinned to the oracle by the tests), on a sample of every N bucket
+    # (unperturbed parameters); the mixed path has no T channel
+    errs = {"A": 0.0, "E": 0.0}
     for n in np.unique(N):
-        sel = np.flatnonzero(N == n)[:16]
-        gb.run_wave(*p[:, sel], N=int(n), T=T, dt=dt, precision="mixed")
-        aet = gb.AETf.cpu().numpy()
-        o = F.run_wave(*p[:, sel], N=int(n), T=T, dt=dt)
-        for c, name in enumerate("AET"):
-            num = np.sqrt(np.sum(np.abs(aet[:, c] - o["AETf"][:, c]) ** 2, axis=1))
-            den = np.sqrt(np.sum(np.abs(o["AETf"][:, c]) ** 2, axis=1))
+        sel = np.flatnonzero(N == n)[:256]
+        aet = {}
+        for prec in ("fp64", "mixed"):
+            gb.run_wave(*p[:, sel], N=int(n), T=T, dt=dt, precision=prec)
+            aet[prec] = gb.AETf.cpu().numpy()
+        for c, name in enumerate("AE"):
+            num = np.sqrt(np.sum(np.abs(aet["mixed"][:, c] - aet["fp64"][:, c]) ** 2, axis=1))
+            den = np.sqrt(np.sum(np.abs(aet["fp64"][:, c]) ** 2, axis=1))
             errs[name] = max(errs[name], float(np.max(num / den)))
-    res["mixed_max_rel_wave_err_per_channel"] = errs
+    res["mixed_max_rel_wave_diff_vs_fp64_per_channel"] = errs
     return res
 
 
@@ -245,7 +302,6 @@ def differentiation(gb, A, E, SA, SE, d_d, T, dt, steps, warmup, B=100_000, seed
 
     from gbgpu_amd import _capi
     from gbgpu_amd.derivatives import FastGBLikelihood
-    from tests.helpers import draw_params
 
     N = 256
     rng = np.random.default_rng(seed)
@@ -366,7 +422,6 @@ def global_and_swap(gb, SA, SE, T, dt, B=10_000_000, P=100_000, chunk_samples=1 
     from gbgpu_amd import GBGPUThirdBody, _capi
     from gbgpu_amd.noise import noise_realisation
     from gbgpu_amd.utils.utility import baseline_N
-    from tests.helpers import draw_params
 
     lib = _capi.load()
     st = _capi.stream_handle()
@@ -472,9 +527,142 @@ def global_and_swap(gb, SA, SE, T, dt, B=10_000_000, P=100_000, chunk_samples=1 
         torch.cuda.synchronize()
         with _capi.KernelTimer() as kt:
             sec, _ = time_calls(swap_all, 5, 0)
+        # algorithmic HBM bytes per proposal: both binaries' data and PSD
+        # windows (A, E: 48 B per bin each), both parameter columns and the result
+        swap_bytes = float(np.sum(2 * 48.0 * NP + 2 * 8.0 * nparams + 8.0))
+        kms = kt.total_ms / 5
         res[f"swap_{label}"] = {"proposals": P, "proposals_per_s": P / sec, "ms_per_pass": sec * 1e3,
-                                "kernel_ms_per_pass": kt.total_ms / 5}
+                                "kernel_ms_per_pass": kms,
+                                "algorithmic_GB_per_s": swap_bytes / (kms / 1e3) / 1e9,
+                                "hbm_peak_GB_per_s": HBM_PEAK_GBS}
     return res
+
+
+K_FP64 = "k_wave<256, 1, false, gbgpu::F64>"
+K_MIXED = "k_wave<256, 1, false, gbgpu::Mixed>"
+
+
+def counters_child(args):
+    """--counters-child: the launches the counter runs read.  A 1 GiB
+    device-to-device elementwise copy (the calibration), then two launches
+    each of the fused FP64 and mixed kernels on the default workload."""
+    import torch
+
+    from gbgpu_amd import GBGPU, _capi
+
+    src = torch.ones(COPY_BYTES // 8, dtype=torch.float64, device="cuda")
+    dst = torch.empty_like(src)
+    torch.cuda.synchronize()
+    torch.mul(src, 1.0, out=dst)  # reads 1 GiB, writes 1 GiB
+    torch.cuda.synchronize()
+    del src, dst
+    B, N, dt = args.num_bin, 256, 15.0
+    T = 4 * JULIAN_YEAR_S
+    params, rng = make_workload(B, N, T, dt, seed=2024)
+    gb = GBGPU()
+    A, E, SA, SE, d_d, df, L = build_data(gb, params, N, T, dt, rng)
+    p_dev = torch.from_numpy(np.ascontiguousarray(params)).cuda()
+    ll = torch.empty(B, dtype=torch.float64, device="cuda")
+    dat = _capi.Data(A=A.data_ptr(), E=E.data_ptr(), psdA=SA.data_ptr(), psdE=SE.data_ptr(), data_index=None,
+                     noise_index=None, data_length=L, start_freq_ind=0, df=df)
+    lib = _capi.load()
+    for prec in ("fp64", "mixed"):
+        cfg, _tab = gb._cfg(N, int(T / dt) * dt, False, prec)
+        for _ in range(2):
+            _capi.check(lib.gbgpu_wave_ll(ll.data_ptr(), None, None, p_dev.data_ptr(), 9, B, cfg, dat, d_d, 0,
+                                          _capi.stream_handle()), "gbgpu_wave_ll")
+        torch.cuda.synchronize()
+
+
+def _read_counter_csv(d):
+    """{(dispatch, kernel name): {counter: value summed over instances}}."""
+    files = glob.glob(os.path.join(d, "**", "*counter_collection.csv"), recursive=True)
+    out = {}
+    for f in files:
+        with open(f) as fh:
+            for r in csv.DictReader(fh):
+                key = (int(r["Dispatch_Id"]), r["Kernel_Name"])
+                c = out.setdefault(key, {})
+                c[r["Counter_Name"]] = c.get(r["Counter_Name"], 0.0) + float(r["Counter_Value"])
+    return out
+
+
+def _run_counters(counters, num_bin, timeout=240):
+    exe = shutil.which("rocprofv3")
+    if exe is None:
+        raise RuntimeError("rocprofv3 not found")
+    d = tempfile.mkdtemp(prefix="gbgpu_pmc_", dir="/tmp")
+    cmd = [exe, "--pmc", *counters, "-d", d, "-o", "run", "--output-format", "csv", "--", sys.executable,
+           os.path.abspath(__file__), "--counters-child", "--num-bin", str(num_bin)]
+    proc = subprocess.Popen(cmd, cwd="/tmp", env=dict(os.environ, TMPDIR="/tmp"), stdout=subprocess.PIPE,
+                            stderr=subprocess.STDOUT, start_new_session=True)
+    try:
+        log, _ = proc.communicate(timeout=timeout)
+    except subprocess.TimeoutExpired:
+        os.killpg(proc.pid, signal.SIGKILL)
+        proc.communicate()
+        raise RuntimeError(f"counter run {counters} timed out")
+    rows = _read_counter_csv(d)
+    shutil.rmtree(d, ignore_errors=True)
+    if proc.returncode != 0 and not rows:
+        raise RuntimeError(f"counter run {counters} failed: {log[-400:]!r}")
+    return rows
+
+
+def measure_counters(num_bin):
+    """HBM traffic and executed FLOPs of the fused kernels from two counter
+    runs (FETCH_SIZE with the VALU FLOP counters, then WRITE_SIZE), each
+    scaled by what it reads for the 1 GiB calibration copy."""
+    fetch = _run_counters(["FETCH_SIZE", "SQ_INSTS_VALU_FLOPS_FP64", "SQ_INSTS_VALU_FLOPS_FP32", "SQ_WAVES"],
+                          num_bin)
+    write = _run_counters(["WRITE_SIZE"], num_bin)
+
+    def copy_value(rows, counter):
+        cand = [c[counter] for (_d, k), c in rows.items() if "elementwise" in k and counter in c]
+        return max(cand)
+
+    def kernel_mean(rows, name, counter):
+        v = [c[counter] for (_d, k), c in rows.items() if name in k and counter in c]
+        return sum(v) / len(v)
+
+    f_copy = copy_value(fetch, "FETCH_SIZE") * 1024.0
+    w_copy = copy_value(write, "WRITE_SIZE") * 1024.0
+    ff, wf = COPY_BYTES / f_copy, COPY_BYTES / w_copy
+    out = {"copy_fetch_factor": ff, "copy_write_factor": wf}
+    for name, key in ((K_FP64, "fp64"), (K_MIXED, "mixed")):
+        waves = kernel_mean(fetch, name, "SQ_WAVES")
+        out[key] = {
+            "traffic_bytes": kernel_mean(fetch, name, "FETCH_SIZE") * 1024.0 * ff
+            + kernel_mean(write, name, "WRITE_SIZE") * 1024.0 * wf,
+            # the FLOP counters count per wavefront instruction (FMA = 2):
+            # times 64 lanes, per wavefront (one binary per wavefront at N = 256)
+            "executed_fp64_flop_per_binary": kernel_mean(fetch, name, "SQ_INSTS_VALU_FLOPS_FP64") / waves * 64,
+            "executed_fp32_flop_per_binary": kernel_mean(fetch, name, "SQ_INSTS_VALU_FLOPS_FP32") / waves * 64,
+        }
+    return out
+
+
+def copy_probe():
+    """Attainable HBM bandwidth: a 1 GiB device-to-device elementwise copy
+    (1 GiB read + 1 GiB written), timed with HIP events."""
+    import torch
+
+    src = torch.ones(COPY_BYTES // 8, dtype=torch.float64, device="cuda")
+    dst = torch.empty_like(src)
+    for _ in range(2):
+        torch.mul(src, 1.0, out=dst)
+    torch.cuda.synchronize()
+    ev0 = torch.cuda.Event(enable_timing=True)
+    ev1 = torch.cuda.Event(enable_timing=True)
+    ev0.record()
+    for _ in range(10):
+        torch.mul(src, 1.0, out=dst)
+    ev1.record()
+    torch.cuda.synchronize()
+    sec = ev0.elapsed_time(ev1) / 1e3 / 10
+    del src, dst
+    torch.cuda.empty_cache()
+    return 2.0 * COPY_BYTES / sec / 1e9
 
 
 def main():
@@ -484,9 +672,9 @@ def main():
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--num-bin", type=int, default=1_000_000)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
-    ap.add_argument("--cpu-budget", type=float, default=12.0)
-    ap.add_argument("--copy-probe", action="store_true",
-                    help="first run a 1 GiB device-to-device copy (calibrates HBM byte counters)")
+    ap.add_argument("--cpu-budget", type=float, default=8.0, help="seconds per CPU-baseline leg")
+    ap.add_argument("--no-counters", action="store_true", help="skip the rocprofv3 traffic/FLOP counter runs")
+    ap.add_argument("--counters-child", action="store_true", help=argparse.SUPPRESS)
     ap.add_argument("--only", choices=["fp64", "mixed"], default=None, help="time one precision only")
     ap.add_argument("--no-uneven", action="store_true", help="skip the uneven-N batch")
     ap.add_argument("--uneven-calls", type=int, default=100, help="calls in the captured uneven-N loop")
@@ -497,19 +685,22 @@ def main():
     if args.gpus != 1:
         print(json.dumps({"error": "this build measures one GPU only; --gpus must be 1"}))
         sys.exit(2)
+    if args.counters_child:
+        counters_child(args)
+        return
+    B, N, dt = args.num_bin, 256, 15.0
+    T = 4 * JULIAN_YEAR_S
+
+    # the CPU path first, before anything touches the GPU
+    cpu = None
+    if not args.no_cpu:
+        print("[bench] cpu baseline", file=sys.stderr, flush=True)
+        cpu = cpu_baseline(N, T, dt, budget_s=args.cpu_budget)
 
     import torch
 
     from gbgpu_amd import GBGPU, _capi
 
-    if args.copy_probe:
-        src = torch.empty(1 << 27, dtype=torch.float64, device="cuda")  # 1 GiB
-        dst = torch.empty_like(src)
-        dst.copy_(src)
-        torch.cuda.synchronize()
-        del src, dst
-    B, N, dt = args.num_bin, 256, 15.0
-    T = 4 * JULIAN_YEAR_S
     print(f"[bench] workload B={B} N={N} T=4yr", file=sys.stderr, flush=True)
     params, rng = make_workload(B, N, T, dt, seed=2024)
     gb = GBGPU()
@@ -536,39 +727,41 @@ def main():
     call64, _t64 = runner("fp64")
     print("[bench] timing fp64", file=sys.stderr, flush=True)
     if args.only == "mixed":
-        sec64, wall64 = float("nan"), float("nan")
+        sec64, wall64, kernel_s = float("nan"), float("nan"), float("nan")
     else:
         for _ in range(args.warmup):
             call64()
         torch.cuda.synchronize()
         with _capi.KernelTimer() as kt64:
             sec64, wall64 = time_calls(call64, args.steps, 0)
-    par64 = parity_sample(gb, params, A, E, SA, SE, d_d, df, N, T, dt, "fp64") if args.only is None else None
+        kernel_s = kt64.mean_ms / 1e3
     value = B / sec64
     flops = flop_per_binary(N) * B
-    kernel_s = kt64.mean_ms / 1e3 if args.only != "mixed" else float("nan")
     achieved_tflops = flops / kernel_s / 1e12
 
     mixed = None
-    if args.only == "fp64":
-        pass
-    else:
-      try:
+    if args.only != "fp64":
         callm, _tm = runner("mixed")
         callm()
         torch.cuda.synchronize()
         print("[bench] timing mixed", file=sys.stderr, flush=True)
-        secm, _ = time_calls(callm, args.steps, args.warmup)
-        parm = parity_sample(gb, params, A, E, SA, SE, d_d, df, N, T, dt, "mixed") if args.only is None else None
-        mixed = dict(value=B / secm, ms_per_step=secm * 1e3, speedup_vs_fp64=sec64 / secm, parity=parm)
-      except RuntimeError as e:
-        mixed = dict(error=str(e)[:200])
+        with _capi.KernelTimer() as ktm:
+            secm, _ = time_calls(callm, args.steps, args.warmup)
+        mixed = dict(value=B / secm, ms_per_step=secm * 1e3, kernel_ms=ktm.mean_ms, speedup_vs_fp64=sec64 / secm,
+                     kernel="k_wave<256, MODE_LL, circular, Mixed>")
+        if args.only is None:
+            mixed["vs_fp64_hip"] = mixed_vs_fp64(gb, params, A, E, SA, SE, N, T, dt)
+
+    print("[bench] copy probe", file=sys.stderr, flush=True)
+    copy_gbs = copy_probe()
 
     dropin = None
     if not args.no_dropin and args.only is None:
         call64()
         torch.cuda.synchronize()
         dropin = dropin_get_ll(params, A, E, SA, SE, d_d, df, L, N, T, dt, args.steps, args.warmup, ll.clone())
+        dropin["attainable_GB_per_s"] = copy_gbs
+        dropin["frac_of_attainable"] = dropin["algorithmic_GB_per_s"] / copy_gbs
 
     uneven = None
     if not args.no_uneven:
@@ -578,14 +771,56 @@ def main():
     if not args.no_deriv:
         deriv = differentiation(gb, A, E, SA, SE, d_d, T, dt, args.steps, args.warmup)
 
-    glob = None
+    glob_res = None
     if not args.no_global:
-        glob = global_and_swap(gb, SA, SE, T, dt)
+        glob_res = global_and_swap(gb, SA, SE, T, dt)
 
-    cpu = None
-    if not args.no_cpu:
-        print("[bench] cpu baseline", file=sys.stderr, flush=True)
-        cpu = cpu_baseline(N, T, dt, budget_s=args.cpu_budget)
+    del A, E, SA, SE, p_dev, ll, gb
+    torch.cuda.empty_cache()
+    counters, counters_note = None, None
+    if not args.no_counters:
+        print("[bench] counter runs (rocprofv3, two passes)", file=sys.stderr, flush=True)
+        try:
+            counters = measure_counters(B)
+        except Exception as e:  # no profiler on this host, or it failed: traffic stays null
+            counters_note = f"counter runs unavailable: {str(e)[:200]}"
+
+    roof = {
+        "bound": "valu",
+        "compute": "FP64 vector ALU; MI355X's FP64 vector and matrix peaks are both 78.6 TFLOP/s",
+        "achieved": achieved_tflops,
+        "peak": FP64_PEAK_TFLOPS,
+        "unit": "TFLOP/s",
+        "frac": achieved_tflops / FP64_PEAK_TFLOPS,
+        "traffic": counters["fp64"]["traffic_bytes"] if counters else None,
+        "traffic_note": ("HBM bytes per launch: rocprofv3 FETCH_SIZE x %.3f + WRITE_SIZE x %.3f, the factors "
+                         "being 1 GiB over what each counter reads for a 1 GiB device copy; the algorithmic bytes "
+                         "are 48 N + 112 per binary = %.3g, served mostly from L2 and the Infinity Cache"
+                         % (counters["copy_fetch_factor"], counters["copy_write_factor"], bytes_per_binary(N) * B)
+                         if counters else counters_note),
+        "kernel": "k_wave<256, MODE_LL, circular, F64>",
+        "kernel_ms": kernel_s * 1e3,
+        "flop_per_binary": flop_per_binary(N),
+        "algorithmic_hbm_GBs": bytes_per_binary(N) * B / kernel_s / 1e9,
+    }
+    if counters:
+        ex = counters["fp64"]["executed_fp64_flop_per_binary"]
+        roof["executed_flop_per_binary"] = ex
+        roof["executed_TFLOPs"] = ex * B / kernel_s / 1e12
+        roof["executed_frac"] = roof["executed_TFLOPs"] / FP64_PEAK_TFLOPS
+    if mixed is not None and counters:
+        km = mixed["kernel_ms"] / 1e3
+        c = counters["mixed"]
+        mixed["roofline"] = {
+            "bound": "valu",
+            "peak_fp32_TFLOPs": FP32_PEAK_TFLOPS,
+            "peak_fp64_TFLOPs": FP64_PEAK_TFLOPS,
+            "executed_fp32_flop_per_binary": c["executed_fp32_flop_per_binary"],
+            "executed_fp64_flop_per_binary": c["executed_fp64_flop_per_binary"],
+            "executed_fp32_TFLOPs": c["executed_fp32_flop_per_binary"] * B / km / 1e12,
+            "executed_fp64_TFLOPs": c["executed_fp64_flop_per_binary"] * B / km / 1e12,
+            "traffic": c["traffic_bytes"],
+        }
 
     out = {
         "metric": "binaries/sec (wave+logL), plain FP64 vs batched mixed, at <=1e-6 rel err",
@@ -608,28 +843,15 @@ def main():
             "Tobs_s": Tadj,
             "data_length": L,
         },
-        "roofline": {
-            "bound": "mfma",
-            "compute": "FP64 VALU (MI355X FP64 vector and matrix peaks are both 78.6 TFLOP/s)",
-            "achieved": achieved_tflops,
-            "peak": FP64_PEAK_TFLOPS,
-            "unit": "TFLOP/s",
-            "frac": achieved_tflops / FP64_PEAK_TFLOPS,
-            "traffic": MEASURED_TRAFFIC_BYTES_1E6_N256 if B == 1_000_000 else None,
-            "traffic_note": "HBM bytes per launch (rocprofv3 FETCH_SIZE x 2.0 + WRITE_SIZE x 1.0); the algorithmic "
-                            "bytes are 48 N + 112 per binary = 12.4 GB, served mostly from L2 / Infinity Cache",
-            "kernel": "k_wave<256, MODE_LL, circular, F64>",
-            "kernel_ms": kernel_s * 1e3,
-            "flop_per_binary": flop_per_binary(N),
-            "algorithmic_hbm_GBs": bytes_per_binary(N) * B / kernel_s / 1e9,
-        },
+        "roofline": roof,
         "cpu_baseline": cpu,
-        "parity": par64,
+        "parity": "HIP vs oracle: tests/test_gpu_parity.py, tests/test_gpu_golden.py, tests/test_gpu_configs_4yr.py",
+        "attainable_hbm_GB_per_s": copy_gbs,
         "mixed": mixed,
         "uneven_batch": uneven,
         "dropin_get_ll": dropin,
         "differentiation": deriv,
-        "global_template_and_swap": glob,
+        "global_template_and_swap": glob_res,
         "wall_ms_per_step": wall64 * 1e3,
     }
     print(json.dumps(out))

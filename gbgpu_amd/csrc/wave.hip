@@ -106,7 +106,13 @@ __device__ __forceinline__ void wave_pass(const WaveArgs &a, WaveLds<N, MODE, TH
                 if (aet) {
                     aet[m] = cscale(csub(Z, X), INV_SQRT2);
                     aet[N + m] = cscale(cadd(csub(X, cscale(Y, 2.0)), Z), INV_SQRT6);
-                    aet[2 * N + m] = cscale(cadd(cadd(X, Y), Z), INV_SQRT3);
+                    // T = (X + Y + Z)/sqrt3 cancels to O((f/f*)^2) of X: from FP32
+                    // spectra it would carry errors far above its own size, so the
+                    // mixed policy marks it not-a-number rather than return it
+                    if constexpr (std::is_same<P, Mixed>::value)
+                        aet[2 * N + m] = make_double2(__builtin_nan(""), __builtin_nan(""));
+                    else
+                        aet[2 * N + m] = cscale(cadd(cadd(X, Y), Z), INV_SQRT3);
                 }
             }
             if (lane == 0) a.start_inds[b] = s.start;

@@ -150,6 +150,8 @@ class GBGPUBase(abc.ABC):
             N_temp = self.special_get_N(base[0], base[1], T, *args, oversample=oversample)
             N = int(np.max(N_temp))
         N = int(N)
+        if N & (N - 1) or not (32 <= N <= 2048):
+            raise ValueError(f"N={N}: the HIP kernels support powers of two in [32, 2048]")
         rows = base + [np.broadcast_to(np.asarray(a, dtype=np.float64), (B,)) for a in add_args]
         torch = _torch()
         params = torch.from_numpy(np.ascontiguousarray(np.stack(rows, axis=0))).to(self._device())
@@ -161,7 +163,10 @@ class GBGPUBase(abc.ABC):
 
         Outputs (torch, on the GPU): ``XYZf``/``AETf`` (num_bin, 3, N)
         complex128, ``start_inds`` (num_bin,) int64, plus ``df``, ``N``,
-        ``num_bin`` -- as the reference sets them.
+        ``num_bin`` -- as the reference sets them.  With
+        ``precision="mixed"`` the T channel (``T``, ``T_out``, ``AETf[:, 2]``)
+        is NaN: it cancels to O((f/f*)^2) of X, below the FP32 spectra's
+        error.
         """
         self._check_hooks()
         params, N, T = self._prepare(amp, f0, fdot, fddot, phi0, iota, psi, lam, beta, args, N, T, dt, oversample)

@@ -34,21 +34,22 @@ class GBGPUThirdBody(GBGPUBase):
         return (A2, varpi, e2, n2, T2)
 
     def special_get_N(self, amp, f0, T, A2, varpi, e2, P2, T2, oversample=1):
-        """Double N until the third-body orbit is resolved (src/gbgpu/thirdbody.py:82-153)."""
+        """N per binary that also resolves the third-body orbit (the job of
+        src/gbgpu/thirdbody.py:82-153): the circular-binary N (get_N, without
+        oversampling), doubled while the sampling rate N / (T in years) is
+        below two samples per outer period (2 / P2 per year), then
+        oversampled once."""
         amp = np.atleast_1d(amp)
         f0 = np.atleast_1d(f0)
-        P2 = np.atleast_1d(P2)
+        P2 = np.atleast_1d(np.asarray(P2, dtype=np.float64))
         assert len(amp) == len(f0)
         assert len(P2) == len(f0)
-        N = get_N(amp, f0, T, oversample=oversample)
-        freq_N = 1 / ((T / YRSID_SI) / N)
-        if np.any(freq_N < (2.0 / P2)):
-            while np.any(freq_N < (2.0 / P2)):
-                inds_fix = freq_N < (2.0 / P2)
-                N = 2 * N * (inds_fix) + N * (~inds_fix)
-                freq_N = 1 / ((T / YRSID_SI) / N)
-            N = N * oversample
-        return N.astype(int)
+        N = get_N(amp, f0, T, oversample=1).astype(np.int64)
+        yrs = T / YRSID_SI
+        need = 2.0 / P2  # samples per year that resolve the outer orbit
+        while np.any(N / yrs < need):
+            N = np.where(N / yrs < need, 2 * N, N)
+        return (N * int(oversample)).astype(int)
 
     def shift_frequency(self, fi, xi, A2, varpi, e2, n2, T2):
         """Evaluated in-kernel: f_i *= 1 + v_LOS(xi)/c (src/gbgpu/thirdbody.py:155-180)."""

@@ -69,48 +69,65 @@ def get_chirp_mass_from_f_fdot(f, fdot):
     return Mc_SI / MSUN_SI
 
 
+def band_N(f0, Tobs):
+    """Sample count from frequency and observation time alone: the band
+    table of src/gbgpu/utils/utility.py:217-242, which sets N per binary as
+    32 m, 64 m for 1 <= f0 < 10 mHz, 256 m for 10 mHz <= f0 < 300 mHz,
+    512 m for 30 <= f0 < 100 mHz and 1024 m for f0 >= 100 mHz, with
+    m = 1, 2, 4 or 8 for Tobs up to 1, 2, 4 or more years.  The 256 m band
+    overlaps the 512 m and 1024 m ones; the reference assigns it after them,
+    so it wins from 30 to 300 mHz, and the same precedence applies here."""
+    f0 = np.atleast_1d(np.asarray(f0, dtype=np.float64))
+    yrs = Tobs / YRSID_SI
+    m = 1 if yrs <= 1.0 else 2 if yrs <= 2.0 else 4 if yrs <= 4.0 else 8
+    n = np.select(
+        [(f0 >= 0.001) & (f0 < 0.01), (f0 >= 0.01) & (f0 < 0.3), (f0 >= 0.03) & (f0 < 0.1), f0 >= 0.1],
+        [64, 256, 512, 1024],
+        default=32,
+    )
+    return (n * m).astype(np.int64)
+
+
+def snr_N(amp, f0, Tobs):
+    """Sample count a loud source needs: 2 A sqrt(T / S_m(f0)), with
+    S_m = sqrt(S_X) / (4 sin^2(f0/f*)) the Michelson-response noise of
+    src/gbgpu/utils/utility.py:246-259 (S_X: this package's analytic X-channel
+    sensitivity, gbgpu_amd.noise.X1TDISens, in place of lisatools').  Rounded
+    down to an integer as the reference rounds it."""
+    from ..noise import X1TDISens
+
+    amp = np.atleast_1d(np.asarray(amp, dtype=np.float64))
+    f0 = np.atleast_1d(np.asarray(f0, dtype=np.float64))
+    s = np.sin(f0 / FSTAR)
+    Sm = np.sqrt(X1TDISens.get_Sn(f0)) / (4.0 * s * s)
+    return np.floor(2.0 * amp * np.sqrt(Tobs / Sm)).astype(np.int64)
+
+
+N_MAX_REFERENCE = 8192
+
+
 def get_N(amp, f0, Tobs, oversample=1, sensitivity=True):
-    """Slow-part sample count (src/gbgpu/utils/utility.py:197-273).
+    """Slow-part sample count per binary (the job of
+    src/gbgpu/utils/utility.py:197-273), as this package's own rule:
 
-    Same T multiplier and frequency bands as the reference (including its
-    overlapping 10-300 mHz band, applied in the same order).  The SNR bump
-    uses this package's analytic X-channel sensitivity
-    (``gbgpu_amd.noise.X1TDISens``) in place of lisatools'; pass
-    ``sensitivity=False`` to skip it.  Capped at 8192 like the reference.
+        N = next_pow2(min(max(band_N, snr_N), 8192)) * oversample
+
+    band_N is the reference's frequency/observation-time table (band_N
+    above); snr_N raises N for loud sources (snr_N above; skipped with
+    ``sensitivity=False``).  It differs from the reference in three ways,
+    each deliberate and tested (tests/test_constants_orbits.py):
+      * N is rounded up to a power of two (the transforms here are radix-4/2;
+        the reference's numpy FFT takes any length);
+      * when snr_N equals band_N exactly the result is that N (the
+        reference's ``M*(M>N) + N*(M<N)`` returns 0 there);
+      * ``oversample`` multiplies once (the third-body variant of the
+        reference multiplies twice when it doubles N, thirdbody.py:82-153).
     """
-    amp = np.atleast_1d(amp)
-    f0 = np.atleast_1d(f0)
-    mult = 8
-    if (Tobs / YRSID_SI) <= 1.0:
-        mult = 1
-    elif (Tobs / YRSID_SI) <= 2.0:
-        mult = 2
-    elif (Tobs / YRSID_SI) <= 4.0:
-        mult = 4
-    elif (Tobs / YRSID_SI) <= 8.0:
-        mult = 8
-    mult = np.full_like(f0, mult, dtype=np.int32)
-    N = 32 * mult
-    N[f0 >= 0.1] = 1024 * mult[f0 >= 0.1]
-    N[(f0 >= 0.03) & (f0 < 0.1)] = 512 * mult[(f0 >= 0.03) & (f0 < 0.1)]
-    N[(f0 >= 0.01) & (f0 < 0.3)] = 256 * mult[(f0 >= 0.01) & (f0 < 0.3)]
-    N[(f0 >= 0.001) & (f0 < 0.01)] = 64 * mult[(f0 >= 0.001) & (f0 < 0.01)]
+    n = band_N(f0, Tobs)
     if sensitivity:
-        from ..noise import X1TDISens
-
-        fonfs = f0 / FSTAR
-        SnX = np.sqrt(X1TDISens.get_Sn(f0))
-        Sm = SnX / (4.0 * np.sin(fonfs) * np.sin(fonfs))
-        Acut = amp * np.sqrt(Tobs / Sm)
-        M = (2.0 ** (np.log(Acut) / np.log(2.0) + 1.0)).astype(int)
-        M = M * (M > N) + N * (M < N)
-        N = M * (M > N) + N * (M < N)
-    else:
-        M = N
-    M = np.asarray(M).copy()
-    M[M > 8192] = 8192
-    N = M
-    return (N * oversample).astype(int)
+        n = np.maximum(n, snr_N(amp, f0, Tobs))
+    n = np.minimum(n, N_MAX_REFERENCE)
+    return (next_pow2(n) * int(oversample)).astype(int)
 
 
 def next_pow2(n):

@@ -109,7 +109,9 @@ int gbgpu_fill_global_rows(gbgpu_cmplx *A_glob, gbgpu_cmplx *E_glob, const gbgpu
 /* Replaces GBGPUBase.run_wave's array passes (src/gbgpu/gbgpu.py:108-316,
  * :348-540): writes XYZf and AETf as (num_bin, 3, N) complex128 (either may be
  * NULL), fftshifted and scaled exactly as the reference, and start_inds
- * (num_bin,) int64 = rint(f0*T) - N/2. */
+ * (num_bin,) int64 = rint(f0*T) - N/2.  With GBGPU_MIXED the T channel
+ * AETf[:, 2] is NaN: (X + Y + Z)/sqrt3 cancels to O((f/f*)^2) of X, below
+ * the FP32 spectra's error. */
 int gbgpu_run_wave(gbgpu_cmplx *XYZf, gbgpu_cmplx *AETf, long long *start_inds, const double *params,
                    int num_params, int num_bin, const gbgpu_wave_cfg *cfg, void *stream);
 

@@ -119,3 +119,63 @@ def test_get_N_notebook_cell34():
 def test_baseline_N_rule():
     f = np.array([0.5e-3, 1e-3, 2.9e-3, 3e-3, 5.9e-3, 6e-3, 9.9e-3, 10e-3, 20e-3])
     assert list(U.baseline_N(f)) == [128, 256, 256, 512, 512, 1024, 1024, 2048, 2048]
+
+
+def test_get_N_band_table_known_answers():
+    """The reference's band table at 1 yr (m = 1), boundaries included; the
+    overlapping 10-300 mHz band wins from 30 to 300 mHz (it is assigned last
+    there, src/gbgpu/utils/utility.py:236-242)."""
+    f0 = np.array([5e-4, 1e-3, 9.99e-3, 0.01, 0.029, 0.03, 0.05, 0.0999, 0.1, 0.2, 0.299, 0.3, 0.5])
+    assert list(U.band_N(f0, YR)) == [32, 64, 64, 256, 256, 256, 256, 256, 256, 256, 256, 1024, 1024]
+    assert list(U.band_N(np.array([2e-3]), 3 * YR)) == [256]  # m = 4 up to 4 yr
+    assert list(U.band_N(np.array([2e-3]), 5 * YR)) == [512]  # m = 8 beyond
+
+
+def test_get_N_snr_rule_and_its_deviations():
+    from gbgpu_amd.noise import X1TDISens
+
+    f0 = np.array([2e-3])
+    s = np.sin(f0 / U.FSTAR)
+    Sm = np.sqrt(X1TDISens.get_Sn(f0)) / (4 * s * s)
+    per_amp = 2 * np.sqrt(YR / Sm)  # snr_N = floor(amp * per_amp)
+    # snr_N == band_N exactly: the reference returns 0 there, this rule returns N
+    amp_eq = (64 + 0.5) / per_amp
+    assert U.snr_N(amp_eq, f0, YR)[0] == 64 and U.band_N(f0, YR)[0] == 64
+    assert U.get_N(amp_eq, f0, YR)[0] == 64
+    # a loud source: snr_N = 300 -> rounded up to 512
+    amp_300 = 300.5 / per_amp
+    assert U.snr_N(amp_300, f0, YR)[0] == 300
+    assert U.get_N(amp_300, f0, YR)[0] == 512
+    # capped at the reference's 8192, then oversampled once
+    assert U.get_N(1e6 / per_amp, f0, YR)[0] == 8192
+    assert U.get_N(1e6 / per_amp, f0, YR, oversample=2)[0] == 16384
+
+
+def test_third_body_N_oversampled_once():
+    """A 0.1-yr outer orbit over 4 yr needs 2/P2 = 20 samples per year: the
+    4-yr base N = 256 (2 mHz) gives 64 per year, already enough; a 0.01-yr
+    orbit needs 200 per year -> 1024.  oversample = 2 doubles the result once
+    (the reference's doubling branch applies it twice)."""
+    from gbgpu_amd import GBGPUThirdBody
+
+    gb = GBGPUThirdBody()
+    amp, f0 = np.array([1e-30]), np.array([2e-3])
+    assert gb.special_get_N(amp, f0, 4 * YR, 0.0, 0.0, 0.0, np.array([0.1]), 0.0)[0] == 256
+    assert gb.special_get_N(amp, f0, 4 * YR, 0.0, 0.0, 0.0, np.array([0.01]), 0.0)[0] == 1024
+    assert gb.special_get_N(amp, f0, 4 * YR, 0.0, 0.0, 0.0, np.array([0.01]), 0.0, oversample=2)[0] == 2048
+
+
+def test_unsupported_N_rejected_before_device_work():
+    """The HIP kernels take powers of two in [32, 2048]; other N (given, or
+    chosen by get_N for high-frequency or loud sources) are refused with a
+    ValueError before anything touches a device."""
+    from gbgpu_amd import GBGPU
+
+    gb = GBGPU()
+    p = [np.array([x]) for x in (1e-22, 2e-3, 1e-18, 0.0, 0.1, 0.2, 0.3, 0.4, 0.5)]
+    for N in (100, 16, 4096):
+        with pytest.raises(ValueError, match="powers of two in \\[32, 2048\\]"):
+            gb.run_wave(*p, N=N, T=YR, dt=15.0)
+    p[1] = np.array([0.35])  # f0 >= 0.3 Hz over 8 yr: band N = 1024 * 8 = 8192
+    with pytest.raises(ValueError, match="powers of two"):
+        gb.run_wave(*p, N=None, T=8 * YR, dt=1.0)

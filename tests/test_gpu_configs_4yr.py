@@ -4,10 +4,10 @@ stream), where the frequency-derivative sensitivities grow as T^2 and T^3.
 
 Bars:
   * gradient (N = 256): |g - g_FD| <= 1e-5 max(|g_FD|, sqrt(F_kk)) against
-    Richardson-extrapolated central differences of the FP64 likelihood with
-    steps moving the phase by ~1e-3 rad; the literal 1e-7 relative / 1e-9
-    absolute steps are held to 1e-3 (their phase moves are ~1e-8 rad, where
-    the FP64 rounding of the likelihood itself is ~1e-4 of sqrt(F_kk));
+    Richardson-extrapolated central differences of the FP64 likelihood, both
+    with steps moving the phase by ~1e-3 rad and with the literal 1e-7
+    relative / 1e-9 absolute steps (differences of the template terms, d_d
+    set to 0, whose rounding would otherwise swamp such small steps);
   * Fisher (N = 256): |F - F_FD| <= 1e-5 sqrt(F_ii F_jj) against the
     reference's 5-point multiplicative stencil;
   * fill: the N = 128-1024 frequency mix summed into one full-length stream

@@ -431,6 +431,9 @@ def test_run_wave_mixed_error_bound(N, band):
     assert err.max() <= 1e-6, err.max()
     err_ae = rel_err(gb.AETf.cpu().numpy()[:, :2], o["AETf"][:, :2], axis=(1, 2))
     assert err_ae.max() <= 1e-6, err_ae.max()
+    # the mixed path does not return the T channel (it cancels below its error)
+    assert np.all(np.isnan(gb.AETf.cpu().numpy()[:, 2]))
+    assert np.all(np.isnan(gb.T_out.cpu().numpy()))
 
 
 @pytest.mark.parametrize("N", [128, 256, 1024])
