@@ -40,9 +40,11 @@ class BatchedLogLike:
             switched off in this build and raises ValueError.
         guard_tol: mixed precision with the "buckets" strategy: binaries
             whose log-likelihood error bound 2.5e-6 (<h|h> + |<d|h>|) exceeds
-            guard_tol are re-evaluated in FP64 within the same call (None or
-            0 disables; loud templates otherwise carry an absolute error
-            that grows with SNR^2).
+            guard_tol are evaluated in FP64 within the same call -- routed
+            there at set-up when an estimate of <h|h> already exceeds the
+            bound, else re-evaluated after the mixed pass (None or 0
+            disables; loud templates otherwise carry an absolute error that
+            grows with SNR^2).
     """
 
     STRATEGIES = {"buckets": _capi.GBGPU_BATCH_BUCKETS, "persistent": _capi.GBGPU_BATCH_PERSISTENT}
@@ -139,8 +141,9 @@ class BatchedLogLike:
                     "gbgpu_batch_set_variants")
 
     def guard_counts(self):
-        """Binaries per N (dict) the accuracy guard re-evaluated in FP64 in the
-        last call (synchronises the current stream)."""
+        """Binaries per N (dict) the accuracy guard evaluated in FP64 in the
+        last call -- routed there at set-up by the <h|h> estimate, or
+        re-evaluated after the mixed pass (synchronises the current stream)."""
         counts = np.zeros(7, dtype=np.int32)
         if self._last_ws is None:
             return {}

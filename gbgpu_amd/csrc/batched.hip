@@ -6,7 +6,9 @@
 // (a power of two in [32, 2048]) and one call does, on the device:
 //   k_batch_reset    clear the control block (a fill kernel, no memset)
 //   k_batch_setup    per-binary constants (Pack) and a histogram over
-//                    (N bucket, log-spaced start-bin cell)
+//                    (N bucket, route, log-spaced start-bin cell); with the
+//                    mixed accuracy guard, binaries whose estimated <h|h>
+//                    already fails it are routed to an FP64 segment
 //   k_batch_scan     offsets of every histogram cell (one workgroup), and
 //                    per-bucket counts and list offsets
 //   k_batch_scatter  binaries into one list ordered by bucket, then by
@@ -36,10 +38,14 @@ void set_data(WaveArgs &a, const gbgpu_data *data);
 int launch_ll_segment(int N, int precision, int pack_precision, int variant, const WaveArgs &a, hipStream_t st);
 
 constexpr int NBUCKET = 7;  // N = 32 << k, k < 7
-constexpr int NCB = 4096;   // start-bin cells per bucket (log-spaced)
-constexpr int NCELL = NBUCKET * NCB;
-enum { CTL_COUNT = 0, CTL_LISTOFF = 16, CTL_ITEMOFF = 24, CTL_ITEMS = 32, CTL_TOTAL = 40, CTL_WORK = 41,
-       CTL_REDO = 48, CTL_HIST = 64, CTL_CUR = CTL_HIST + NCELL, CTL_WORDS = CTL_CUR + NCELL };
+// List segments: segment k holds bucket k evaluated in the batch's
+// precision, segment NBUCKET + k the binaries of bucket k routed to FP64.
+constexpr int NSEG = 2 * NBUCKET;
+constexpr int NCB = 4096;  // start-bin cells per segment (log-spaced)
+constexpr int NCELL = NSEG * NCB;
+enum { CTL_COUNT = 0, CTL_LISTOFF = 16, CTL_ITEMOFF = 32, CTL_ITEMS = 40, CTL_TOTAL = 48, CTL_WORK = 49,
+       CTL_REDO = 56, CTL_HIST = 64, CTL_CUR = CTL_HIST + NCELL, CTL_WORDS = CTL_CUR + NCELL };
+static_assert(CTL_LISTOFF - CTL_COUNT >= NSEG && CTL_ITEMOFF - CTL_LISTOFF >= NSEG, "control block layout");
 
 __host__ __device__ constexpr int bucket_bpw(int k) {
     return WG_SIZE / (((32 << k) / 4 < WG_SIZE) ? (32 << k) / 4 : WG_SIZE);
@@ -60,6 +66,7 @@ struct BatchArgs {
     int *redo;  // accuracy-guard list (same per-bucket offsets as `list`)
     int *ctl;
     double guard_tol;
+    double route_tol;  // > 0: route binaries whose estimated error bound exceeds it to FP64
 };
 
 __global__ void k_batch_reset(int *ctl) {
@@ -67,7 +74,21 @@ __global__ void k_batch_reset(int *ctl) {
     if (i < CTL_WORDS) ctl[i] = 0;
 }
 
-// Histogram cell of a binary: bucket k, then its start bin within the data
+// Estimated <h|h> of a binary from its amplitude, frequency and the A-channel
+// PSD at its centre bin: 1.1 A^2 T (4 wL sin wL)^2 / S_A(f0), wL = f0/f*.
+// The factor 1.1 is the median, over sky positions and orientations, of the
+// exact <h|h> over the rest of the expression (0.27 to 3.84 across 2400
+// sources at 0.1-20 mHz, 4 yr); the estimate only routes work, and binaries
+// it misses are still caught by the guard after the mixed pass.
+__device__ __forceinline__ double hh_estimate(const Src &s, int N, const WaveArgs &a) {
+    long long j = s.start + N / 2 - a.start_freq_ind;
+    j = j < 0 ? 0 : (j >= a.L ? a.L - 1 : j);
+    const double wL = s.f0 * a.fstar_inv;
+    const double r = 4.0 * wL * sin(wL);
+    return 1.1 * s.amp * s.amp * a.T * r * r / a.SA[j];
+}
+
+// Histogram cell of a binary: segment k, then its start bin within the data
 // on a logarithmic grid of NCB cells.  Galactic binaries crowd at low
 // frequency; log-spaced cells keep the per-cell counts (and so the atomic
 // contention on each counter) even, while neighbouring binaries still land
@@ -100,7 +121,8 @@ __global__ void __launch_bounds__(256) k_batch_setup(BatchArgs ba, Pack<P> *__re
     pk.s = make_src(ba.a.params, ba.a.num_params, ba.a.B, b, N, ba.a.T, ba.a.tdi2, ba.a.armlength);
     pk.aux = P::make_aux(pk.s);
     pack[b] = pk;
-    const int cell = batch_cell(k, pk.s.start, ba.a);
+    const bool routed = ba.route_tol > 0.0 && GUARD_REL * hh_estimate(pk.s, N, ba.a) > ba.route_tol;
+    const int cell = batch_cell(routed ? NBUCKET + k : k, pk.s.start, ba.a);
     ba.cell[b] = cell;
     atomicAdd(&ba.ctl[CTL_HIST + cell], 1);
 }
@@ -137,10 +159,10 @@ __global__ void __launch_bounds__(1024) k_batch_scan(int *ctl) {
     }
     __syncthreads();
     if (t == 0) {
-        // bucket k spans [offset of cell k NCB, offset of cell (k+1) NCB)
-        for (int k = 0; k < NBUCKET; k++) {
+        // segment k spans [offset of cell k NCB, offset of cell (k+1) NCB)
+        for (int k = 0; k < NSEG; k++) {
             const int o0 = ctl[CTL_HIST + k * NCB];
-            const int o1 = k + 1 < NBUCKET ? ctl[CTL_HIST + (k + 1) * NCB] : s_total;
+            const int o1 = k + 1 < NSEG ? ctl[CTL_HIST + (k + 1) * NCB] : s_total;
             ctl[CTL_LISTOFF + k] = o0;
             ctl[CTL_COUNT + k] = o1 - o0;
         }
@@ -337,6 +359,7 @@ template <class P>
 static int launch_buckets(const BatchArgs &ba, int precision, hipStream_t st, const int *variants,
                           hipEvent_t *bucket_events) {
     const bool guard = precision == GBGPU_MIXED && ba.guard_tol > 0.0;
+    const bool route = precision == GBGPU_MIXED && ba.route_tol > 0.0;
     for (int k = NBUCKET - 1; k >= 0; k--) {
         if (!ba.tabs[k]) continue;
         const int N = 32 << k;
@@ -366,6 +389,18 @@ static int launch_buckets(const BatchArgs &ba, int precision, hipStream_t st, co
             if (!r.tw) return hip_fail(hipErrorOutOfMemory, "twiddles");
             // the FP64 pass takes the FP64 tile of the bucket (tuned or default),
             // as the plain FP64 batch does: same speed, same results
+            rc = launch_ll_segment(N, GBGPU_FP64, GBGPU_MIXED, g_variant[0][k].load(), r, st);
+            if (rc) return rc;
+        }
+        if (route) {
+            // the binaries routed to FP64 at set-up: FP64 tile, mixed constants
+            WaveArgs r = a;
+            r.redo_list = nullptr;
+            r.redo_count = nullptr;
+            r.seg_count = ba.ctl + CTL_COUNT + NBUCKET + k;
+            r.seg_off = ba.ctl + CTL_LISTOFF + NBUCKET + k;
+            r.tw = twiddles_f64(N);
+            if (!r.tw) return hip_fail(hipErrorOutOfMemory, "twiddles");
             rc = launch_ll_segment(N, GBGPU_FP64, GBGPU_MIXED, g_variant[0][k].load(), r, st);
             if (rc) return rc;
         }
@@ -502,6 +537,7 @@ static int make_batch_args(BatchArgs &ba, double *loglike, gbgpu_cmplx *d_h, gbg
     a.h_h = reinterpret_cast<double2 *>(h_h);
     ba.Nb = N_bin;
     ba.guard_tol = cfg->guard_tol;
+    ba.route_tol = (cfg->precision == GBGPU_MIXED && cfg->guard_tol > 0.0) ? cfg->guard_tol : 0.0;
     for (int k = 0; k < NBUCKET; k++) {
         ba.tabs[k] = cfg->orbit_tables[k];
         ba.tws[k] = nullptr;
@@ -609,10 +645,16 @@ extern "C" int gbgpu_batch_guard_counts(const void *workspace, int num_bin, int 
     const char *ctl = static_cast<const char *>(workspace) + al(pack * (size_t)num_bin) +
                       3 * al(sizeof(int) * (size_t)num_bin);
     hipStream_t st = static_cast<hipStream_t>(stream);
-    hipError_t e = hipMemcpyAsync(counts, ctl + sizeof(int) * CTL_REDO, sizeof(int) * NBUCKET, hipMemcpyDeviceToHost,
-                                  st);
+    // binaries evaluated in FP64: re-evaluated after the mixed pass, plus
+    // those routed to FP64 at set-up
+    int redo[NBUCKET], routed[NBUCKET];
+    hipError_t e = hipMemcpyAsync(redo, ctl + sizeof(int) * CTL_REDO, sizeof(int) * NBUCKET, hipMemcpyDeviceToHost, st);
+    if (e == hipSuccess)
+        e = hipMemcpyAsync(routed, ctl + sizeof(int) * (CTL_COUNT + NBUCKET), sizeof(int) * NBUCKET,
+                           hipMemcpyDeviceToHost, st);
     if (e == hipSuccess) e = hipStreamSynchronize(st);
     if (e != hipSuccess) return hip_fail(e, "gbgpu_batch_guard_counts");
+    for (int k = 0; k < NBUCKET; k++) counts[k] = redo[k] + (precision == GBGPU_MIXED ? routed[k] : 0);
     return GBGPU_OK;
 }
 

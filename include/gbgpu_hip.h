@@ -208,7 +208,10 @@ typedef struct gbgpu_batch_cfg {
                                       GBGPU_BATCH_PERSISTENT: one persistent launch */
     double guard_tol;              /* mixed precision with GBGPU_BATCH_BUCKETS: binaries whose
                                       log-likelihood error bound 2.5e-6 (<h|h> + |<d|h>|) exceeds
-                                      guard_tol are re-evaluated in FP64; <= 0 disables */
+                                      guard_tol are evaluated in FP64: routed there at set-up when
+                                      an estimate of <h|h> from amplitude, frequency and the PSD
+                                      already exceeds the bound, else re-evaluated after the mixed
+                                      pass when its own result does; <= 0 disables */
 } gbgpu_batch_cfg;
 
 /* Workspace bytes gbgpu_wave_ll_batched needs for num_bin binaries. */
@@ -219,7 +222,7 @@ long long gbgpu_batch_workspace_bytes(int num_bin, int precision);
  * by N and start bin, then either each bucket runs the uniform fused kernel
  * over its device-side segment (GBGPU_BATCH_BUCKETS) or one persistent
  * launch pulls N-bucketed work items from an atomic queue, largest N first
- * (GBGPU_BATCH_PERSISTENT).  The reference needs one run_wave +
+ * (GBGPU_BATCH_PERSISTENT, switched off in this build: GBGPU_E_ARG).  The reference needs one run_wave +
  * get_ll call per distinct N (src/gbgpu/gbgpu.py:205-207, gbgpu_utils.cu:126).
  * Outputs as gbgpu_wave_ll; binaries with an unsupported N or a NULL orbit
  * table get NaN.  workspace: gbgpu_batch_workspace_bytes() of device memory,
@@ -250,8 +253,9 @@ int gbgpu_batch_autotune(double *loglike, const double *params, int num_params, 
 int gbgpu_batch_set_variants(int precision, const int *variants);
 
 /* Per-bucket counts (7 ints, N = 32 << k) of binaries the accuracy guard
- * re-evaluated in FP64 in the last gbgpu_wave_ll_batched call that used this
- * workspace.  Diagnostic: copies to host memory and synchronises `stream`. */
+ * evaluated in FP64 (routed at set-up plus re-evaluated after the mixed
+ * pass) in the last gbgpu_wave_ll_batched call that used this workspace.
+ * Diagnostic: copies to host memory and synchronises `stream`. */
 int gbgpu_batch_guard_counts(const void *workspace, int num_bin, int precision, int *counts, void *stream);
 
 /* Sampler utility with no counterpart in the reference: seeded Gaussian

@@ -687,7 +687,10 @@ def test_mixed_batch_accuracy_guard():
     assert np.abs(llraw - ll64).max() > 1e-3  # without the guard the loud ones miss
     assert np.abs(llg - ll64).max() <= 1e-3
     redone = llg == ll64
-    assert np.all(redone[loud & (bound > 2e-3)])  # clearly loud: re-evaluated in FP64
+    assert np.all(redone[loud & (bound > 2e-3)])  # clearly loud: evaluated in FP64
+    # every binary the guard evaluated in FP64 (routed at set-up or redone)
+    # carries the FP64 batch's result bit for bit
+    assert int(redone.sum()) >= sum(counts.values())
 
 
 @pytest.mark.parametrize("precision", ["fp64", "mixed"])
