@@ -9,7 +9,7 @@ HIP kernels behind the C ABI in ``include/gbgpu_hip.h``.
 
 from .cutils import KNOWN_BACKENDS, get_backend
 from .gbgpu import GBGPU, GBGPUBase
-from .orbits import EqualArmlengthOrbits, Orbits
+from .orbits import EqualArmlengthOrbits, InterpolatedOrbits, Orbits
 from .thirdbody import GBGPUThirdBody
 
 __all__ = [
@@ -17,6 +17,7 @@ __all__ = [
     "GBGPUBase",
     "GBGPUThirdBody",
     "EqualArmlengthOrbits",
+    "InterpolatedOrbits",
     "Orbits",
     "KNOWN_BACKENDS",
     "get_backend",

@@ -48,6 +48,29 @@ class AnalyticOrbits:
         return np.stack(np.broadcast_arrays(x, y, z), axis=-1)
 
 
+class TableOrbits:
+    """Orbits from sampled positions t (K,), x (K, 3, 3), linearly
+    interpolated per spacecraft and coordinate with numpy's ``interp``: the
+    oracle-side statement of an orbit file read with lisatools'
+    ``linear_interp_setup`` (src/gbgpu/gbgpu.py:86-101, :390-397)."""
+
+    def __init__(self, t, x, armlength=LARM):
+        self.t = np.asarray(t, dtype=np.float64)
+        self.x = np.asarray(x, dtype=np.float64)
+        self.armlength = armlength
+        self.t_base = self.t
+
+    def get_pos(self, t, sc):
+        t = np.asarray(t, dtype=np.float64)
+        t, sc = np.broadcast_arrays(t, np.asarray(sc))
+        out = np.empty(t.shape + (3,))
+        for s in (1, 2, 3):
+            m = sc == s
+            for c in range(3):
+                out[..., c][m] = np.interp(t[m], self.t, self.x[:, s - 1, c])
+        return out
+
+
 # ---------------------------------------------------------------------------
 # utilities (src/gbgpu/utils/utility.py)
 # ---------------------------------------------------------------------------
