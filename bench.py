@@ -681,6 +681,7 @@ def main():
     ap.add_argument("--no-deriv", action="store_true", help="skip the differentiation workload")
     ap.add_argument("--no-global", action="store_true", help="skip the global-template and swap workload")
     ap.add_argument("--no-dropin", action="store_true", help="skip the drop-in run_wave + native get_ll path")
+    ap.add_argument("--no-transforms", action="store_true", help="skip the matrix-core transform variants")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build measures one GPU only; --gpus must be 1"}))
@@ -712,8 +713,8 @@ def main():
     lib = _capi.load()
     Tadj = int(T / dt) * dt
 
-    def runner(precision):
-        cfg, _tab = gb._cfg(N, Tadj, False, precision)
+    def runner(precision, transform="lds"):
+        cfg, _tab = gb._cfg(N, Tadj, False, precision, _capi.TRANSFORMS[transform])
         dat = _capi.Data(A=A.data_ptr(), E=E.data_ptr(), psdA=SA.data_ptr(), psdE=SE.data_ptr(), data_index=None,
                          noise_index=None, data_length=L, start_freq_ind=0, df=df)
         st = _capi.stream_handle()
@@ -751,6 +752,27 @@ def main():
                      kernel="k_wave<256, MODE_LL, circular, Mixed>")
         if args.only is None:
             mixed["vs_fp64_hip"] = mixed_vs_fp64(gb, params, A, E, SA, SE, N, T, dt)
+        if not args.no_transforms:
+            ll64 = None
+            if args.only is None:
+                call64()
+                torch.cuda.synchronize()
+                ll64 = ll.cpu().numpy()
+            mixed["transforms"] = {}
+            for tr in ("mfma_f32", "mfma_bf16x3", "mfma_bf16x6"):
+                print(f"[bench] timing mixed transform={tr}", file=sys.stderr, flush=True)
+                callt, _tt = runner("mixed", tr)
+                callt()
+                torch.cuda.synchronize()
+                with _capi.KernelTimer() as ktt:
+                    sect, _ = time_calls(callt, args.steps, args.warmup)
+                r = dict(value=B / sect, kernel_ms=ktt.mean_ms, speedup_vs_lds_transform=secm / sect)
+                if ll64 is not None:
+                    r["max_abs_logL_diff_vs_fp64_hip"] = float(np.max(np.abs(ll.cpu().numpy() - ll64)))
+                    gb.transform = tr
+                    r["vs_fp64_hip"] = mixed_vs_fp64(gb, params, A, E, SA, SE, N, T, dt)
+                    gb.transform = "lds"
+                mixed["transforms"][tr] = r
 
     print("[bench] copy probe", file=sys.stderr, flush=True)
     copy_gbs = copy_probe()

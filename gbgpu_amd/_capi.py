@@ -17,6 +17,8 @@ LIB_PATH = Path(__file__).resolve().parent / "libgbgpu_hip.so"
 
 GBGPU_FP64 = 0
 GBGPU_MIXED = 1
+# transform of the slow part (gbgpu_wave_cfg.transform, gbgpu_batch_cfg.transform)
+TRANSFORMS = {"lds": 0, "mfma_f32": 1, "mfma_bf16x3": 2, "mfma_bf16x6": 3}
 GBGPU_BATCH_BUCKETS = 0
 GBGPU_BATCH_PERSISTENT = 1
 ORBIT_ROWS = 22
@@ -38,6 +40,7 @@ class WaveCfg(ctypes.Structure):
         ("orbit_table", c_void_p),
         ("tdi2", c_int),
         ("precision", c_int),
+        ("transform", c_int),
     ]
 
 
@@ -68,6 +71,7 @@ class BatchCfg(ctypes.Structure):
         ("orbit_tables", c_void_p * 7),
         ("strategy", c_int),
         ("guard_tol", c_double),
+        ("transform", c_int),
     ]
 
 

@@ -45,12 +45,17 @@ class BatchedLogLike:
             bound, else re-evaluated after the mixed pass (None or 0
             disables; loud templates otherwise carry an absolute error that
             grows with SNR^2).
+        transform: transform of the mixed N = 256 bucket ("lds",
+            "mfma_f32", "mfma_bf16x3", "mfma_bf16x6"; see GBGPU.transform);
+            None takes gb.transform.  Every other bucket and every FP64 pass
+            uses the LDS transform.
     """
 
     STRATEGIES = {"buckets": _capi.GBGPU_BATCH_BUCKETS, "persistent": _capi.GBGPU_BATCH_PERSISTENT}
 
     def __init__(self, gb, data, psd, T, dt, precision="mixed", start_freq_ind=0, N_levels=DEFAULT_N_LEVELS,
-                 f_edges=DEFAULT_F_EDGES, phase_marginalize=False, strategy="buckets", guard_tol=1e-3):
+                 f_edges=DEFAULT_F_EDGES, phase_marginalize=False, strategy="buckets", guard_tol=1e-3,
+                 transform=None):
         torch = _torch()
         if gb.d_d is None:
             raise ValueError("gb.d_d must be set before building a likelihood")
@@ -76,9 +81,13 @@ class BatchedLogLike:
         self.SA = gb._to_dev(psd[0], np.float64).reshape(-1)
         self.SE = gb._to_dev(psd[1], np.float64).reshape(-1)
         self.L = int(self.A.numel())
+        transform = gb.transform if transform is None else transform
+        if transform not in _capi.TRANSFORMS:
+            raise ValueError(f"transform must be one of {list(_capi.TRANSFORMS)}")
         self.cfg = _capi.BatchCfg(T=float(self.T), armlength=float(gb.orbits.armlength), tdi2=0,
                                   precision=1 if precision == "mixed" else 0, strategy=self.STRATEGIES[strategy],
-                                  guard_tol=self.guard_tol)
+                                  guard_tol=self.guard_tol,
+                                  transform=_capi.TRANSFORMS[transform] if precision == "mixed" else 0)
         self._tabs = {}
         for N in self.N_levels:
             self._add_table(int(N))
@@ -110,9 +119,12 @@ class BatchedLogLike:
         return self._ws
 
     def autotune(self, params, N=None):
-        """Time both tile shapes per N bucket on this batch and keep the faster
-        (process-wide, per precision); returns {N: shape} (0: min(N/4, 256)
-        threads per binary, 1: half of that).  Synchronises; not capturable."""
+        """Time the kernel variants per N bucket on this batch and keep the
+        fastest (process-wide, per precision); returns {N: variant}: 0 =
+        min(N/4, 256) threads per binary, 1 = half of that, and for the mixed
+        N = 256 bucket also 2 / 4 = the "mfma_f32" / "mfma_bf16x6" transforms
+        (FP32-accurate; the two-piece "mfma_bf16x3", variant 3, only on
+        request).  Synchronises; not capturable."""
         torch = _torch()
         p = params if torch.is_tensor(params) else torch.from_numpy(np.ascontiguousarray(params, np.float64))
         p = p.to(self.dev).contiguous()
@@ -133,7 +145,8 @@ class BatchedLogLike:
         return {32 << k: int(c) for k, c in enumerate(chosen) if c >= 0}
 
     def set_shapes(self, shapes):
-        """Pin the per-bucket tile shapes ({N: 0 or 1}; others 0) for this precision."""
+        """Pin the per-bucket variants ({N: 0 or 1}, and 2-4 for the mixed
+        N = 256 bucket as in autotune; others 0) for this precision."""
         v = np.zeros(7, dtype=np.int32)
         for n, c in shapes.items():
             v[int(np.log2(n)) - 5] = c

@@ -32,6 +32,11 @@ CFLAGS = [
     "-target-feature",
     "-Xclang",
     "-packed-fp32-ops",
+    # matrix-core results in architectural VGPRs rather than accumulation
+    # registers: the transform's accumulators then share the kernel's VGPR
+    # budget instead of raising its register total (occupancy 7, not 5)
+    "-mllvm",
+    "-amdgpu-mfma-vgpr-form=1",
     f"-I{INCLUDE}",
     f"-I{CSRC}",
 ]

@@ -449,6 +449,10 @@ static int check_deriv_cfg(const double *params, int num_params, int num_bin, co
         set_error("derivatives are computed in FP64 only");
         return GBGPU_E_ARG;
     }
+    if (cfg->transform != GBGPU_TRANSFORM_LDS) {
+        set_error("derivatives use the LDS transform only");
+        return GBGPU_E_ARG;
+    }
     return GBGPU_OK;
 }
 

@@ -67,6 +67,7 @@ struct BatchArgs {
     int *ctl;
     double guard_tol;
     double route_tol;  // > 0: route binaries whose estimated error bound exceeds it to FP64
+    int transform;     // GBGPU_TRANSFORM_* requested for the mixed N = 256 bucket
 };
 
 __global__ void k_batch_reset(int *ctl) {
@@ -343,8 +344,10 @@ static int launch_persistent(BatchArgs &ba, hipStream_t st) {
     return GBGPU_OK;
 }
 
-// Tile shape of the per-bucket kernel, per (precision, bucket): 0 = N/4
-// threads per binary, 1 = half as many; set by gbgpu_batch_autotune.
+// Variant of the per-bucket kernel, per (precision, bucket): 0 = N/4 threads
+// per binary, 1 = half as many; for the mixed N = 256 bucket also
+// VARIANT_DFT0 + t - 1 = matrix-core transform t (GBGPU_TRANSFORM_*); set by
+// gbgpu_batch_autotune or gbgpu_batch_set_variants.
 static std::atomic<int> g_variant[2][NBUCKET] = {};
 
 // One launch of the uniform fused kernel per populated bucket.  With the
@@ -363,7 +366,10 @@ static int launch_buckets(const BatchArgs &ba, int precision, hipStream_t st, co
     for (int k = NBUCKET - 1; k >= 0; k--) {
         if (!ba.tabs[k]) continue;
         const int N = 32 << k;
-        const int var = variants ? variants[k] : g_variant[precision == GBGPU_MIXED ? 1 : 0][k].load();
+        int var = variants ? variants[k] : g_variant[precision == GBGPU_MIXED ? 1 : 0][k].load();
+        // a requested matrix-core transform of the mixed N = 256 bucket wins
+        if (precision == GBGPU_MIXED && N == 256 && ba.transform != GBGPU_TRANSFORM_LDS)
+            var = VARIANT_DFT0 + ba.transform - 1;
         if (bucket_events) (void)hipEventRecord(bucket_events[2 * k], st);
         WaveArgs a = ba.a;
         a.tab = ba.tabs[k];
@@ -518,6 +524,11 @@ static int make_batch_args(BatchArgs &ba, double *loglike, gbgpu_cmplx *d_h, gbg
         set_error("unknown batch strategy");
         return GBGPU_E_ARG;
     }
+    if (cfg->transform < GBGPU_TRANSFORM_LDS || cfg->transform > GBGPU_TRANSFORM_MFMA_BF16X6 ||
+        (cfg->transform != GBGPU_TRANSFORM_LDS && cfg->precision != GBGPU_MIXED)) {
+        set_error("transform: unknown, or a matrix-core transform without GBGPU_MIXED");
+        return GBGPU_E_ARG;
+    }
     int rc = check_data(data);
     if (rc) return rc;
     ba = BatchArgs{};
@@ -536,6 +547,12 @@ static int make_batch_args(BatchArgs &ba, double *loglike, gbgpu_cmplx *d_h, gbg
     a.d_h = reinterpret_cast<double2 *>(d_h);
     a.h_h = reinterpret_cast<double2 *>(h_h);
     ba.Nb = N_bin;
+    ba.transform = cfg->transform;
+    if (cfg->precision == GBGPU_MIXED && cfg->orbit_tables[3]) {
+        // the N = 256 bucket may run a matrix-core transform (requested or tuned)
+        a.dftc = dft256_table();
+        if (!a.dftc) return hip_fail(hipErrorOutOfMemory, "transform constants");
+    }
     ba.guard_tol = cfg->guard_tol;
     ba.route_tol = (cfg->precision == GBGPU_MIXED && cfg->guard_tol > 0.0) ? cfg->guard_tol : 0.0;
     for (int k = 0; k < NBUCKET; k++) {
@@ -582,13 +599,27 @@ extern "C" int gbgpu_batch_autotune(double *loglike, const double *params, int n
     hipEvent_t ev[2 * NBUCKET];
     for (int i = 0; i < 2 * NBUCKET; i++)
         if (hipEventCreate(&ev[i]) != hipSuccess) return hip_fail(hipErrorOutOfMemory, "hipEventCreate");
-    constexpr int NVAR = 2, REPS = 3;
+    // candidates: the two tile shapes for every bucket; for the mixed N = 256
+    // bucket also the FP32-accurate matrix-core transforms (FP32 and
+    // three-piece bf16 products; the two-piece split is less accurate than the
+    // LDS transform and is used only on request)
+    constexpr int NVAR = 5, REPS = 3;
+    auto allowed = [&](int v, int k) {
+        return v < VARIANT_DFT0 ||
+               (cfg->precision == GBGPU_MIXED && k == 3 && cfg->transform == GBGPU_TRANSFORM_LDS &&
+                (v == VARIANT_DFT0 + GBGPU_TRANSFORM_MFMA_F32 - 1 || v == VARIANT_DFT0 + GBGPU_TRANSFORM_MFMA_BF16X6 - 1));
+    };
     float best[NVAR][NBUCKET];
     for (int v = 0; v < NVAR; v++)
         for (int k = 0; k < NBUCKET; k++) best[v][k] = 3.0e38f;
     for (int v = 0; v < NVAR && !rc; v++) {
         int var[NBUCKET];
-        for (int k = 0; k < NBUCKET; k++) var[k] = v;
+        bool any = false;
+        for (int k = 0; k < NBUCKET; k++) {
+            var[k] = allowed(v, k) ? v : 0;
+            any = any || (v >= VARIANT_DFT0 && allowed(v, k) && ba.tabs[k]);
+        }
+        if (v >= VARIANT_DFT0 && !any) continue;
         for (int r = 0; r < REPS && !rc; r++) {  // the first repetition warms up
             rc = cfg->precision == GBGPU_MIXED
                      ? launch_batch<Mixed>(ba, cfg->precision, cfg->strategy, workspace, st, var, ev)
@@ -600,7 +631,7 @@ extern "C" int gbgpu_batch_autotune(double *loglike, const double *params, int n
                 break;
             }
             for (int k = 0; k < NBUCKET && r > 0; k++) {
-                if (!ba.tabs[k]) continue;
+                if (!ba.tabs[k] || !allowed(v, k)) continue;
                 float ms = 0.0f;
                 if (hipEventElapsedTime(&ms, ev[2 * k], ev[2 * k + 1]) == hipSuccess && ms < best[v][k])
                     best[v][k] = ms;
@@ -612,7 +643,9 @@ extern "C" int gbgpu_batch_autotune(double *loglike, const double *params, int n
     for (int k = 0; k < NBUCKET; k++) {
         int pick = -1;
         if (ba.tabs[k]) {
-            pick = best[1][k] < best[0][k] ? 1 : 0;
+            pick = 0;
+            for (int v = 1; v < NVAR; v++)
+                if (best[v][k] < best[pick][k]) pick = v;
             g_variant[pi][k].store(pick);
         }
         if (chosen) chosen[k] = pick;
@@ -625,11 +658,15 @@ extern "C" int gbgpu_batch_set_variants(int precision, const int *variants) {
         set_error("gbgpu_batch_set_variants: bad arguments");
         return GBGPU_E_ARG;
     }
-    for (int k = 0; k < NBUCKET; k++)
-        if (variants[k] != 0 && variants[k] != 1) {
-            set_error("gbgpu_batch_set_variants: variants are 0 or 1");
+    for (int k = 0; k < NBUCKET; k++) {
+        const bool dft = precision == GBGPU_MIXED && k == 3 && variants[k] >= VARIANT_DFT0 &&
+                         variants[k] <= VARIANT_DFT0 + GBGPU_TRANSFORM_MFMA_BF16X6 - 1;
+        if (variants[k] != 0 && variants[k] != 1 && !dft) {
+            set_error("gbgpu_batch_set_variants: variants are 0 or 1, and 2-4 (matrix-core transforms) for the "
+                      "mixed N = 256 bucket");
             return GBGPU_E_ARG;
         }
+    }
     for (int k = 0; k < NBUCKET; k++) g_variant[precision == GBGPU_MIXED ? 1 : 0][k].store(variants[k]);
     return GBGPU_OK;
 }

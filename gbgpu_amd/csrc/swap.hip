@@ -200,6 +200,10 @@ extern "C" int gbgpu_swap_ll_diff(double *dll, double *terms, const double *para
                                   int num_prop, const gbgpu_wave_cfg *cfg, const gbgpu_data *data, void *stream) {
     int rc = check_cfg(params_pairs, num_params, 2 * num_prop, cfg);
     if (rc) return rc;
+    if (cfg->transform != GBGPU_TRANSFORM_LDS) {
+        set_error("gbgpu_swap_ll_diff uses the LDS transform only");
+        return GBGPU_E_ARG;
+    }
     if ((rc = check_data(data))) return rc;
     if (!dll) {
         set_error("dll must not be null");

@@ -27,7 +27,9 @@ namespace gbgpu {
 // Arrays a variant does not use have length zero, so that the transform
 // buffers alone set the footprint (32 KB for the FP64 likelihood at N = 256:
 // five workgroups per CU).
-template <int N, int MODE, bool THIRD, class P, class PK = P, int M = 1>
+// With the matrix-core transform (DFT != 0) the samples and spectra stay in
+// registers and the workgroup needs no LDS.
+template <int N, int MODE, bool THIRD, class P, class PK = P, int M = 1, int DFT = GBGPU_TRANSFORM_LDS>
 struct WaveLds {
     using G = Geo<N, M>;
     using CT = typename P::CT;
@@ -35,7 +37,7 @@ struct WaveLds {
     // a wavefront that owns its binary reads the Pack straight into scalar registers
     static constexpr bool PACK_IN_LDS = G::TPB < 64;
     static constexpr int NEDGE = (THIRD && G::WPB > 1) ? 6 * G::WPB : 0;
-    CT sbuf[G::BPW * NCH * N];
+    CT sbuf[DFT == GBGPU_TRANSFORM_LDS ? G::BPW * NCH * N : 0];
     Pack<PK> spack[PACK_IN_LDS ? G::BPW : 0];
     double stot[G::WPB > 1 ? G::BPW * G::WPB : 0];
     double s_edge[G::BPW * NEDGE];
@@ -51,12 +53,12 @@ __device__ __forceinline__ typename P::Aux aux_of(const Pack<PK> &pk) {
         return P::make_aux(pk.s);
 }
 
-template <int N, int MODE, bool THIRD, class P, class PK = P, int M = 1>
-__device__ __forceinline__ void wave_pass(const WaveArgs &a, WaveLds<N, MODE, THIRD, P, PK, M> &L, long long blk,
-                                          long long count, long long first, int tid, const double *tab,
-                                          const void *twp) {
+template <int N, int MODE, bool THIRD, class P, class PK = P, int M = 1, int DFT = GBGPU_TRANSFORM_LDS>
+__device__ __forceinline__ void wave_pass(const WaveArgs &a, WaveLds<N, MODE, THIRD, P, PK, M, DFT> &L,
+                                          long long blk, long long count, long long first, int tid,
+                                          const double *tab, const void *twp) {
     using G = Geo<N, M>;
-    using W = WaveLds<N, MODE, THIRD, P, PK, M>;
+    using W = WaveLds<N, MODE, THIRD, P, PK, M, DFT>;
     using CT = typename P::CT;
     constexpr int TPB = G::TPB, BPW = G::BPW, WPB = G::WPB, SPT = G::SPT;
     constexpr int NCH = W::NCH;
@@ -83,8 +85,20 @@ __device__ __forceinline__ void wave_pass(const WaveArgs &a, WaveLds<N, MODE, TH
     double *tot = stot + slot * WPB;
     ThirdScratch ts{L.s_edge + slot * W::NEDGE};
 
-    binary_spectrum<N, NCH, THIRD, P, M>(s, per_group<TPB>(aux_of<P, PK>(*pk)), valid, lane, tab, a.fstar_inv,
-                                         a.scale, buf, static_cast<const CT *>(twp), ts, tot);
+    constexpr bool REG = DFT != GBGPU_TRANSFORM_LDS;
+    CT xr[REG ? NCH * SPT : 1];  // register-resident spectra (matrix-core transform)
+    binary_spectrum<N, NCH, THIRD, P, M, DFT>(s, per_group<TPB>(aux_of<P, PK>(*pk)), valid, lane, tab,
+                                              a.fstar_inv, a.scale, buf, static_cast<const CT *>(twp), ts, tot,
+                                              xr, a.dftc);
+    // bin of output position q of this thread: fftshifted position m(q),
+    // spectrum value spec(c, q) of channel c
+    auto pos = [&](int q) { return REG ? (dft256_index(lane, q) + N / 2) & (N - 1) : lane + q * TPB; };
+    auto spec = [&](int c, int q) {
+        if constexpr (REG)
+            return xr[c * SPT + q];
+        else
+            return buf[c * N + ((lane + q * TPB + N / 2) & (N - 1))];  // fftshift
+    };
 
     const double2 fac = P::fac(s);
     if constexpr (MODE == MODE_WAVE) {
@@ -93,11 +107,10 @@ __device__ __forceinline__ void wave_pass(const WaveArgs &a, WaveLds<N, MODE, TH
             double2 *aet = a.AETf ? a.AETf + b * 3 * N : nullptr;
 #pragma unroll
             for (int q = 0; q < SPT; q++) {
-                const int m = lane + q * TPB;
-                const int idx = (m + N / 2) & (N - 1);  // fftshift
-                const double2 X = cmul(P::up(buf[idx]), fac);
-                const double2 Y = cmul(P::up(buf[N + idx]), fac);
-                const double2 Z = cmul(P::up(buf[2 * N + idx]), fac);
+                const int m = pos(q);
+                const double2 X = cmul(P::up(spec(0, q)), fac);
+                const double2 Y = cmul(P::up(spec(1, q)), fac);
+                const double2 Z = cmul(P::up(spec(2, q)), fac);
                 if (xyz) {
                     xyz[m] = X;
                     xyz[N + m] = Y;
@@ -122,10 +135,9 @@ __device__ __forceinline__ void wave_pass(const WaveArgs &a, WaveLds<N, MODE, TH
             double2 *ae = a.AETf + b * 2 * N;
 #pragma unroll
             for (int q = 0; q < SPT; q++) {
-                const int m = lane + q * TPB;
-                const int idx = (m + N / 2) & (N - 1);  // fftshift
-                ae[m] = cmul(P::up(buf[idx]), fac);
-                ae[N + m] = cmul(P::up(buf[N + idx]), fac);
+                const int m = pos(q);
+                ae[m] = cmul(P::up(spec(0, q)), fac);
+                ae[N + m] = cmul(P::up(spec(1, q)), fac);
             }
             if (lane == 0) a.start_inds[b] = s.start;
         }
@@ -134,7 +146,10 @@ __device__ __forceinline__ void wave_pass(const WaveArgs &a, WaveLds<N, MODE, TH
         if (valid) {
             const long long doff = (a.di ? (long long)a.di[b] : 0LL) * a.L;
             const long long noff = (a.ni ? (long long)a.ni[b] : 0LL) * a.L;
-            ll_partial<N, P, M>(buf, fac, lane, s.start - a.start_freq_ind, doff, noff, a, dhr, dhi, hh);
+            if constexpr (REG)
+                ll_partial_regs<P>(xr, fac, lane, s.start - a.start_freq_ind, doff, noff, a, dhr, dhi, hh);
+            else
+                ll_partial<N, P, M>(buf, fac, lane, s.start - a.start_freq_ind, doff, noff, a, dhr, dhi, hh);
         }
         dhr = group_total<TPB>(dhr, lane, tot);
         dhi = group_total<TPB>(dhi, lane, tot);
@@ -155,11 +170,11 @@ __device__ __forceinline__ void wave_pass(const WaveArgs &a, WaveLds<N, MODE, TH
     }
 }
 
-template <int N, int MODE, bool THIRD, class P>
+template <int N, int MODE, bool THIRD, class P, int DFT = GBGPU_TRANSFORM_LDS>
 __global__ void __launch_bounds__((Geo<N, wave_tile<N>()>::WG)) k_wave(WaveArgs a) {
     constexpr int M = wave_tile<N>();
-    __shared__ WaveLds<N, MODE, THIRD, P, P, M> lds;
-    wave_pass<N, MODE, THIRD, P, P, M>(a, lds, blockIdx.x, a.B, 0, threadIdx.x, a.tab, a.tw);
+    __shared__ WaveLds<N, MODE, THIRD, P, P, M, DFT> lds;
+    wave_pass<N, MODE, THIRD, P, P, M, DFT>(a, lds, blockIdx.x, a.B, 0, threadIdx.x, a.tab, a.tw);
 }
 
 // Fused likelihood over one device-side segment of a batch: item i <
@@ -168,10 +183,10 @@ __global__ void __launch_bounds__((Geo<N, wave_tile<N>()>::WG)) k_wave(WaveArgs 
 // arithmetic are the same in every pass: opaque copies of the pointers and
 // of the thread index keep the compiler from hoisting all of them out of the
 // loop into registers (which would cost occupancy).
-template <int N, class P, class PK, int M>
+template <int N, class P, class PK, int M, int DFT = GBGPU_TRANSFORM_LDS>
 __global__ void __launch_bounds__((Geo<N, M>::WG)) k_wave_seg(WaveArgs a) {
     constexpr int BPW = Geo<N, M>::BPW;
-    __shared__ WaveLds<N, MODE_LL, false, P, PK, M> lds;
+    __shared__ WaveLds<N, MODE_LL, false, P, PK, M, DFT> lds;
     const long long count = *a.seg_count;
     const long long first = *a.seg_off;
     for (long long blk = blockIdx.x; blk * BPW < count; blk += gridDim.x) {
@@ -179,15 +194,43 @@ __global__ void __launch_bounds__((Geo<N, M>::WG)) k_wave_seg(WaveArgs a) {
         const void *twp = a.tw;
         int tid = threadIdx.x;
         asm volatile("" : "+s"(tab), "+s"(twp), "+v"(tid));
-        wave_pass<N, MODE_LL, false, P, PK, M>(a, lds, blk, count, first, tid, tab, twp);
+        wave_pass<N, MODE_LL, false, P, PK, M, DFT>(a, lds, blk, count, first, tid, tab, twp);
         __syncthreads();  // LDS is reused by the next pass
     }
+}
+
+// k_wave with the requested transform; the matrix-core variants exist for
+// the mixed policy at N = 256 without a third body (check_cfg has rejected
+// every other combination).
+template <int NN, int MODE, bool THIRD, class P>
+static void launch_k_wave(int transform, long long grid, hipStream_t st, const WaveArgs &aa) {
+    constexpr int WG = Geo<NN, wave_tile<NN>()>::WG;
+    if constexpr (NN == 256 && !THIRD && std::is_same<P, Mixed>::value) {
+        switch (transform) {
+            case GBGPU_TRANSFORM_MFMA_F32:
+                hipLaunchKernelGGL((k_wave<NN, MODE, THIRD, P, GBGPU_TRANSFORM_MFMA_F32>), dim3((unsigned)grid),
+                                   dim3(WG), 0, st, aa);
+                return;
+            case GBGPU_TRANSFORM_MFMA_BF16X3:
+                hipLaunchKernelGGL((k_wave<NN, MODE, THIRD, P, GBGPU_TRANSFORM_MFMA_BF16X3>), dim3((unsigned)grid),
+                                   dim3(WG), 0, st, aa);
+                return;
+            case GBGPU_TRANSFORM_MFMA_BF16X6:
+                hipLaunchKernelGGL((k_wave<NN, MODE, THIRD, P, GBGPU_TRANSFORM_MFMA_BF16X6>), dim3((unsigned)grid),
+                                   dim3(WG), 0, st, aa);
+                return;
+            default:
+                break;
+        }
+    }
+    hipLaunchKernelGGL((k_wave<NN, MODE, THIRD, P>), dim3((unsigned)grid), dim3(WG), 0, st, aa);
 }
 
 // ws / ws_bytes: caller-provided per-binary workspace (nullptr: stream-ordered
 // scratch allocated by with_pack).
 template <int MODE, bool THIRD, class P>
-static int launch_wave(int N, const WaveArgs &a, hipStream_t st, char *ws = nullptr, size_t ws_bytes = 0) {
+static int launch_wave(int N, const WaveArgs &a, hipStream_t st, char *ws = nullptr, size_t ws_bytes = 0,
+                       int transform = GBGPU_TRANSFORM_LDS) {
 #define GBGPU_CASE(NN)                                                                              \
     case NN: {                                                                                      \
         using GG = Geo<NN, wave_tile<NN>()>;                                                        \
@@ -196,8 +239,7 @@ static int launch_wave(int N, const WaveArgs &a, hipStream_t st, char *ws = null
             a, NN, st,                                                                              \
             [&](const WaveArgs &aa) -> int {                                                        \
                 KernelTimer kt(st);                                                                 \
-                hipLaunchKernelGGL((k_wave<NN, MODE, THIRD, P>), dim3((unsigned)grid), dim3(GG::WG), \
-                                   0, st, aa);                                                      \
+                launch_k_wave<NN, MODE, THIRD, P>(transform, grid, st, aa);                         \
                 GBGPU_CHECK_LAUNCH("k_wave");                                                       \
                 return GBGPU_OK;                                                                    \
             },                                                                                      \
@@ -222,7 +264,7 @@ static int launch_wave(int N, const WaveArgs &a, hipStream_t st, char *ws = null
 // launches of gbgpu_wave_ll_batched): a.pack, a.order, a.seg_count and
 // a.seg_off are set by the caller; the segment's size is known only on the
 // device, so the grid covers a few passes of resident workgroups and strides.
-template <class P, class PK, int NN, int M>
+template <class P, class PK, int NN, int M, int DFT = GBGPU_TRANSFORM_LDS>
 static int launch_segment_n(const WaveArgs &a, hipStream_t st) {
     // several passes' worth of resident workgroups (initialised once, thread-safe)
     static const int grid = [] {
@@ -234,17 +276,34 @@ static int launch_segment_n(const WaveArgs &a, hipStream_t st) {
     long long g = (a.B + Geo<NN, M>::BPW - 1) / Geo<NN, M>::BPW;
     if (g > grid) g = grid;
     KernelTimer kt(st);
-    hipLaunchKernelGGL((k_wave_seg<NN, P, PK, M>), dim3((unsigned)g), dim3(Geo<NN, M>::WG), 0, st, a);
+    hipLaunchKernelGGL((k_wave_seg<NN, P, PK, M, DFT>), dim3((unsigned)g), dim3(Geo<NN, M>::WG), 0, st, a);
     GBGPU_CHECK_LAUNCH("k_wave_seg");
     return GBGPU_OK;
 }
 
 // variant 0: the waveform kernels' tile (wave_tile); variant 1: half the
-// threads per binary of that tile.
+// threads per binary of that tile; VARIANT_DFT0.. (mixed N = 256 only): the
+// matrix-core transforms, one wavefront per binary.
 template <class P, class PK, int NN>
 static int launch_segment_m(int variant, const WaveArgs &a, hipStream_t st) {
     constexpr int M0 = wave_tile<NN>();
     constexpr int M1 = M0 == 0 ? 1 : 2;
+    if constexpr (NN == 256 && std::is_same<P, Mixed>::value && std::is_same<PK, Mixed>::value) {
+        if (variant >= VARIANT_DFT0 && !a.dftc) {
+            set_error("matrix-core transform requested without its constant table");
+            return GBGPU_E_ARG;
+        }
+        switch (variant - VARIANT_DFT0 + 1) {
+            case GBGPU_TRANSFORM_MFMA_F32:
+                return launch_segment_n<P, PK, NN, 1, GBGPU_TRANSFORM_MFMA_F32>(a, st);
+            case GBGPU_TRANSFORM_MFMA_BF16X3:
+                return launch_segment_n<P, PK, NN, 1, GBGPU_TRANSFORM_MFMA_BF16X3>(a, st);
+            case GBGPU_TRANSFORM_MFMA_BF16X6:
+                return launch_segment_n<P, PK, NN, 1, GBGPU_TRANSFORM_MFMA_BF16X6>(a, st);
+            default:
+                break;
+        }
+    }
     return variant == 1 ? launch_segment_n<P, PK, NN, M1>(a, st) : launch_segment_n<P, PK, NN, M0>(a, st);
 }
 
@@ -277,12 +336,21 @@ int launch_ll_segment(int N, int precision, int pack_precision, int variant, con
 
 template <int MODE>
 static int launch_any(int N, bool third, int precision, const WaveArgs &a, hipStream_t st, char *ws = nullptr,
-                      size_t ws_bytes = 0) {
+                      size_t ws_bytes = 0, int transform = GBGPU_TRANSFORM_LDS) {
     if (precision == GBGPU_MIXED)
         return third ? launch_wave<MODE, true, Mixed>(N, a, st, ws, ws_bytes)
-                     : launch_wave<MODE, false, Mixed>(N, a, st, ws, ws_bytes);
+                     : launch_wave<MODE, false, Mixed>(N, a, st, ws, ws_bytes, transform);
     return third ? launch_wave<MODE, true, F64>(N, a, st, ws, ws_bytes)
                  : launch_wave<MODE, false, F64>(N, a, st, ws, ws_bytes);
+}
+
+// a.dftc from the per-device constant cache when a matrix-core transform is
+// requested (check_cfg has validated the combination).
+static int set_transform(const gbgpu_wave_cfg *cfg, WaveArgs &a) {
+    if (cfg->transform == GBGPU_TRANSFORM_LDS) return GBGPU_OK;
+    a.dftc = dft256_table();
+    if (!a.dftc) return hip_fail(hipErrorOutOfMemory, "transform constants");
+    return GBGPU_OK;
 }
 
 int check_cfg(const double *params, int num_params, int num_bin, const gbgpu_wave_cfg *cfg) {
@@ -304,6 +372,15 @@ int check_cfg(const double *params, int num_params, int num_bin, const gbgpu_wav
     }
     if (cfg->precision != GBGPU_FP64 && cfg->precision != GBGPU_MIXED) {
         set_error("unknown precision");
+        return GBGPU_E_ARG;
+    }
+    if (cfg->transform < GBGPU_TRANSFORM_LDS || cfg->transform > GBGPU_TRANSFORM_MFMA_BF16X6) {
+        set_error("unknown transform");
+        return GBGPU_E_ARG;
+    }
+    if (cfg->transform != GBGPU_TRANSFORM_LDS &&
+        (cfg->precision != GBGPU_MIXED || cfg->N != 256 || num_params != GBGPU_NPARAMS)) {
+        set_error("the matrix-core transforms need precision GBGPU_MIXED, N = 256 and no third body");
         return GBGPU_E_ARG;
     }
     return GBGPU_OK;
@@ -350,7 +427,8 @@ const void *twiddles_for(int N, int precision) {
 // ---- allocation-free fused likelihood (gbgpu_wave_ll_noalloc) ---------------
 //
 // `scratch` holds, at fixed offsets: the FP64 and FP32 twiddle tables of every
-// supported N (N = 32 << k, k < 7, back to back), then the per-binary
+// supported N (N = 32 << k, k < 7, back to back), the per-lane constants of
+// the matrix-core transforms (dft256_table's layout), then the per-binary
 // workspace of with_pack (Pack array, sort keys/values, radix-sort storage)
 // for up to GBGPU_NOALLOC_MAX_BIN binaries.  Each device holds its own copy
 // of the array for the life of the process.  In host code `scratch` names a
@@ -360,7 +438,8 @@ constexpr int NOALLOC_NK = 7;
 constexpr size_t NOALLOC_TW_ENTRIES = 32 * ((1 << NOALLOC_NK) - 1);  // sum of N over 32..2048
 constexpr size_t NOALLOC_TW64_BYTES = (16 * NOALLOC_TW_ENTRIES + 255) & ~(size_t)255;
 constexpr size_t NOALLOC_TW32_BYTES = (8 * NOALLOC_TW_ENTRIES + 255) & ~(size_t)255;
-constexpr size_t NOALLOC_TW_BYTES = NOALLOC_TW64_BYTES + NOALLOC_TW32_BYTES;
+constexpr size_t NOALLOC_DFT_BYTES = (size_t)DFT_TABLE_ROWS * 64 * 16;
+constexpr size_t NOALLOC_TW_BYTES = NOALLOC_TW64_BYTES + NOALLOC_TW32_BYTES + NOALLOC_DFT_BYTES;
 constexpr size_t NOALLOC_BYTES = (size_t)24 << 20;
 constexpr size_t NOALLOC_WS_BYTES = NOALLOC_BYTES - NOALLOC_TW_BYTES;
 static_assert(sizeof(Pack<F64>) * (size_t)GBGPU_NOALLOC_MAX_BIN + ((size_t)2 << 20) < NOALLOC_WS_BYTES,
@@ -415,8 +494,13 @@ int noalloc_base(char **out) {
         host_twiddles_f64(N, t64.data() + tw_offset(N));
         host_twiddles_f32(N, t32.data() + tw_offset(N));
     }
+    std::vector<uint32_t> dft(NOALLOC_DFT_BYTES / 4, 0u);
+    host_dft256_table(dft.data());
     e = hipMemcpy(ptr, t64.data(), 16 * NOALLOC_TW_ENTRIES, hipMemcpyHostToDevice);
     if (e == hipSuccess) e = hipMemcpy(ptr + NOALLOC_TW64_BYTES, t32.data(), 8 * NOALLOC_TW_ENTRIES, hipMemcpyHostToDevice);
+    if (e == hipSuccess)
+        e = hipMemcpy(ptr + NOALLOC_TW64_BYTES + NOALLOC_TW32_BYTES, dft.data(), NOALLOC_DFT_BYTES,
+                      hipMemcpyHostToDevice);
     if (e != hipSuccess) return hip_fail(e, "hipMemcpy(twiddles to scratch)");
     g_noalloc_base[dev] = ptr;
     *out = ptr;
@@ -443,8 +527,9 @@ extern "C" int gbgpu_run_wave(gbgpu_cmplx *XYZf, gbgpu_cmplx *AETf, long long *s
     a.start_inds = start_inds;
     a.tw = twiddles_for(cfg->N, cfg->precision);
     if (!a.tw) return hip_fail(hipErrorOutOfMemory, "twiddles");
+    if ((rc = set_transform(cfg, a))) return rc;
     return launch_any<MODE_WAVE>(cfg->N, num_params == GBGPU_NPARAMS_THIRD, cfg->precision, a,
-                                 static_cast<hipStream_t>(stream));
+                              static_cast<hipStream_t>(stream), nullptr, 0, cfg->transform);
 }
 
 extern "C" int gbgpu_run_wave_ae(gbgpu_cmplx *AEf, long long *start_inds, const double *params, int num_params,
@@ -461,8 +546,9 @@ extern "C" int gbgpu_run_wave_ae(gbgpu_cmplx *AEf, long long *start_inds, const 
     a.start_inds = start_inds;
     a.tw = twiddles_for(cfg->N, cfg->precision);
     if (!a.tw) return hip_fail(hipErrorOutOfMemory, "twiddles");
+    if ((rc = set_transform(cfg, a))) return rc;
     return launch_any<MODE_AE>(cfg->N, num_params == GBGPU_NPARAMS_THIRD, cfg->precision, a,
-                               static_cast<hipStream_t>(stream));
+                              static_cast<hipStream_t>(stream), nullptr, 0, cfg->transform);
 }
 
 extern "C" int gbgpu_wave_ll(double *loglike, gbgpu_cmplx *d_h, gbgpu_cmplx *h_h, const double *params,
@@ -485,8 +571,9 @@ extern "C" int gbgpu_wave_ll(double *loglike, gbgpu_cmplx *d_h, gbgpu_cmplx *h_h
     a.h_h = reinterpret_cast<double2 *>(h_h);
     a.tw = twiddles_for(cfg->N, cfg->precision);
     if (!a.tw) return hip_fail(hipErrorOutOfMemory, "twiddles");
+    if ((rc = set_transform(cfg, a))) return rc;
     return launch_any<MODE_LL>(cfg->N, num_params == GBGPU_NPARAMS_THIRD, cfg->precision, a,
-                               static_cast<hipStream_t>(stream));
+                              static_cast<hipStream_t>(stream), nullptr, 0, cfg->transform);
 }
 
 extern "C" int gbgpu_wave_ll_noalloc(double *loglike, gbgpu_cmplx *d_h, gbgpu_cmplx *h_h, const double *params,
@@ -517,6 +604,8 @@ extern "C" int gbgpu_wave_ll_noalloc(double *loglike, gbgpu_cmplx *d_h, gbgpu_cm
     a.tw = cfg->precision == GBGPU_MIXED ? (const void *)(reinterpret_cast<const float2 *>(base + NOALLOC_TW64_BYTES) +
                                                          tw_offset(cfg->N))
                                          : (const void *)(reinterpret_cast<const double2 *>(base) + tw_offset(cfg->N));
+    a.dftc = base + NOALLOC_TW64_BYTES + NOALLOC_TW32_BYTES;
     return launch_any<MODE_LL>(cfg->N, num_params == GBGPU_NPARAMS_THIRD, cfg->precision, a,
-                               static_cast<hipStream_t>(stream), base + NOALLOC_TW_BYTES, NOALLOC_WS_BYTES);
+                               static_cast<hipStream_t>(stream), base + NOALLOC_TW_BYTES, NOALLOC_WS_BYTES,
+                               cfg->transform);
 }

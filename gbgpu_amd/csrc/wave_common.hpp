@@ -6,6 +6,7 @@
 #include <type_traits>
 
 #include "common.hpp"
+#include "dft_mfma.hpp"
 #include "fft_lds.hpp"
 #include "prec.hpp"
 #include "source.hpp"
@@ -41,6 +42,11 @@ constexpr int wave_tile() {
     return N / 4 > WG_SIZE ? 0 : 1;
 }
 
+// Per-bucket kernel variants of the batched path: 0 and 1 are tile shapes;
+// VARIANT_DFT0 + t - 1 runs the mixed N = 256 bucket with matrix-core
+// transform t (GBGPU_TRANSFORM_MFMA_*).
+constexpr int VARIANT_DFT0 = 2;
+
 struct WaveArgs {
     const double *params;
     int num_params;
@@ -75,6 +81,7 @@ struct WaveArgs {
     double redo_tol;
     unsigned *keys;    // k_setup output: start-bin sort keys, or null
     int *vals;         // k_setup output: binary indices, or null
+    const void *dftc;  // per-lane constants of the matrix-core transform (dft256_table)
 };
 
 // Bound on |logL_mixed - logL_FP64| / (<h|h> + |<d|h>|) for the mixed
@@ -252,14 +259,24 @@ struct ThirdScratch {
 // samples stay in its registers; only the predecessor of its first sample
 // comes from the previous lane (a shuffle) or the previous wavefront
 // (ThirdScratch).
-template <int N, int NCH, bool THIRD, class P, int M = 1>
+//
+// DFT != GBGPU_TRANSFORM_LDS (mixed policy, N = 256, one wavefront per
+// binary, no third body): each thread evaluates the samples the transform's
+// register layout gives it, xr[c][i] holding sample dft256_index(lane, i) of
+// channel c, and dft256_mfma transforms them in place (same index on exit);
+// buf and tw are not touched.
+template <int N, int NCH, bool THIRD, class P, int M = 1, int DFT = GBGPU_TRANSFORM_LDS>
 __device__ __forceinline__ void binary_spectrum(const Src &s, const typename P::Aux &aux, bool valid, int lane,
                                                 const double *__restrict__ tab, double fstar_inv, double scale,
                                                 typename P::CT *buf, const typename P::CT *__restrict__ tw,
-                                                ThirdScratch ts, double *tot) {
+                                                ThirdScratch ts, double *tot, typename P::CT *xr = nullptr,
+                                                const void *dftc = nullptr) {
     using G = Geo<N, M>;
     using CT = typename P::CT;
     constexpr int TPB = G::TPB, SPT = G::SPT;
+    static_assert(DFT == GBGPU_TRANSFORM_LDS ||
+                      (N == 256 && TPB == 64 && !THIRD && std::is_same<CT, float2>::value),
+                  "the matrix-core transform covers mixed N = 256 without a third body");
     constexpr int NT = THIRD ? SPT : 1;
     float vf[3][NT];   // v_LOS at xi_sc(t_n)
     double gc[3][NT];  // the cumulative phase term at t_n
@@ -348,7 +365,7 @@ __device__ __forceinline__ void binary_spectrum(const Src &s, const typename P::
         for (int q = 0; q < SPT; q++) {
             // third body: the thread's own consecutive samples (pre-pass values in
             // registers); otherwise strided, so orbit-table loads coalesce
-            const int n = THIRD ? SPT * lane + q : lane + q * TPB;
+            const int n = THIRD ? SPT * lane + q : (DFT != GBGPU_TRANSFORM_LDS ? dft256_index(lane, q) : lane + q * TPB);
             CT X{}, Y{}, Z{}, F{};
             if (valid) {
                 float v3[3] = {0.f, 0.f, 0.f};
@@ -362,21 +379,35 @@ __device__ __forceinline__ void binary_spectrum(const Src &s, const typename P::
                 }
                 P::template sample<THIRD, TIER>(s, aux, tab, N, n, fstar_inv, scale, v3, t3, X, Y, Z, F);
             }
+            CT out[NCH];
             if constexpr (NCH == 3) {
-                buf[n] = cmul(F, X);
-                buf[N + n] = cmul(F, Y);
-                buf[2 * N + n] = cmul(F, Z);
+                out[0] = cmul(F, X);
+                out[1] = cmul(F, Y);
+                out[2] = cmul(F, Z);
             } else {
                 // A = (Z - X)/sqrt2, E = (X - 2Y + Z)/sqrt6 (utility.py:34-38); linear,
                 // so it commutes with the transform, and X, Y, Z share the factor F.
-                buf[n] = cmul(cscale(F, (R)INV_SQRT2), csub(Z, X));
-                buf[N + n] = cmul(cscale(F, (R)INV_SQRT6), cadd(csub(X, cscale(Y, (R)2)), Z));
+                out[0] = cmul(cscale(F, (R)INV_SQRT2), csub(Z, X));
+                out[1] = cmul(cscale(F, (R)INV_SQRT6), cadd(csub(X, cscale(Y, (R)2)), Z));
+            }
+#pragma unroll
+            for (int c = 0; c < NCH; c++) {
+                if constexpr (DFT != GBGPU_TRANSFORM_LDS)
+                    xr[c * SPT + q] = out[c];
+                else
+                    buf[c * N + n] = out[c];
             }
         }
     };
     with_tier(s.tier, samples);
-    __syncthreads();
-    fft_lds<N, NCH, TPB>(buf, lane, tw);
+    if constexpr (DFT != GBGPU_TRANSFORM_LDS) {
+        const Dft256Lane<DFT> k = load_dft256<DFT>(dftc, lane);
+#pragma unroll
+        for (int c = 0; c < NCH; c++) dft256_mfma<DFT>(xr + c * SPT, k);
+    } else {
+        __syncthreads();
+        fft_lds<N, NCH, TPB>(buf, lane, tw);
+    }
 }
 
 // Accumulates this thread's share of <d|h> (complex) and <h|h> for a binary
@@ -399,6 +430,31 @@ __device__ __forceinline__ void ll_partial(const typename P::CT *buf, double2 fa
         const double2 dA = a.dA[doff + j], dE = a.dE[doff + j];
         const double iA = rcp_f64(a.SA[noff + j]), iE = rcp_f64(a.SE[noff + j]);
         // conj(d) x / S
+        xr += (dA.x * hA.x + dA.y * hA.y) * iA + (dE.x * hE.x + dE.y * hE.y) * iE;
+        xi += (dA.x * hA.y - dA.y * hA.x) * iA + (dE.x * hE.y - dE.y * hE.x) * iE;
+        xx += (hA.x * hA.x + hA.y * hA.y) * iA + (hE.x * hE.x + hE.y * hE.y) * iE;
+    }
+    dhr += fac.x * xr - fac.y * xi;
+    dhi += fac.x * xi + fac.y * xr;
+    hh += (fac.x * fac.x + fac.y * fac.y) * xx;
+}
+
+// ll_partial for spectra held in registers by the matrix-core transform:
+// h[c * 4 + i] is bin dft256_index(lane, i) of channel c (A, E).
+template <class P>
+__device__ __forceinline__ void ll_partial_regs(const typename P::CT *h, double2 fac, int lane, long long j0,
+                                                long long doff, long long noff, const WaveArgs &a, double &dhr,
+                                                double &dhi, double &hh) {
+    constexpr int N = 256;
+    double xr = 0.0, xi = 0.0, xx = 0.0;
+#pragma unroll
+    for (int i = 0; i < 4; i++) {
+        const int m = (dft256_index(lane, i) + N / 2) & (N - 1);  // fftshifted position
+        const long long j = j0 + m;
+        if (j < 0 || j >= a.L) continue;
+        const double2 hA = P::up(h[i]), hE = P::up(h[4 + i]);
+        const double2 dA = a.dA[doff + j], dE = a.dE[doff + j];
+        const double iA = rcp_f64(a.SA[noff + j]), iE = rcp_f64(a.SE[noff + j]);
         xr += (dA.x * hA.x + dA.y * hA.y) * iA + (dE.x * hE.x + dE.y * hE.y) * iE;
         xi += (dA.x * hA.y - dA.y * hA.x) * iA + (dE.x * hE.y - dE.y * hE.x) * iE;
         xx += (hA.x * hA.x + hA.y * hA.y) * iA + (hE.x * hE.x + hE.y * hE.y) * iE;

@@ -47,11 +47,12 @@ class GBGPUBase(abc.ABC):
     # cannot run on the GPU path
     _kernel_variant = "circular"
 
-    def __init__(self, orbits: Orbits = None, force_backend=None, precision: str = "fp64"):
+    def __init__(self, orbits: Orbits = None, force_backend=None, precision: str = "fp64", transform: str = "lds"):
         self.backend = get_backend(force_backend)
         self.d_d = None
         self.orbits = orbits
         self.precision = precision
+        self.transform = transform
         self._tables = {}
         self._wave_state = None  # (params tensor, N, T, tdi2, precision) for lazy templates
         self._XYZf = None
@@ -89,6 +90,28 @@ class GBGPUBase(abc.ABC):
         self._tables = {}
 
     @property
+    def transform(self):
+        """Transform of the slow part for mixed-precision N = 256 calls of
+        run_wave, run_wave_ae-style templates and get_ll without a third body:
+        "lds" (radix-4 transform in LDS, default), "mfma_f32", "mfma_bf16x3"
+        or "mfma_bf16x6" (two 16 x 16 complex matrix products on the matrix
+        cores; bf16x3 carries about 16 significant bits, the others FP32
+        accuracy).  Every other call uses "lds"."""
+        return self._transform
+
+    @transform.setter
+    def transform(self, t):
+        if t not in _capi.TRANSFORMS:
+            raise ValueError(f"transform must be one of {list(_capi.TRANSFORMS)}")
+        self._transform = t
+
+    def _transform_code(self, N, precision, num_params):
+        p = precision or self.precision
+        if p == "mixed" and int(N) == 256 and num_params == 9:
+            return _capi.TRANSFORMS[self._transform]
+        return 0
+
+    @property
     def precision(self):
         return self._precision
 
@@ -114,7 +137,7 @@ class GBGPUBase(abc.ABC):
             self._tables[key] = t
         return t
 
-    def _cfg(self, N, T, tdi2, precision):
+    def _cfg(self, N, T, tdi2, precision, transform=0):
         if N & (N - 1) or not (32 <= N <= 2048):
             raise ValueError(f"N={N}: the HIP kernels support powers of two in [32, 2048]")
         tab = self._table(T, N)
@@ -125,6 +148,7 @@ class GBGPUBase(abc.ABC):
             orbit_table=tab.data_ptr(),
             tdi2=int(bool(tdi2)),
             precision=_PRECISIONS[precision or self.precision],
+            transform=int(transform),
         )
         return cfg, tab
 
@@ -178,7 +202,7 @@ class GBGPUBase(abc.ABC):
         self.num_bin = B
         self.N = N
         self.df = 1 / T
-        cfg, _tab = self._cfg(N, T, tdi2, precision)
+        cfg, _tab = self._cfg(N, T, tdi2, precision, self._transform_code(N, precision, params.shape[0]))
         dev = params.device
         XYZf = torch.empty((B, 3, N), dtype=torch.complex128, device=dev)
         AETf = torch.empty((B, 3, N), dtype=torch.complex128, device=dev)
@@ -348,7 +372,7 @@ class GBGPUBase(abc.ABC):
         B = p.shape[1]
         self.num_bin, self.N, self.df = B, N, 1 / T
         dA, dE, SA, SE, di, ni, L = self._prep_data(data, psd, data_index, noise_index, B)
-        cfg, _tab = self._cfg(N, T, tdi2, precision)
+        cfg, _tab = self._cfg(N, T, tdi2, precision, self._transform_code(N, precision, p.shape[0]))
         dev = p.device
         ll = torch.empty(B, dtype=torch.float64, device=dev)
         d_h = torch.empty(B, dtype=torch.complex128, device=dev)
@@ -538,7 +562,7 @@ class GBGPUBase(abc.ABC):
         p, N, T = self._prepare(*params[:9], params[9:], N, T, dt, oversample)
         B = p.shape[1]
         self.num_bin, self.N, self.df = B, N, 1 / T
-        cfg, _tab = self._cfg(N, T, tdi2, precision)
+        cfg, _tab = self._cfg(N, T, tdi2, precision, self._transform_code(N, precision, p.shape[0]))
         AE = torch.empty((B, 2, N), dtype=torch.complex128, device=p.device)
         start = torch.empty((B,), dtype=torch.int64, device=p.device)
         rc = _capi.load().gbgpu_run_wave_ae(AE.data_ptr(), start.data_ptr(), p.data_ptr(), p.shape[0], B, cfg,

@@ -33,6 +33,22 @@ enum {
 
 enum { GBGPU_FP64 = 0, GBGPU_MIXED = 1 };
 
+/* Length-N transform of the slow part (the reference's xp.fft.fft,
+ * src/gbgpu/gbgpu.py:378).  GBGPU_TRANSFORM_LDS: radix-4 Stockham transform
+ * in LDS, every N and precision.  The MFMA variants evaluate the transform as
+ * two 16 x 16 complex matrix products on the matrix cores, in registers, and
+ * exist for GBGPU_MIXED at N = 256 without a third body only (other
+ * combinations return GBGPU_E_ARG): _F32 with FP32 matrix instructions,
+ * _BF16X3 / _BF16X6 with bf16 matrix instructions on a two-piece (three
+ * products, about 16 significant bits) or three-piece (six products, FP32
+ * level) split of each operand. */
+enum {
+    GBGPU_TRANSFORM_LDS = 0,
+    GBGPU_TRANSFORM_MFMA_F32 = 1,
+    GBGPU_TRANSFORM_MFMA_BF16X3 = 2,
+    GBGPU_TRANSFORM_MFMA_BF16X6 = 3
+};
+
 /* Parameter rows of the (num_params, num_bin) float64 parameter array:
  * amp, f0, fdot, fddot, phi0, iota, psi, lam, beta   (src/gbgpu/gbgpu.py:108-120)
  * and, when num_params == 14, the third-body rows A2, varpi, e2, n2, T2 with
@@ -51,6 +67,9 @@ typedef struct gbgpu_wave_cfg {
                                   for s = 1..3 (xyz), arms r12, r13, r23 (xyz) */
     int tdi2;                  /* 1: multiply by 2i sin(2wL) e^{-2iwL} (gbgpu.py:301-304) */
     int precision;             /* GBGPU_FP64 or GBGPU_MIXED */
+    int transform;             /* GBGPU_TRANSFORM_*; 0 (LDS) when zero-initialised.  Only
+                                  gbgpu_run_wave, gbgpu_run_wave_ae, gbgpu_wave_ll and
+                                  gbgpu_wave_ll_noalloc accept a non-zero value */
 } gbgpu_wave_cfg;
 
 /* Data against which templates are scored (get_ll's data/psd arguments,
@@ -212,6 +231,8 @@ typedef struct gbgpu_batch_cfg {
                                       an estimate of <h|h> from amplitude, frequency and the PSD
                                       already exceeds the bound, else re-evaluated after the mixed
                                       pass when its own result does; <= 0 disables */
+    int transform;                 /* GBGPU_TRANSFORM_* for the mixed N = 256 bucket (every
+                                      other bucket and every FP64 pass uses the LDS transform) */
 } gbgpu_batch_cfg;
 
 /* Workspace bytes gbgpu_wave_ll_batched needs for num_bin binaries. */
@@ -236,10 +257,12 @@ int gbgpu_wave_ll_batched(double *loglike, gbgpu_cmplx *d_h, gbgpu_cmplx *h_h, c
                           void *stream);
 
 /* Autotuner of the per-bucket launches (GBGPU_BATCH_BUCKETS): runs the batch
- * with each tile shape (0: N/4 threads per binary, in workgroups of
+ * with each kernel variant (0: N/4 threads per binary, in workgroups of
  * max(N/4, 256) threads; 1: half as many threads per binary, twice the
- * binaries per workgroup) three times, times
- * every bucket with HIP events, and keeps the faster shape per (precision,
+ * binaries per workgroup; for the GBGPU_MIXED N = 256 bucket, when
+ * cfg->transform is GBGPU_TRANSFORM_LDS, also 2 and 4: the
+ * GBGPU_TRANSFORM_MFMA_F32 and _BF16X6 transforms) three times, times
+ * every bucket with HIP events, and keeps the fastest variant per (precision,
  * bucket) for all later calls in the process.  chosen (7 ints, N = 32 << k,
  * may be NULL) receives the pick, -1 for buckets without an orbit table.
  * loglike receives the last run's results.  Synchronises `stream`; call it
@@ -248,8 +271,10 @@ int gbgpu_batch_autotune(double *loglike, const double *params, int num_params, 
                          const gbgpu_batch_cfg *cfg, const gbgpu_data *data, double d_d, void *workspace,
                          int *chosen, void *stream);
 
-/* Sets the per-bucket tile shapes (7 ints, 0 or 1 as for the autotuner) used
- * by GBGPU_BATCH_BUCKETS calls of the given precision. */
+/* Sets the per-bucket variants (7 ints: 0 or 1 as for the autotuner; for the
+ * GBGPU_MIXED N = 256 bucket also 1 + t, the matrix-core transform t =
+ * GBGPU_TRANSFORM_MFMA_*) used by GBGPU_BATCH_BUCKETS calls of the given
+ * precision. */
 int gbgpu_batch_set_variants(int precision, const int *variants);
 
 /* Per-bucket counts (7 ints, N = 32 << k) of binaries the accuracy guard
@@ -272,7 +297,7 @@ int gbgpu_perturb_params(double *out, const double *in, int num_params, int num_
 /* Message describing the last failure on the calling thread. */
 const char *gbgpu_last_error(void);
 
-/* ABI version of this header (4). */
+/* ABI version of this header (5). */
 int gbgpu_abi_version(void);
 
 /* Measurement hook with no counterpart in the reference: while enabled,

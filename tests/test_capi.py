@@ -32,7 +32,7 @@ def test_library_exports_every_declared_symbol():
         assert hasattr(lib, n), n
     # the ctypes table covers exactly the declared functions
     assert set(_capi.SIGNATURES) == set(declared_functions())
-    assert lib.gbgpu_abi_version() == 4
+    assert lib.gbgpu_abi_version() == 5
 
 
 def test_invalid_arguments_return_errors_without_gpu_work():
@@ -51,6 +51,33 @@ def test_invalid_arguments_return_errors_without_gpu_work():
     assert rc == -1
     # zero-size batches are no-ops
     assert lib.gbgpu_get_ll(None, None, None, None, None, None, None, None, 1.0, None, 4, 0, None, None, 10, None) == 0
+
+
+@pytest.mark.parametrize("precision,N,num_params,transform,ok", [
+    (0, 256, 9, 1, False),   # matrix-core transforms are mixed-precision only
+    (1, 128, 9, 2, False),   # ... at N = 256 only
+    (1, 256, 14, 3, False),  # ... without a third body
+    (1, 256, 9, 4, False),   # unknown transform
+    (1, 256, 9, -1, False),
+    (1, 256, 9, 3, True),
+])
+def test_transform_combinations_checked_before_gpu_work(precision, N, num_params, transform, ok):
+    """Unsupported transform combinations return GBGPU_E_ARG before any HIP
+    call; a supported one passes the checks (num_bin = 0: nothing runs)."""
+    from gbgpu_amd import _capi
+
+    lib = _capi.load()
+    cfg = _capi.WaveCfg(N=N, T=1e7, armlength=2.5e9, orbit_table=1, tdi2=0, precision=precision,
+                        transform=transform)
+    rc = lib.gbgpu_run_wave(1, 1, 1, 1, num_params, 0 if ok else 4, cfg, None)
+    if ok:
+        assert rc == 0
+    else:
+        assert rc == -1 and b"transform" in lib.gbgpu_last_error()
+    # swap and derivatives take the LDS transform only
+    if ok:
+        rc = lib.gbgpu_swap_ll_diff(1, None, 1, 9, 4, cfg, None, None)
+        assert rc == -1 and b"transform" in lib.gbgpu_last_error()
 
 
 def test_empty_batches_are_noops_on_every_entry_point():

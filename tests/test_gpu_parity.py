@@ -696,7 +696,9 @@ def test_mixed_batch_accuracy_guard():
 @pytest.mark.parametrize("precision", ["fp64", "mixed"])
 def test_batch_autotune_shapes_agree(precision):
     """Both tile shapes give the same log-likelihoods (to summation order);
-    the autotuner's pick is reproducible in its results."""
+    the autotuner's pick is reproducible in its results.  A matrix-core
+    transform picked for the mixed N = 256 bucket rounds differently: it is
+    held to the mixed-precision bar against the LDS transform instead."""
     from gbgpu_amd.batch import BatchedLogLike
 
     B = 20_000
@@ -721,8 +723,14 @@ def test_batch_autotune_shapes_agree(precision):
     scale = np.abs(out[0]) + 1.0
     assert np.max(np.abs(out[1] - out[0]) / scale) <= 1e-12
     assert set(chosen) == set(int(n) for n in np.unique(N))
+    assert all(c in (0, 1) or (precision == "mixed" and n == 256 and c in (2, 4)) for n, c in chosen.items())
     ref = np.where(np.isin(N, [n for n, c in chosen.items() if c == 1]), out[1], out[0])
-    assert np.max(np.abs(tuned - ref) / scale) <= 1e-12
+    tr = np.isin(N, [n for n, c in chosen.items() if c >= 2])
+    assert np.max(np.abs(tuned - ref)[~tr] / scale[~tr], initial=0.0) <= 1e-12
+    if tr.any():
+        _, d_h, h_h = bl(p, N=N, d_h=True)
+        bound = 2e-6 * (np.abs(h_h.cpu().numpy()) + np.abs(d_h.cpu().numpy())) + 2e-6
+        assert np.all(np.abs(tuned - ref)[tr] <= bound[tr])
 
 
 @pytest.mark.parametrize("strategy", ["buckets", pytest.param("persistent", marks=pytest.mark.skip(reason=PERSISTENT_OFF))])
