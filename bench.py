@@ -538,8 +538,20 @@ def global_and_swap(gb, SA, SE, T, dt, B=10_000_000, P=100_000, chunk_samples=1 
     return res
 
 
-K_FP64 = "k_wave<256, 1, false, gbgpu::F64>"
-K_MIXED = "k_wave<256, 1, false, gbgpu::Mixed>"
+# kernel-name prefixes of the fused likelihood at N = 256 with the LDS
+# transform; the template's trailing arguments (transform variant) follow
+K_FP64 = "k_wave<256, 1, false, gbgpu::F64"
+K_MIXED = "k_wave<256, 1, false, gbgpu::Mixed"
+
+
+def _is_kernel(full_name, prefix):
+    """full_name is the launch of `prefix` with the LDS transform (its last
+    template argument, if any, is 0)."""
+    i = full_name.find(prefix)
+    if i < 0:
+        return False
+    rest = full_name[i + len(prefix):]
+    return rest.startswith(">") or rest.startswith(", 0>")
 
 
 def counters_child(args):
@@ -622,7 +634,9 @@ def measure_counters(num_bin):
         return max(cand)
 
     def kernel_mean(rows, name, counter):
-        v = [c[counter] for (_d, k), c in rows.items() if name in k and counter in c]
+        v = [c[counter] for (_d, k), c in rows.items() if _is_kernel(k, name) and counter in c]
+        if not v:
+            raise RuntimeError(f"no {counter} rows for {name}")
         return sum(v) / len(v)
 
     f_copy = copy_value(fetch, "FETCH_SIZE") * 1024.0
