@@ -453,6 +453,10 @@ static int check_deriv_cfg(const double *params, int num_params, int num_bin, co
         set_error("derivatives use the LDS transform only");
         return GBGPU_E_ARG;
     }
+    if (cfg->N > 2048) {
+        set_error("derivatives support N <= 2048 (Fisher matrices N <= 1024)");
+        return GBGPU_E_ARG;
+    }
     return GBGPU_OK;
 }
 

@@ -204,6 +204,10 @@ extern "C" int gbgpu_swap_ll_diff(double *dll, double *terms, const double *para
         set_error("gbgpu_swap_ll_diff uses the LDS transform only");
         return GBGPU_E_ARG;
     }
+    if (cfg->N > 2048) {
+        set_error("gbgpu_swap_ll_diff supports N <= 2048");
+        return GBGPU_E_ARG;
+    }
     if ((rc = check_data(data))) return rc;
     if (!dll) {
         set_error("dll must not be null");

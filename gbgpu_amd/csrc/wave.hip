@@ -2,12 +2,15 @@
 // the fused waveform + likelihood, for both precision policies (prec.hpp).
 //
 // Work decomposition: a group of TPB = min(N/4, 256) threads owns one binary
-// and a 256-thread workgroup holds BPW = 256/TPB binaries.  Each thread
-// evaluates the slow part at SPT = N/TPB samples straight into LDS, the group
-// runs the length-N transform in LDS, and the epilogue either streams the
-// fftshifted, scaled spectra to HBM (run_wave) or reads the binary's
-// data/PSD window with coalesced loads and reduces <d|h>, <h|h> over the
-// group (wave_ll).
+// and a 256-thread workgroup holds BPW = 256/TPB binaries (N = 2048: one
+// 512-thread workgroup per binary).  Each thread evaluates the slow part at
+// SPT = N/TPB samples straight into LDS, the group runs the length-N
+// transform in LDS, and the epilogue either streams the fftshifted, scaled
+// spectra to HBM (run_wave) or reads the binary's data/PSD window with
+// coalesced loads and reduces <d|h>, <h|h> over the group (wave_ll); one
+// workgroup pass is wave_pass (wave_pass.hpp).  N = 4096 and 8192 run the
+// large-N kernels of wave_big.hip; the batch segment kernels are in
+// wave_seg.hip.
 #include <hipcub/hipcub.hpp>
 #include <map>
 #include <mutex>
@@ -17,186 +20,18 @@
 #include "common.hpp"
 #include "dispatch.hpp"
 #include "wave_common.hpp"
+#include "wave_pass.hpp"
 
 namespace gbgpu {
 
-
-// LDS of one workgroup of k_wave.  PK is the precision policy whose Pack
-// layout the per-binary constants were written in (normally P; the FP64
-// re-evaluation of guarded mixed binaries reads the mixed packs).
-// Arrays a variant does not use have length zero, so that the transform
-// buffers alone set the footprint (32 KB for the FP64 likelihood at N = 256:
-// five workgroups per CU).
-// With the matrix-core transform (DFT != 0) the samples and spectra stay in
-// registers and the workgroup needs no LDS.
-template <int N, int MODE, bool THIRD, class P, class PK = P, int M = 1, int DFT = GBGPU_TRANSFORM_LDS>
-struct WaveLds {
-    using G = Geo<N, M>;
-    using CT = typename P::CT;
-    static constexpr int NCH = (MODE == MODE_WAVE) ? 3 : 2;
-    // a wavefront that owns its binary reads the Pack straight into scalar registers
-    static constexpr bool PACK_IN_LDS = G::TPB < 64;
-    static constexpr int NEDGE = (THIRD && G::WPB > 1) ? 6 * G::WPB : 0;
-    CT sbuf[DFT == GBGPU_TRANSFORM_LDS ? G::BPW * NCH * N : 0];
-    Pack<PK> spack[PACK_IN_LDS ? G::BPW : 0];
-    double stot[G::WPB > 1 ? G::BPW * G::WPB : 0];
-    double s_edge[G::BPW * NEDGE];
-};
-
-// One pass of a workgroup: items blk*BPW .. blk*BPW + BPW-1 of the
-// `count` items (item i is binary order[first + i], or i without an order).
-template <class P, class PK>
-__device__ __forceinline__ typename P::Aux aux_of(const Pack<PK> &pk) {
-    if constexpr (std::is_same<P, PK>::value)
-        return pk.aux;
-    else
-        return P::make_aux(pk.s);
-}
-
-template <int N, int MODE, bool THIRD, class P, class PK = P, int M = 1, int DFT = GBGPU_TRANSFORM_LDS>
-__device__ __forceinline__ void wave_pass(const WaveArgs &a, WaveLds<N, MODE, THIRD, P, PK, M, DFT> &L,
-                                          long long blk, long long count, long long first, int tid,
-                                          const double *tab, const void *twp) {
-    using G = Geo<N, M>;
-    using W = WaveLds<N, MODE, THIRD, P, PK, M, DFT>;
-    using CT = typename P::CT;
-    constexpr int TPB = G::TPB, BPW = G::BPW, WPB = G::WPB, SPT = G::SPT;
-    constexpr int NCH = W::NCH;
-    CT *sbuf = L.sbuf;
-    Pack<PK> *spack = L.spack;
-    double *stot = L.stot;
-    const int slot = per_group<TPB>(tid / TPB);
-    const int lane = tid % TPB;
-    const long long item = blk * BPW + slot;
-    const bool valid = item < count;
-    const long long b = (valid && a.order) ? (long long)a.order[first + item] : item;
-
-    const Pack<PK> *pk;
-    if constexpr (W::PACK_IN_LDS) {
-        if (valid) load_pack<PK, TPB>(a, b, lane, spack + slot);
-        __syncthreads();
-        pk = spack + slot;
-    } else {
-        // wave-uniform index; an empty slot reads binary 0 and discards it
-        pk = static_cast<const Pack<PK> *>(a.pack) + (valid ? b : 0);
-    }
-    const Src s = per_group<TPB>(pk->s);
-    CT *buf = sbuf + slot * NCH * N;
-    double *tot = stot + slot * WPB;
-    ThirdScratch ts{L.s_edge + slot * W::NEDGE};
-
-    constexpr bool REG = DFT != GBGPU_TRANSFORM_LDS;
-    CT xr[REG ? NCH * SPT : 1];  // register-resident spectra (matrix-core transform)
-    binary_spectrum<N, NCH, THIRD, P, M, DFT>(s, per_group<TPB>(aux_of<P, PK>(*pk)), valid, lane, tab,
-                                              a.fstar_inv, a.scale, buf, static_cast<const CT *>(twp), ts, tot,
-                                              xr, a.dftc);
-    // bin of output position q of this thread: fftshifted position m(q),
-    // spectrum value spec(c, q) of channel c
-    auto pos = [&](int q) { return REG ? (dft256_index(lane, q) + N / 2) & (N - 1) : lane + q * TPB; };
-    auto spec = [&](int c, int q) {
-        if constexpr (REG)
-            return xr[c * SPT + q];
-        else
-            return buf[c * N + ((lane + q * TPB + N / 2) & (N - 1))];  // fftshift
-    };
-
-    const double2 fac = P::fac(s);
-    if constexpr (MODE == MODE_WAVE) {
-        if (valid) {
-            double2 *xyz = a.XYZf ? a.XYZf + b * 3 * N : nullptr;
-            double2 *aet = a.AETf ? a.AETf + b * 3 * N : nullptr;
-#pragma unroll
-            for (int q = 0; q < SPT; q++) {
-                const int m = pos(q);
-                const double2 X = cmul(P::up(spec(0, q)), fac);
-                const double2 Y = cmul(P::up(spec(1, q)), fac);
-                const double2 Z = cmul(P::up(spec(2, q)), fac);
-                if (xyz) {
-                    xyz[m] = X;
-                    xyz[N + m] = Y;
-                    xyz[2 * N + m] = Z;
-                }
-                if (aet) {
-                    aet[m] = cscale(csub(Z, X), INV_SQRT2);
-                    aet[N + m] = cscale(cadd(csub(X, cscale(Y, 2.0)), Z), INV_SQRT6);
-                    // T = (X + Y + Z)/sqrt3 cancels to O((f/f*)^2) of X: from FP32
-                    // spectra it would carry errors far above its own size, so the
-                    // mixed policy marks it not-a-number rather than return it
-                    if constexpr (std::is_same<P, Mixed>::value)
-                        aet[2 * N + m] = make_double2(__builtin_nan(""), __builtin_nan(""));
-                    else
-                        aet[2 * N + m] = cscale(cadd(cadd(X, Y), Z), INV_SQRT3);
-                }
-            }
-            if (lane == 0) a.start_inds[b] = s.start;
-        }
-    } else if constexpr (MODE == MODE_AE) {
-        if (valid) {
-            double2 *ae = a.AETf + b * 2 * N;
-#pragma unroll
-            for (int q = 0; q < SPT; q++) {
-                const int m = pos(q);
-                ae[m] = cmul(P::up(spec(0, q)), fac);
-                ae[N + m] = cmul(P::up(spec(1, q)), fac);
-            }
-            if (lane == 0) a.start_inds[b] = s.start;
-        }
-    } else {
-        double dhr = 0.0, dhi = 0.0, hh = 0.0;
-        if (valid) {
-            const long long doff = (a.di ? (long long)a.di[b] : 0LL) * a.L;
-            const long long noff = (a.ni ? (long long)a.ni[b] : 0LL) * a.L;
-            if constexpr (REG)
-                ll_partial_regs<P>(xr, fac, lane, s.start - a.start_freq_ind, doff, noff, a, dhr, dhi, hh);
-            else
-                ll_partial<N, P, M>(buf, fac, lane, s.start - a.start_freq_ind, doff, noff, a, dhr, dhi, hh);
-        }
-        dhr = group_total<TPB>(dhr, lane, tot);
-        dhi = group_total<TPB>(dhi, lane, tot);
-        hh = group_total<TPB>(hh, lane, tot);
-        if (valid && lane == 0) {
-            dhr *= a.df4;
-            dhi *= a.df4;
-            hh *= a.df4;
-            if (a.d_h) a.d_h[b] = make_double2(dhr, dhi);
-            if (a.h_h) a.h_h[b] = make_double2(hh, 0.0);
-            const double dh_use = a.phase_marg ? sqrt(dhr * dhr + dhi * dhi) : dhr;
-            a.loglike[b] = -0.5 * (a.d_d + hh - 2.0 * dh_use);
-            if (a.redo_list && GUARD_REL * (fabs(hh) + sqrt(dhr * dhr + dhi * dhi)) > a.redo_tol) {
-                const int pos = atomicAdd(a.redo_count, 1);
-                a.redo_list[first + pos] = (int)b;
-            }
-        }
-    }
-}
+int launch_wave_big(int N, int mode, bool third, int precision, const WaveArgs &a, hipStream_t st, char *ws,
+                    size_t ws_bytes);
 
 template <int N, int MODE, bool THIRD, class P, int DFT = GBGPU_TRANSFORM_LDS>
 __global__ void __launch_bounds__((Geo<N, wave_tile<N>()>::WG)) k_wave(WaveArgs a) {
     constexpr int M = wave_tile<N>();
     __shared__ WaveLds<N, MODE, THIRD, P, P, M, DFT> lds;
     wave_pass<N, MODE, THIRD, P, P, M, DFT>(a, lds, blockIdx.x, a.B, 0, threadIdx.x, a.tab, a.tw);
-}
-
-// Fused likelihood over one device-side segment of a batch: item i <
-// *a.seg_count is binary a.order[*a.seg_off + i]; the grid strides over the
-// segment.  The orbit-table and twiddle reads and the per-thread index
-// arithmetic are the same in every pass: opaque copies of the pointers and
-// of the thread index keep the compiler from hoisting all of them out of the
-// loop into registers (which would cost occupancy).
-template <int N, class P, class PK, int M, int DFT = GBGPU_TRANSFORM_LDS>
-__global__ void __launch_bounds__((Geo<N, M>::WG)) k_wave_seg(WaveArgs a) {
-    constexpr int BPW = Geo<N, M>::BPW;
-    __shared__ WaveLds<N, MODE_LL, false, P, PK, M, DFT> lds;
-    const long long count = *a.seg_count;
-    const long long first = *a.seg_off;
-    for (long long blk = blockIdx.x; blk * BPW < count; blk += gridDim.x) {
-        const double *tab = a.tab;
-        const void *twp = a.tw;
-        int tid = threadIdx.x;
-        asm volatile("" : "+s"(tab), "+s"(twp), "+v"(tid));
-        wave_pass<N, MODE_LL, false, P, PK, M, DFT>(a, lds, blk, count, first, tid, tab, twp);
-        __syncthreads();  // LDS is reused by the next pass
-    }
 }
 
 // k_wave with the requested transform; the matrix-core variants exist for
@@ -253,85 +88,15 @@ static int launch_wave(int N, const WaveArgs &a, hipStream_t st, char *ws = null
         GBGPU_CASE(512)
         GBGPU_CASE(1024)
         GBGPU_CASE(2048)
+        case 4096:
+        case 8192:
+            return launch_wave_big(N, MODE, THIRD, std::is_same<P, Mixed>::value ? GBGPU_MIXED : GBGPU_FP64, a, st,
+                                   ws, ws_bytes);
         default:
-            set_error("N must be a power of two in [32, 2048]");
+            set_error("N must be a power of two in [32, 8192]");
             return GBGPU_E_ARG;
     }
 #undef GBGPU_CASE
-}
-
-// Likelihood kernel over one device-side segment of a batch (the N-bucket
-// launches of gbgpu_wave_ll_batched): a.pack, a.order, a.seg_count and
-// a.seg_off are set by the caller; the segment's size is known only on the
-// device, so the grid covers a few passes of resident workgroups and strides.
-template <class P, class PK, int NN, int M, int DFT = GBGPU_TRANSFORM_LDS>
-static int launch_segment_n(const WaveArgs &a, hipStream_t st) {
-    // several passes' worth of resident workgroups (initialised once, thread-safe)
-    static const int grid = [] {
-        int dev = 0, ncu = 0;
-        (void)hipGetDevice(&dev);
-        (void)hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev);
-        return 16 * (ncu > 0 ? ncu : 256);
-    }();
-    long long g = (a.B + Geo<NN, M>::BPW - 1) / Geo<NN, M>::BPW;
-    if (g > grid) g = grid;
-    KernelTimer kt(st);
-    hipLaunchKernelGGL((k_wave_seg<NN, P, PK, M, DFT>), dim3((unsigned)g), dim3(Geo<NN, M>::WG), 0, st, a);
-    GBGPU_CHECK_LAUNCH("k_wave_seg");
-    return GBGPU_OK;
-}
-
-// variant 0: the waveform kernels' tile (wave_tile); variant 1: half the
-// threads per binary of that tile; VARIANT_DFT0.. (mixed N = 256 only): the
-// matrix-core transforms, one wavefront per binary.
-template <class P, class PK, int NN>
-static int launch_segment_m(int variant, const WaveArgs &a, hipStream_t st) {
-    constexpr int M0 = wave_tile<NN>();
-    constexpr int M1 = M0 == 0 ? 1 : 2;
-    if constexpr (NN == 256 && std::is_same<P, Mixed>::value && std::is_same<PK, Mixed>::value) {
-        if (variant >= VARIANT_DFT0 && !a.dftc) {
-            set_error("matrix-core transform requested without its constant table");
-            return GBGPU_E_ARG;
-        }
-        switch (variant - VARIANT_DFT0 + 1) {
-            case GBGPU_TRANSFORM_MFMA_F32:
-                return launch_segment_n<P, PK, NN, 1, GBGPU_TRANSFORM_MFMA_F32>(a, st);
-            case GBGPU_TRANSFORM_MFMA_BF16X3:
-                return launch_segment_n<P, PK, NN, 1, GBGPU_TRANSFORM_MFMA_BF16X3>(a, st);
-            case GBGPU_TRANSFORM_MFMA_BF16X6:
-                return launch_segment_n<P, PK, NN, 1, GBGPU_TRANSFORM_MFMA_BF16X6>(a, st);
-            default:
-                break;
-        }
-    }
-    return variant == 1 ? launch_segment_n<P, PK, NN, M1>(a, st) : launch_segment_n<P, PK, NN, M0>(a, st);
-}
-
-// precision: arithmetic of the pass; pack_precision: layout of a.pack;
-// variant: tile shape (0: wave_tile, 1: half the threads per binary).
-int launch_ll_segment(int N, int precision, int pack_precision, int variant, const WaveArgs &a, hipStream_t st) {
-    if (precision == GBGPU_MIXED && pack_precision != GBGPU_MIXED) {
-        set_error("mixed segments need mixed per-binary constants");
-        return GBGPU_E_ARG;
-    }
-#define GBGPU_SEG(NN)                                                                           \
-    case NN:                                                                                    \
-        if (precision == GBGPU_MIXED) return launch_segment_m<Mixed, Mixed, NN>(variant, a, st); \
-        if (pack_precision == GBGPU_MIXED) return launch_segment_m<F64, Mixed, NN>(variant, a, st); \
-        return launch_segment_m<F64, F64, NN>(variant, a, st);
-    switch (N) {
-        GBGPU_SEG(32)
-        GBGPU_SEG(64)
-        GBGPU_SEG(128)
-        GBGPU_SEG(256)
-        GBGPU_SEG(512)
-        GBGPU_SEG(1024)
-        GBGPU_SEG(2048)
-        default:
-            set_error("N must be a power of two in [32, 2048]");
-            return GBGPU_E_ARG;
-    }
-#undef GBGPU_SEG
 }
 
 template <int MODE>
@@ -362,8 +127,8 @@ int check_cfg(const double *params, int num_params, int num_bin, const gbgpu_wav
         set_error("num_params must be 9 or 14");
         return GBGPU_E_ARG;
     }
-    if (num_bin < 0 || !is_pow2(cfg->N) || cfg->N < 32 || cfg->N > 2048) {
-        set_error("num_bin < 0 or N not a power of two in [32, 2048]");
+    if (num_bin < 0 || !is_pow2(cfg->N) || cfg->N < 32 || cfg->N > GBGPU_N_MAX) {
+        set_error("num_bin < 0 or N not a power of two in [32, 8192]");
         return GBGPU_E_ARG;
     }
     if (!(cfg->T > 0) || !(cfg->armlength > 0)) {
@@ -427,15 +192,15 @@ const void *twiddles_for(int N, int precision) {
 // ---- allocation-free fused likelihood (gbgpu_wave_ll_noalloc) ---------------
 //
 // `scratch` holds, at fixed offsets: the FP64 and FP32 twiddle tables of every
-// supported N (N = 32 << k, k < 7, back to back), the per-lane constants of
+// supported N (N = 32 << k, k < 9, back to back), the per-lane constants of
 // the matrix-core transforms (dft256_table's layout), then the per-binary
 // workspace of with_pack (Pack array, sort keys/values, radix-sort storage)
 // for up to GBGPU_NOALLOC_MAX_BIN binaries.  Each device holds its own copy
 // of the array for the life of the process.  In host code `scratch` names a
 // host-side stand-in: every host use goes through the device address that
 // hipGetSymbolAddress returns for the current device (noalloc_base).
-constexpr int NOALLOC_NK = 7;
-constexpr size_t NOALLOC_TW_ENTRIES = 32 * ((1 << NOALLOC_NK) - 1);  // sum of N over 32..2048
+constexpr int NOALLOC_NK = 9;
+constexpr size_t NOALLOC_TW_ENTRIES = 32 * ((1 << NOALLOC_NK) - 1);  // sum of N over 32..8192
 constexpr size_t NOALLOC_TW64_BYTES = (16 * NOALLOC_TW_ENTRIES + 255) & ~(size_t)255;
 constexpr size_t NOALLOC_TW32_BYTES = (8 * NOALLOC_TW_ENTRIES + 255) & ~(size_t)255;
 constexpr size_t NOALLOC_DFT_BYTES = (size_t)DFT_TABLE_ROWS * 64 * 16;

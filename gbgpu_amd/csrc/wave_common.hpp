@@ -23,13 +23,15 @@ constexpr int WG_SIZE = 256;
 // samples.  Tile shapes:
 //   M = 1: TPB = min(N/4, 256) in 256-thread workgroups (SPT = 4, 8 at N = 2048);
 //   M = 2: half the threads per binary, twice the binaries per workgroup;
-//   M = 0 ("wide"): TPB = N/4 (SPT = 4) in workgroups of max(N/4, 256)
+//   M = 0 ("wide"): TPB = min(N/4, 1024) in workgroups of max(TPB, 256)
 //          threads -- at N = 2048 a 512-thread workgroup per binary, twice
-//          the wavefronts per byte of LDS of M = 1.
+//          the wavefronts per byte of LDS of M = 1; at N = 4096 and 8192
+//          one 1024-thread workgroup per binary (SPT = 4 and 8).
 // Workgroups hold BPW binaries.
+constexpr int WG_MAX = 1024;
 template <int N, int M = 1>
 struct Geo {
-    static constexpr int TPB = M == 0 ? N / 4 : ((N / 4 < WG_SIZE) ? N / 4 : WG_SIZE) / M;
+    static constexpr int TPB = M == 0 ? (N / 4 > WG_MAX ? WG_MAX : N / 4) : ((N / 4 < WG_SIZE) ? N / 4 : WG_SIZE) / M;
     static constexpr int WG = TPB > WG_SIZE ? TPB : WG_SIZE;
     static constexpr int BPW = WG / TPB;
     static constexpr int SPT = N / TPB;
@@ -254,6 +256,11 @@ struct ThirdScratch {
 // X, Y, Z; NCH == 2 gives A, E.  Invalid slots (beyond the batch) write
 // zeros and still take part in every barrier.
 //
+// Channel passes (C0, NCP): only channels C0 .. C0 + NCP - 1 are stored and
+// transformed, buf then holding NCP x N values.  The large-N kernels call
+// this once per group of channels that fits their LDS, recomputing the slow
+// part (shared by all channels) in each pass.
+//
 // Each thread evaluates SPT samples: n = lane + q TPB, or with a third body
 // n = SPT lane + q, so that the pre-pass values of a thread's own consecutive
 // samples stay in its registers; only the predecessor of its first sample
@@ -265,7 +272,7 @@ struct ThirdScratch {
 // register layout gives it, xr[c][i] holding sample dft256_index(lane, i) of
 // channel c, and dft256_mfma transforms them in place (same index on exit);
 // buf and tw are not touched.
-template <int N, int NCH, bool THIRD, class P, int M = 1, int DFT = GBGPU_TRANSFORM_LDS>
+template <int N, int NCH, bool THIRD, class P, int M = 1, int DFT = GBGPU_TRANSFORM_LDS, int C0 = 0, int NCP = NCH>
 __device__ __forceinline__ void binary_spectrum(const Src &s, const typename P::Aux &aux, bool valid, int lane,
                                                 const double *__restrict__ tab, double fstar_inv, double scale,
                                                 typename P::CT *buf, const typename P::CT *__restrict__ tw,
@@ -277,6 +284,8 @@ __device__ __forceinline__ void binary_spectrum(const Src &s, const typename P::
     static_assert(DFT == GBGPU_TRANSFORM_LDS ||
                       (N == 256 && TPB == 64 && !THIRD && std::is_same<CT, float2>::value),
                   "the matrix-core transform covers mixed N = 256 without a third body");
+    static_assert(C0 >= 0 && NCP >= 1 && C0 + NCP <= NCH && (DFT == GBGPU_TRANSFORM_LDS || NCP == NCH),
+                  "channel pass out of range");
     constexpr int NT = THIRD ? SPT : 1;
     float vf[3][NT];   // v_LOS at xi_sc(t_n)
     double gc[3][NT];  // the cumulative phase term at t_n
@@ -394,8 +403,8 @@ __device__ __forceinline__ void binary_spectrum(const Src &s, const typename P::
             for (int c = 0; c < NCH; c++) {
                 if constexpr (DFT != GBGPU_TRANSFORM_LDS)
                     xr[c * SPT + q] = out[c];
-                else
-                    buf[c * N + n] = out[c];
+                else if (c >= C0 && c < C0 + NCP)
+                    buf[(c - C0) * N + n] = out[c];
             }
         }
     };
@@ -406,7 +415,7 @@ __device__ __forceinline__ void binary_spectrum(const Src &s, const typename P::
         for (int c = 0; c < NCH; c++) dft256_mfma<DFT>(xr + c * SPT, k);
     } else {
         __syncthreads();
-        fft_lds<N, NCH, TPB>(buf, lane, tw);
+        fft_lds<N, NCP, TPB>(buf, lane, tw);
     }
 }
 
@@ -437,6 +446,29 @@ __device__ __forceinline__ void ll_partial(const typename P::CT *buf, double2 fa
     dhr += fac.x * xr - fac.y * xi;
     dhi += fac.x * xi + fac.y * xr;
     hh += (fac.x * fac.x + fac.y * fac.y) * xx;
+}
+
+// One channel's share of ll_partial: channel CH (0 = A, 1 = E) of x in
+// `buf` (natural order), adding conj(d) x / S and |x|^2 / S of this thread's
+// bins to xr, xi, xx (before the factor fac, applied once per binary).
+template <int N, class P, int CH, int M = 1>
+__device__ __forceinline__ void ll_channel(const typename P::CT *buf, int lane, long long j0, long long doff,
+                                           long long noff, const WaveArgs &a, double &xr, double &xi, double &xx) {
+    using G = Geo<N, M>;
+    const double2 *d = CH == 0 ? a.dA : a.dE;
+    const double *S = CH == 0 ? a.SA : a.SE;
+#pragma unroll
+    for (int q = 0; q < G::SPT; q++) {
+        const int m = lane + q * G::TPB;
+        const long long j = j0 + m;
+        if (j < 0 || j >= a.L) continue;
+        const double2 h = P::up(buf[(m + N / 2) & (N - 1)]);
+        const double2 dv = d[doff + j];
+        const double iS = rcp_f64(S[noff + j]);
+        xr += (dv.x * h.x + dv.y * h.y) * iS;
+        xi += (dv.x * h.y - dv.y * h.x) * iS;
+        xx += (h.x * h.x + h.y * h.y) * iS;
+    }
 }
 
 // ll_partial for spectra held in registers by the matrix-core transform:

@@ -31,6 +31,18 @@ def _torch():
     return torch
 
 
+N_MAX = 8192  # the reference's get_N caps N here (src/gbgpu/utils/utility.py:268)
+
+
+def _check_N(N):
+    """The waveform kernels take powers of two in [32, 8192]: the band table and
+    the SNR rule of get_N only produce those (gbgpu_amd.utils.get_N).  Other
+    lengths, which the reference's numpy FFT would take, raise ValueError
+    before any device work."""
+    if N & (N - 1) or not (32 <= N <= N_MAX):
+        raise ValueError(f"N={N}: the HIP kernels support powers of two in [32, {N_MAX}]")
+
+
 class GBGPUBase(abc.ABC):
     """Generate Galactic-binary waveforms and likelihoods in batches.
 
@@ -138,8 +150,7 @@ class GBGPUBase(abc.ABC):
         return t
 
     def _cfg(self, N, T, tdi2, precision, transform=0):
-        if N & (N - 1) or not (32 <= N <= 2048):
-            raise ValueError(f"N={N}: the HIP kernels support powers of two in [32, 2048]")
+        _check_N(N)
         tab = self._table(T, N)
         cfg = _capi.WaveCfg(
             N=int(N),
@@ -174,8 +185,7 @@ class GBGPUBase(abc.ABC):
             N_temp = self.special_get_N(base[0], base[1], T, *args, oversample=oversample)
             N = int(np.max(N_temp))
         N = int(N)
-        if N & (N - 1) or not (32 <= N <= 2048):
-            raise ValueError(f"N={N}: the HIP kernels support powers of two in [32, 2048]")
+        _check_N(N)
         rows = base + [np.broadcast_to(np.asarray(a, dtype=np.float64), (B,)) for a in add_args]
         torch = _torch()
         params = torch.from_numpy(np.ascontiguousarray(np.stack(rows, axis=0))).to(self._device())

@@ -58,9 +58,16 @@ enum {
 #define GBGPU_NPARAMS_THIRD 14
 #define GBGPU_ORBIT_ROWS 22
 
+/* Largest N of the waveform entry points (the reference's get_N caps N at
+ * 8192, src/gbgpu/utils/utility.py:268).  gbgpu_run_wave, gbgpu_run_wave_ae,
+ * gbgpu_wave_ll and gbgpu_wave_ll_noalloc take every power of two in
+ * [32, GBGPU_N_MAX]; gbgpu_swap_ll_diff, gbgpu_wave_ll_grad and the batched
+ * path take N <= 2048, gbgpu_fisher N <= 1024 (GBGPU_E_ARG otherwise). */
+#define GBGPU_N_MAX 8192
+
 /* Waveform configuration shared by every waveform entry point. */
 typedef struct gbgpu_wave_cfg {
-    int N;                     /* samples per binary: power of two, 32..2048 */
+    int N;                     /* samples per binary: power of two, 32..GBGPU_N_MAX */
     double T;                  /* observation time, already int(T/dt)*dt (gbgpu.py:177-179) */
     double armlength;          /* metres; f* = c / (2 pi armlength) (gbgpu.py:243) */
     const double *orbit_table; /* (22, N) float64: t_n, guiding centre Pc (xyz), offsets P_s - Pc

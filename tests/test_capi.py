@@ -121,6 +121,32 @@ def test_empty_batches_are_noops_on_every_entry_point():
     del ctypes
 
 
+def test_N_limits_checked_before_gpu_work():
+    """Waveform and likelihood entries take powers of two up to 8192; swap and
+    gradients stop at 2048 and Fisher matrices at 1024, with GBGPU_E_ARG
+    before any HIP call (dummy pointers are never dereferenced)."""
+    from gbgpu_amd import _capi
+
+    lib = _capi.load()
+    P = 8
+    dat = _capi.Data(A=P, E=P, psdA=P, psdE=P, data_index=None, noise_index=None, data_length=10,
+                     start_freq_ind=0, df=1e-7)
+    cfg = _capi.WaveCfg(N=16384, T=1e7, armlength=2.5e9, orbit_table=P, tdi2=0, precision=0)
+    assert lib.gbgpu_run_wave(P, P, P, P, 9, 4, cfg, None) == -1
+    assert b"[32, 8192]" in lib.gbgpu_last_error()
+    assert lib.gbgpu_wave_ll(P, None, None, P, 9, 4, cfg, dat, 0.0, 0, None) == -1
+    for N in (4096, 8192):
+        cfg.N = N
+        # accepted by the waveform entries (nothing runs for an empty batch)
+        assert lib.gbgpu_run_wave(P, P, P, P, 9, 0, cfg, None) == 0
+        assert lib.gbgpu_wave_ll(P, None, None, P, 9, 0, cfg, dat, 0.0, 0, None) == 0
+        assert lib.gbgpu_swap_ll_diff(P, None, P, 9, 4, cfg, dat, None) == -1
+        assert b"2048" in lib.gbgpu_last_error()
+        assert lib.gbgpu_wave_ll_grad(P, P, None, None, P, 9, 4, cfg, dat, 0.0, 0, None) == -1
+        assert b"2048" in lib.gbgpu_last_error()
+        assert lib.gbgpu_fisher(P, None, None, P, 9, 4, cfg, dat, 0.0, None) == -1
+
+
 def test_missing_library_fails_loudly(tmp_path):
     from gbgpu_amd import _capi
 

@@ -166,16 +166,16 @@ def test_third_body_N_oversampled_once():
 
 
 def test_unsupported_N_rejected_before_device_work():
-    """The HIP kernels take powers of two in [32, 2048]; other N (given, or
-    chosen by get_N for high-frequency or loud sources) are refused with a
+    """The HIP kernels take powers of two in [32, 8192] (get_N's cap); other
+    N (given, or chosen by get_N with oversampling) are refused with a
     ValueError before anything touches a device."""
     from gbgpu_amd import GBGPU
 
     gb = GBGPU()
     p = [np.array([x]) for x in (1e-22, 2e-3, 1e-18, 0.0, 0.1, 0.2, 0.3, 0.4, 0.5)]
-    for N in (100, 16, 4096):
-        with pytest.raises(ValueError, match="powers of two in \\[32, 2048\\]"):
+    for N in (100, 16, 6144, 16384):
+        with pytest.raises(ValueError, match="powers of two in \\[32, 8192\\]"):
             gb.run_wave(*p, N=N, T=YR, dt=15.0)
-    p[1] = np.array([0.35])  # f0 >= 0.3 Hz over 8 yr: band N = 1024 * 8 = 8192
+    p[1] = np.array([0.35])  # f0 >= 0.3 Hz over 8 yr: band N = 1024 * 8 = 8192, oversampled twice
     with pytest.raises(ValueError, match="powers of two"):
-        gb.run_wave(*p, N=None, T=8 * YR, dt=1.0)
+        gb.run_wave(*p, N=None, T=8 * YR, dt=1.0, oversample=2)
