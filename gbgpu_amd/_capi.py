@@ -23,6 +23,8 @@ GBGPU_BATCH_BUCKETS = 0
 GBGPU_BATCH_PERSISTENT = 1
 ORBIT_ROWS = 22
 NOALLOC_MAX_BIN = 65536
+N_MAX = 8192
+NBUCKET = 9  # N buckets of the batched path: N = 32 << k, k < 9
 
 c_void_p = ctypes.c_void_p
 c_int = ctypes.c_int
@@ -68,7 +70,7 @@ class BatchCfg(ctypes.Structure):
         ("armlength", c_double),
         ("tdi2", c_int),
         ("precision", c_int),
-        ("orbit_tables", c_void_p * 7),
+        ("orbit_tables", c_void_p * NBUCKET),
         ("strategy", c_int),
         ("guard_tol", c_double),
         ("transform", c_int),

@@ -3,8 +3,9 @@
 ``BatchedLogLike`` keeps the data, PSD, per-N orbit tables and the library
 workspace resident on the GPU and evaluates the fused waveform + likelihood
 for a batch in which every binary has its own N (``gbgpu_wave_ll_batched``:
-device-side N bucketing, then one launch per bucket or one persistent launch).  ``run_loop`` evaluates a sequence of calls whose
-parameters are seeded Gaussian relative jumps of a base point
+device-side N bucketing, then one launch per bucket).  ``run_loop``
+evaluates a sequence of calls whose parameters are seeded Gaussian relative
+jumps of a base point
 (``gbgpu_perturb_params``; N is re-derived from the jumped f0 by the step
 rule), either eagerly or as one captured hipGraph (``capture_loop``).
 """
@@ -30,7 +31,8 @@ class BatchedLogLike:
 
     Args:
         gb: a ``GBGPU`` instance (supplies orbits, d_d and the device).
-        data, psd: [A, E] arrays (1-D), numpy or torch.
+        data, psd: [A, E] arrays, numpy or torch: 1-D, or 2-D (streams,
+            data_length) selected per binary by data_index / noise_index.
         T, dt: observation time and cadence (T is reduced to int(T/dt)*dt).
         precision: "fp64" or "mixed".
         N_levels, f_edges: the N step rule used when N is derived from f0:
@@ -47,15 +49,24 @@ class BatchedLogLike:
             grows with SNR^2).
         transform: transform of the mixed N = 256 bucket ("lds",
             "mfma_f32", "mfma_bf16x3", "mfma_bf16x6"; see GBGPU.transform);
-            None takes gb.transform.  Every other bucket and every FP64 pass
-            uses the LDS transform.
+            None takes gb.transform.  Every other bucket, every FP64 pass and
+            every third-body batch uses the LDS transform.
+        tdi2: second-generation TDI factor, as get_ll's ``tdi2``.
+        data_index, noise_index: (B,) int32 stream of each binary for 2-D
+            data / psd (None: stream 0), as get_ll's; fixed for the
+            evaluator, so every call takes B binaries in the same order.
+
+    Parameters are (9, B) as get_ll's, or (14, B) for third-body binaries
+    with rows 9-13 in kernel form (A2, varpi, e2, n2 = 2 pi / P2 [1/s],
+    T2 [s]): ``kernel_params`` converts the user rows (P2, T2 in years).
+    N per binary is a power of two in [32, 8192].
     """
 
     STRATEGIES = {"buckets": _capi.GBGPU_BATCH_BUCKETS, "persistent": _capi.GBGPU_BATCH_PERSISTENT}
 
     def __init__(self, gb, data, psd, T, dt, precision="mixed", start_freq_ind=0, N_levels=DEFAULT_N_LEVELS,
                  f_edges=DEFAULT_F_EDGES, phase_marginalize=False, strategy="buckets", guard_tol=1e-3,
-                 transform=None):
+                 transform=None, tdi2=False, data_index=None, noise_index=None):
         torch = _torch()
         if gb.d_d is None:
             raise ValueError("gb.d_d must be set before building a likelihood")
@@ -76,15 +87,23 @@ class BatchedLogLike:
             raise ValueError("the accuracy guard runs with the 'buckets' strategy only; pass guard_tol=None "
                              "for an unguarded persistent launch")
         dev = gb._device()
+        if data[0].ndim not in (1, 2) or psd[0].ndim not in (1, 2):
+            raise ValueError("data and psd must be 1-D or 2-D")
+        self.L = int(data[0].shape[-1])
+        if int(psd[0].shape[-1]) != self.L:
+            raise ValueError("data and psd must have the same length")
+        if data[0].ndim == 1 and data_index is not None or psd[0].ndim == 1 and noise_index is not None:
+            raise ValueError("data_index / noise_index need 2-D data / psd")
         self.A = gb._to_dev(data[0], np.complex128).reshape(-1)
         self.E = gb._to_dev(data[1], np.complex128).reshape(-1)
         self.SA = gb._to_dev(psd[0], np.float64).reshape(-1)
         self.SE = gb._to_dev(psd[1], np.float64).reshape(-1)
-        self.L = int(self.A.numel())
+        self.data_index = self._index(gb, data_index, self.A.numel())
+        self.noise_index = self._index(gb, noise_index, self.SA.numel())
         transform = gb.transform if transform is None else transform
         if transform not in _capi.TRANSFORMS:
             raise ValueError(f"transform must be one of {list(_capi.TRANSFORMS)}")
-        self.cfg = _capi.BatchCfg(T=float(self.T), armlength=float(gb.orbits.armlength), tdi2=0,
+        self.cfg = _capi.BatchCfg(T=float(self.T), armlength=float(gb.orbits.armlength), tdi2=int(bool(tdi2)),
                                   precision=1 if precision == "mixed" else 0, strategy=self.STRATEGIES[strategy],
                                   guard_tol=self.guard_tol,
                                   transform=_capi.TRANSFORMS[transform] if precision == "mixed" else 0)
@@ -92,7 +111,8 @@ class BatchedLogLike:
         for N in self.N_levels:
             self._add_table(int(N))
         self.data = _capi.Data(A=self.A.data_ptr(), E=self.E.data_ptr(), psdA=self.SA.data_ptr(),
-                               psdE=self.SE.data_ptr(), data_index=None, noise_index=None, data_length=self.L,
+                               psdE=self.SE.data_ptr(), data_index=_capi.ptr(self.data_index),
+                               noise_index=_capi.ptr(self.noise_index), data_length=self.L,
                                start_freq_ind=int(start_freq_ind), df=float(1.0 / self.T))
         self.dev = dev
         self._ws = None
@@ -100,11 +120,37 @@ class BatchedLogLike:
         self._last_B = 0
         self._graph = None
 
+    def _index(self, gb, idx, total):
+        """Per-binary stream index on the device (None: NULL, stream 0)."""
+        if idx is None:
+            return None
+        t = gb._to_dev(np.asarray(idx, dtype=np.int32) if not _torch().is_tensor(idx) else idx, np.int32)
+        if t.numel() and (int(t.min()) < 0 or (int(t.max()) + 1) * self.L > total):
+            raise ValueError("data_index/noise_index point outside the provided streams")
+        return t
+
+    def kernel_params(self, params):
+        """(14, B) third-body parameters as get_ll takes them (P2, T2 in
+        years) -> the kernel rows (n2 = 2 pi / P2 [1/s], T2 [s]); (9, B) are
+        returned unchanged."""
+        p = np.asarray(params, dtype=np.float64)
+        if p.shape[0] == 9:
+            return p
+        return np.vstack([p[:9], np.array(self.gb.prepare_additional_args(*p[9:]))])
+
+    def _check_batch(self, p):
+        if p.shape[0] not in (9, 14):
+            raise ValueError("params must have 9 rows, or 14 for third-body binaries")
+        B = p.shape[1]
+        for idx in (self.data_index, self.noise_index):
+            if idx is not None and idx.numel() != B:
+                raise ValueError("data_index / noise_index must have one entry per binary")
+
     def _add_table(self, N):
         k = int(np.log2(N)) - 5
         if N not in self._tabs:
-            if N & (N - 1) or not 32 <= N <= 2048:
-                raise ValueError(f"N={N}: powers of two in [32, 2048] only")
+            if N & (N - 1) or not 32 <= N <= _capi.N_MAX:
+                raise ValueError(f"N={N}: powers of two in [32, {_capi.N_MAX}] only")
             self._tabs[N] = self.gb._table(self.T, N)
             self.cfg.orbit_tables[k] = self._tabs[N].data_ptr()
 
@@ -128,6 +174,7 @@ class BatchedLogLike:
         torch = _torch()
         p = params if torch.is_tensor(params) else torch.from_numpy(np.ascontiguousarray(params, np.float64))
         p = p.to(self.dev).contiguous()
+        self._check_batch(p)
         B = p.shape[1]
         if N is None:
             N = self.N_of(p[1].cpu().numpy())
@@ -137,8 +184,9 @@ class BatchedLogLike:
         Nt = torch.from_numpy(Nn).to(self.dev)
         ll = torch.empty(B, dtype=torch.float64, device=self.dev)
         ws = self._workspace(B)
-        chosen = np.zeros(7, dtype=np.int32)
-        rc = _capi.load().gbgpu_batch_autotune(ll.data_ptr(), p.data_ptr(), 9, B, Nt.data_ptr(), self.cfg, self.data,
+        chosen = np.zeros(_capi.NBUCKET, dtype=np.int32)
+        rc = _capi.load().gbgpu_batch_autotune(ll.data_ptr(), p.data_ptr(), p.shape[0], B, Nt.data_ptr(), self.cfg,
+                                               self.data,
                                                float(np.real(self.gb.d_d)), ws.data_ptr(), chosen.ctypes.data,
                                                _capi.stream_handle())
         _capi.check(rc, "gbgpu_batch_autotune")
@@ -147,7 +195,7 @@ class BatchedLogLike:
     def set_shapes(self, shapes):
         """Pin the per-bucket variants ({N: 0 or 1}, and 2-4 for the mixed
         N = 256 bucket as in autotune; others 0) for this precision."""
-        v = np.zeros(7, dtype=np.int32)
+        v = np.zeros(_capi.NBUCKET, dtype=np.int32)
         for n, c in shapes.items():
             v[int(np.log2(n)) - 5] = c
         _capi.check(_capi.load().gbgpu_batch_set_variants(self.cfg.precision, v.ctypes.data),
@@ -157,7 +205,7 @@ class BatchedLogLike:
         """Binaries per N (dict) the accuracy guard evaluated in FP64 in the
         last call -- routed there at set-up by the <h|h> estimate, or
         re-evaluated after the mixed pass (synchronises the current stream)."""
-        counts = np.zeros(7, dtype=np.int32)
+        counts = np.zeros(_capi.NBUCKET, dtype=np.int32)
         if self._last_ws is None:
             return {}
         _capi.check(_capi.load().gbgpu_batch_guard_counts(self._last_ws.data_ptr(), self._last_B, self.cfg.precision,
@@ -170,10 +218,12 @@ class BatchedLogLike:
         return self.N_levels[np.searchsorted(self.f_edges, np.abs(np.asarray(f0)), side="right")]
 
     def __call__(self, params, N=None, d_h=False):
-        """log-likelihood of (9, B) params with per-binary N (default: from f0)."""
+        """log-likelihood of (9, B) or (14, B) kernel-form params with
+        per-binary N (default: from f0)."""
         torch = _torch()
         p = params if torch.is_tensor(params) else torch.from_numpy(np.ascontiguousarray(params, np.float64))
         p = p.to(self.dev).contiguous()
+        self._check_batch(p)
         B = p.shape[1]
         if N is None:
             N = self.N_of(p[1].cpu().numpy())
@@ -191,7 +241,7 @@ class BatchedLogLike:
         ws = self._workspace(B)
         self._last_B = B
         self._last_ws = ws
-        rc = _capi.load().gbgpu_wave_ll_batched(ll.data_ptr(), _capi.ptr(dh), _capi.ptr(hh), p.data_ptr(), 9, B,
+        rc = _capi.load().gbgpu_wave_ll_batched(ll.data_ptr(), _capi.ptr(dh), _capi.ptr(hh), p.data_ptr(), p.shape[0], B,
                                                 Nt.data_ptr(), self.cfg, self.data, float(np.real(self.gb.d_d)),
                                                 int(self.phase_marginalize), ws.data_ptr(), _capi.stream_handle())
         _capi.check(rc, "gbgpu_wave_ll_batched")
@@ -201,6 +251,7 @@ class BatchedLogLike:
         torch = _torch()
         base = params if torch.is_tensor(params) else torch.from_numpy(np.ascontiguousarray(params, np.float64))
         base = base.to(self.dev).contiguous()
+        self._check_batch(base)
         B = base.shape[1]
         out = torch.empty((n_calls, B), dtype=torch.float64, device=self.dev)
         pj = torch.empty_like(base)

@@ -3,7 +3,7 @@
 // The reference evaluates one N per call (src/gbgpu/gbgpu.py:205-207 takes
 // N.max(); the native get_ll takes a scalar M, gbgpu_utils.cu:126), so an
 // uneven batch costs one call per N.  Here every binary carries its own N
-// (a power of two in [32, 2048]) and one call does, on the device:
+// (a power of two in [32, 8192]) and one call does, on the device:
 //   k_batch_reset    clear the control block (a fill kernel, no memset)
 //   k_batch_setup    per-binary constants (Pack) and a histogram over
 //                    (N bucket, route, log-spaced start-bin cell); with the
@@ -35,24 +35,30 @@ namespace gbgpu {
 int check_data(const gbgpu_data *data);
 void set_data(WaveArgs &a, const gbgpu_data *data);
 
-int launch_ll_segment(int N, int precision, int pack_precision, int variant, const WaveArgs &a, hipStream_t st);
+int launch_ll_segment(int N, int precision, int pack_precision, int variant, bool third, const WaveArgs &a,
+                      hipStream_t st);
 
-constexpr int NBUCKET = 7;  // N = 32 << k, k < 7
+constexpr int NBUCKET = 9;  // N = 32 << k, k < 9
+// The persistent strategy (switched off) covers N <= 2048 only.
+constexpr int NBUCKET_PERSISTENT = 7;
 // List segments: segment k holds bucket k evaluated in the batch's
 // precision, segment NBUCKET + k the binaries of bucket k routed to FP64.
 constexpr int NSEG = 2 * NBUCKET;
 constexpr int NCB = 4096;  // start-bin cells per segment (log-spaced)
 constexpr int NCELL = NSEG * NCB;
-enum { CTL_COUNT = 0, CTL_LISTOFF = 16, CTL_ITEMOFF = 32, CTL_ITEMS = 40, CTL_TOTAL = 48, CTL_WORK = 49,
-       CTL_REDO = 56, CTL_HIST = 64, CTL_CUR = CTL_HIST + NCELL, CTL_WORDS = CTL_CUR + NCELL };
-static_assert(CTL_LISTOFF - CTL_COUNT >= NSEG && CTL_ITEMOFF - CTL_LISTOFF >= NSEG, "control block layout");
+enum { CTL_COUNT = 0, CTL_LISTOFF = 32, CTL_ITEMOFF = 64, CTL_ITEMS = 80, CTL_TOTAL = 96, CTL_WORK = 97,
+       CTL_REDO = 104, CTL_HIST = 128, CTL_CUR = CTL_HIST + NCELL, CTL_WORDS = CTL_CUR + NCELL };
+static_assert(CTL_LISTOFF - CTL_COUNT >= NSEG && CTL_ITEMOFF - CTL_LISTOFF >= NSEG &&
+                  CTL_ITEMS - CTL_ITEMOFF >= NBUCKET && CTL_TOTAL - CTL_ITEMS >= NBUCKET &&
+                  CTL_HIST - CTL_REDO >= NBUCKET,
+              "control block layout");
 
 __host__ __device__ constexpr int bucket_bpw(int k) {
     return WG_SIZE / (((32 << k) / 4 < WG_SIZE) ? (32 << k) / 4 : WG_SIZE);
 }
 
 __device__ __forceinline__ int bucket_of(int N) {
-    if (N < 32 || N > 2048 || (N & (N - 1))) return -1;
+    if (N < 32 || N > (32 << (NBUCKET - 1)) || (N & (N - 1))) return -1;
     return 26 - __clz(N);  // log2(N) - 5 for 32-bit N
 }
 
@@ -68,6 +74,7 @@ struct BatchArgs {
     double guard_tol;
     double route_tol;  // > 0: route binaries whose estimated error bound exceeds it to FP64
     int transform;     // GBGPU_TRANSFORM_* requested for the mixed N = 256 bucket
+    bool third;        // 14 parameter rows (third-body binaries)
 };
 
 __global__ void k_batch_reset(int *ctl) {
@@ -76,17 +83,19 @@ __global__ void k_batch_reset(int *ctl) {
 }
 
 // Estimated <h|h> of a binary from its amplitude, frequency and the A-channel
-// PSD at its centre bin: 1.1 A^2 T (4 wL sin wL)^2 / S_A(f0), wL = f0/f*.
-// The factor 1.1 is the median, over sky positions and orientations, of the
-// exact <h|h> over the rest of the expression (0.27 to 3.84 across 2400
-// sources at 0.1-20 mHz, 4 yr); the estimate only routes work, and binaries
-// it misses are still caught by the guard after the mixed pass.
-__device__ __forceinline__ double hh_estimate(const Src &s, int N, const WaveArgs &a) {
+// PSD of its noise stream at its centre bin: 1.1 A^2 T (4 wL sin wL)^2
+// |fac_TDI2|^2 / S_A(f0), wL = f0/f* (fac_TDI2 = 1 for TDI1).  The factor 1.1
+// is the median, over sky positions and orientations, of the exact <h|h>
+// over the rest of the expression (0.27 to 3.84 across 2400 sources at
+// 0.1-20 mHz, 4 yr); the estimate only routes work, and binaries it misses
+// are still caught by the guard after the mixed pass.
+__device__ __forceinline__ double hh_estimate(const Src &s, long long b, int N, const WaveArgs &a) {
     long long j = s.start + N / 2 - a.start_freq_ind;
     j = j < 0 ? 0 : (j >= a.L ? a.L - 1 : j);
+    const long long noff = (a.ni ? (long long)a.ni[b] : 0LL) * a.L;
     const double wL = s.f0 * a.fstar_inv;
     const double r = 4.0 * wL * sin(wL);
-    return 1.1 * s.amp * s.amp * a.T * r * r / a.SA[j];
+    return 1.1 * s.amp * s.amp * a.T * r * r * (s.tdr * s.tdr + s.tdi * s.tdi) / a.SA[noff + j];
 }
 
 // Histogram cell of a binary: segment k, then its start bin within the data
@@ -122,7 +131,7 @@ __global__ void __launch_bounds__(256) k_batch_setup(BatchArgs ba, Pack<P> *__re
     pk.s = make_src(ba.a.params, ba.a.num_params, ba.a.B, b, N, ba.a.T, ba.a.tdi2, ba.a.armlength);
     pk.aux = P::make_aux(pk.s);
     pack[b] = pk;
-    const bool routed = ba.route_tol > 0.0 && GUARD_REL * hh_estimate(pk.s, N, ba.a) > ba.route_tol;
+    const bool routed = ba.route_tol > 0.0 && GUARD_REL * hh_estimate(pk.s, b, N, ba.a) > ba.route_tol;
     const int cell = batch_cell(routed ? NBUCKET + k : k, pk.s.start, ba.a);
     ba.cell[b] = cell;
     atomicAdd(&ba.ctl[CTL_HIST + cell], 1);
@@ -168,7 +177,7 @@ __global__ void __launch_bounds__(1024) k_batch_scan(int *ctl) {
             ctl[CTL_COUNT + k] = o1 - o0;
         }
         int item_run = 0;
-        for (int k = NBUCKET - 1; k >= 0; k--) {
+        for (int k = NBUCKET_PERSISTENT - 1; k >= 0; k--) {
             const int c = ctl[CTL_COUNT + k];
             const int bpw = bucket_bpw(k);
             ctl[CTL_ITEMOFF + k] = item_run;
@@ -266,7 +275,7 @@ __device__ __forceinline__ void batch_loop(const BatchArgs &ba, char *lds) {
         const int item = s_item;
         __syncthreads();
         if (item >= total) break;  // uniform: every thread read the same item
-        int k = NBUCKET - 1;
+        int k = NBUCKET_PERSISTENT - 1;
         while (k > 0 && item >= ba.ctl[CTL_ITEMOFF + k] + ba.ctl[CTL_ITEMS + k]) k--;
         const int local = item - ba.ctl[CTL_ITEMOFF + k];
         const int count = ba.ctl[CTL_COUNT + k];
@@ -367,9 +376,12 @@ static int launch_buckets(const BatchArgs &ba, int precision, hipStream_t st, co
         if (!ba.tabs[k]) continue;
         const int N = 32 << k;
         int var = variants ? variants[k] : g_variant[precision == GBGPU_MIXED ? 1 : 0][k].load();
-        // a requested matrix-core transform of the mixed N = 256 bucket wins
+        // a requested matrix-core transform of the mixed N = 256 bucket wins;
+        // third-body binaries take the LDS transform on the same tile shape
         if (precision == GBGPU_MIXED && N == 256 && ba.transform != GBGPU_TRANSFORM_LDS)
             var = VARIANT_DFT0 + ba.transform - 1;
+        if (ba.third && var >= VARIANT_DFT0) var = 0;
+        const int var64 = g_variant[0][k].load();
         if (bucket_events) (void)hipEventRecord(bucket_events[2 * k], st);
         WaveArgs a = ba.a;
         a.tab = ba.tabs[k];
@@ -383,7 +395,7 @@ static int launch_buckets(const BatchArgs &ba, int precision, hipStream_t st, co
             a.redo_count = ba.ctl + CTL_REDO + k;
             a.redo_tol = ba.guard_tol;
         }
-        int rc = launch_ll_segment(N, precision, precision, var, a, st);
+        int rc = launch_ll_segment(N, precision, precision, var, ba.third, a, st);
         if (rc) return rc;
         if (guard) {
             WaveArgs r = a;
@@ -395,7 +407,7 @@ static int launch_buckets(const BatchArgs &ba, int precision, hipStream_t st, co
             if (!r.tw) return hip_fail(hipErrorOutOfMemory, "twiddles");
             // the FP64 pass takes the FP64 tile of the bucket (tuned or default),
             // as the plain FP64 batch does: same speed, same results
-            rc = launch_ll_segment(N, GBGPU_FP64, GBGPU_MIXED, g_variant[0][k].load(), r, st);
+            rc = launch_ll_segment(N, GBGPU_FP64, GBGPU_MIXED, var64, ba.third, r, st);
             if (rc) return rc;
         }
         if (route) {
@@ -407,7 +419,7 @@ static int launch_buckets(const BatchArgs &ba, int precision, hipStream_t st, co
             r.seg_off = ba.ctl + CTL_LISTOFF + NBUCKET + k;
             r.tw = twiddles_f64(N);
             if (!r.tw) return hip_fail(hipErrorOutOfMemory, "twiddles");
-            rc = launch_ll_segment(N, GBGPU_FP64, GBGPU_MIXED, g_variant[0][k].load(), r, st);
+            rc = launch_ll_segment(N, GBGPU_FP64, GBGPU_MIXED, var64, ba.third, r, st);
             if (rc) return rc;
         }
         if (bucket_events) (void)hipEventRecord(bucket_events[2 * k + 1], st);
@@ -505,8 +517,8 @@ static int make_batch_args(BatchArgs &ba, double *loglike, gbgpu_cmplx *d_h, gbg
         set_error("null cfg, params, N_bin or loglike");
         return GBGPU_E_ARG;
     }
-    if (num_params != GBGPU_NPARAMS) {
-        set_error("the batched path takes the 9 circular-binary parameter rows");
+    if (num_params != GBGPU_NPARAMS && num_params != GBGPU_NPARAMS_THIRD) {
+        set_error("num_params must be 9 or 14");
         return GBGPU_E_ARG;
     }
     if (num_bin < 0 || !(cfg->T > 0) || !(cfg->armlength > 0) ||
@@ -548,6 +560,7 @@ static int make_batch_args(BatchArgs &ba, double *loglike, gbgpu_cmplx *d_h, gbg
     a.h_h = reinterpret_cast<double2 *>(h_h);
     ba.Nb = N_bin;
     ba.transform = cfg->transform;
+    ba.third = num_params == GBGPU_NPARAMS_THIRD;
     if (cfg->precision == GBGPU_MIXED && cfg->orbit_tables[3]) {
         // the N = 256 bucket may run a matrix-core transform (requested or tuned)
         a.dftc = dft256_table();
@@ -606,7 +619,7 @@ extern "C" int gbgpu_batch_autotune(double *loglike, const double *params, int n
     constexpr int NVAR = 5, REPS = 3;
     auto allowed = [&](int v, int k) {
         return v < VARIANT_DFT0 ||
-               (cfg->precision == GBGPU_MIXED && k == 3 && cfg->transform == GBGPU_TRANSFORM_LDS &&
+               (cfg->precision == GBGPU_MIXED && k == 3 && cfg->transform == GBGPU_TRANSFORM_LDS && !ba.third &&
                 (v == VARIANT_DFT0 + GBGPU_TRANSFORM_MFMA_F32 - 1 || v == VARIANT_DFT0 + GBGPU_TRANSFORM_MFMA_BF16X6 - 1));
     };
     float best[NVAR][NBUCKET];

@@ -5,6 +5,7 @@
 #include "common.hpp"
 #include "dispatch.hpp"
 #include "wave_common.hpp"
+#include "wave_pass.hpp"
 
 namespace gbgpu {
 
@@ -31,27 +32,38 @@ constexpr int big_channels_per_pass() {
     return fit >= NCH ? NCH : fit;
 }
 
-template <int N, int MODE, bool THIRD, class P>
-__global__ void __launch_bounds__(WG_MAX) k_wave_big(WaveArgs a) {
+// LDS of one large-N workgroup.
+template <int N, int NCH, bool THIRD, class P>
+struct BigLds {
     using G = Geo<N, 0>;
     using CT = typename P::CT;
-    constexpr int TPB = G::TPB, SPT = G::SPT, WPB = G::WPB;
+    static constexpr int CPP = big_channels_per_pass<N, NCH, CT>();
+    static_assert(CPP >= 1, "one channel must fit in LDS");
+    CT sbuf[CPP * N];
+    double stot[G::WPB];
+    double s_edge[THIRD ? 6 * G::WPB : 1];
+};
+
+// One binary b on the whole workgroup.  PK: layout of the per-binary
+// constants (the FP64 passes of a guarded mixed batch read mixed ones).
+// `first`: offset of the segment in redo_list (guarded mixed likelihood).
+template <int N, int MODE, bool THIRD, class P, class PK>
+__device__ __forceinline__ void big_binary(const WaveArgs &a, long long b, long long first,
+                                           BigLds<N, MODE == MODE_WAVE ? 3 : 2, THIRD, P> &L) {
+    using G = Geo<N, 0>;
+    using CT = typename P::CT;
+    constexpr int TPB = G::TPB, SPT = G::SPT;
     static_assert(G::BPW == 1 && TPB == WG_MAX, "one 1024-thread workgroup per binary");
     constexpr int NCH = (MODE == MODE_WAVE) ? 3 : 2;
-    constexpr int CPP = big_channels_per_pass<N, NCH, CT>();
-    static_assert(CPP >= 1, "one channel must fit in LDS");
-    __shared__ CT sbuf[CPP * N];
-    __shared__ double stot[WPB];
-    __shared__ double s_edge[THIRD ? 6 * WPB : 1];
+    constexpr int CPP = BigLds<N, NCH, THIRD, P>::CPP;
+    CT *sbuf = L.sbuf;
 
     const int lane = threadIdx.x;
-    const long long item = blockIdx.x;
-    const long long b = a.order ? (long long)a.order[item] : item;
-    const Pack<P> *pk = static_cast<const Pack<P> *>(a.pack) + b;
+    const Pack<PK> *pk = static_cast<const Pack<PK> *>(a.pack) + b;
     const Src s = to_sgpr(pk->s);
-    const typename P::Aux aux = to_sgpr(pk->aux);
+    const typename P::Aux aux = to_sgpr(aux_of<P, PK>(*pk));
     const double2 fac = P::fac(s);
-    ThirdScratch ts{s_edge};
+    ThirdScratch ts{L.s_edge};
 
     // MODE_LL: sums over both channels of conj(d) x / S and |x|^2 / S
     double xr = 0.0, xi = 0.0, xx = 0.0;
@@ -65,7 +77,7 @@ __global__ void __launch_bounds__(WG_MAX) k_wave_big(WaveArgs a) {
         constexpr int C0 = decltype(c0_c)::value;
         constexpr int NCP = (NCH - C0) < CPP ? (NCH - C0) : CPP;
         binary_spectrum<N, NCH, THIRD, P, 0, GBGPU_TRANSFORM_LDS, C0, NCP>(
-            s, aux, true, lane, a.tab, a.fstar_inv, a.scale, sbuf, static_cast<const CT *>(a.tw), ts, stot);
+            s, aux, true, lane, a.tab, a.fstar_inv, a.scale, sbuf, static_cast<const CT *>(a.tw), ts, L.stot);
         // value of channel c (in this pass) at fftshifted output position m
         auto spec = [&](int c, int m) { return cmul(P::up(sbuf[(c - C0) * N + ((m + N / 2) & (N - 1))]), fac); };
         if constexpr (MODE == MODE_LL) {
@@ -121,9 +133,9 @@ __global__ void __launch_bounds__(WG_MAX) k_wave_big(WaveArgs a) {
         double dhr = fac.x * xr - fac.y * xi;
         double dhi = fac.x * xi + fac.y * xr;
         double hh = (fac.x * fac.x + fac.y * fac.y) * xx;
-        dhr = group_total<TPB>(dhr, lane, stot);
-        dhi = group_total<TPB>(dhi, lane, stot);
-        hh = group_total<TPB>(hh, lane, stot);
+        dhr = group_total<TPB>(dhr, lane, L.stot);
+        dhi = group_total<TPB>(dhi, lane, L.stot);
+        hh = group_total<TPB>(hh, lane, L.stot);
         if (lane == 0) {
             dhr *= a.df4;
             dhi *= a.df4;
@@ -132,9 +144,35 @@ __global__ void __launch_bounds__(WG_MAX) k_wave_big(WaveArgs a) {
             if (a.h_h) a.h_h[b] = make_double2(hh, 0.0);
             const double dh_use = a.phase_marg ? sqrt(dhr * dhr + dhi * dhi) : dhr;
             a.loglike[b] = -0.5 * (a.d_d + hh - 2.0 * dh_use);
+            if (a.redo_list && GUARD_REL * (fabs(hh) + sqrt(dhr * dhr + dhi * dhi)) > a.redo_tol) {
+                const int pos = atomicAdd(a.redo_count, 1);
+                a.redo_list[first + pos] = (int)b;
+            }
         }
     } else {
         if (lane == 0) a.start_inds[b] = s.start;
+    }
+}
+
+template <int N, int MODE, bool THIRD, class P>
+__global__ void __launch_bounds__(WG_MAX) k_wave_big(WaveArgs a) {
+    __shared__ BigLds<N, MODE == MODE_WAVE ? 3 : 2, THIRD, P> lds;
+    const long long item = blockIdx.x;
+    const long long b = a.order ? (long long)a.order[item] : item;
+    big_binary<N, MODE, THIRD, P, P>(a, b, 0, lds);
+}
+
+// Fused likelihood over one device-side segment of an uneven-N batch (item
+// i < *a.seg_count is binary a.order[*a.seg_off + i]); the grid strides over
+// the segment, one binary per workgroup pass.
+template <int N, bool THIRD, class P, class PK>
+__global__ void __launch_bounds__(WG_MAX) k_wave_big_seg(WaveArgs a) {
+    __shared__ BigLds<N, 2, THIRD, P> lds;
+    const long long count = *a.seg_count;
+    const long long first = *a.seg_off;
+    for (long long item = blockIdx.x; item < count; item += gridDim.x) {
+        const long long b = a.order[first + item];
+        big_binary<N, MODE_LL, THIRD, P, PK>(a, b, first, lds);
     }
 }
 
@@ -177,6 +215,38 @@ int launch_wave_big(int N, int mode, bool third, int precision, const WaveArgs &
 #undef GBGPU_BIG
     set_error("unknown waveform mode");
     return GBGPU_E_ARG;
+}
+
+template <bool THIRD, class P, class PK>
+static int big_segment(int N, const WaveArgs &a, hipStream_t st) {
+    // the segment's size is known only on the device: a few resident
+    // workgroups per CU (one fits at a time), striding
+    static const int grid = [] {
+        int dev = 0, ncu = 0;
+        (void)hipGetDevice(&dev);
+        (void)hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev);
+        return 4 * (ncu > 0 ? ncu : 256);
+    }();
+    const long long g = a.B < grid ? a.B : grid;
+    KernelTimer kt(st);
+    if (N == 4096)
+        hipLaunchKernelGGL((k_wave_big_seg<4096, THIRD, P, PK>), dim3((unsigned)g), dim3(WG_MAX), 0, st, a);
+    else
+        hipLaunchKernelGGL((k_wave_big_seg<8192, THIRD, P, PK>), dim3((unsigned)g), dim3(WG_MAX), 0, st, a);
+    GBGPU_CHECK_LAUNCH("k_wave_big_seg");
+    return GBGPU_OK;
+}
+
+int launch_big_segment(int N, int precision, int pack_precision, bool third, const WaveArgs &a, hipStream_t st) {
+    if (N != 4096 && N != 8192) {
+        set_error("large-N segments take N = 4096 or 8192");
+        return GBGPU_E_ARG;
+    }
+    if (precision == GBGPU_MIXED)
+        return third ? big_segment<true, Mixed, Mixed>(N, a, st) : big_segment<false, Mixed, Mixed>(N, a, st);
+    if (pack_precision == GBGPU_MIXED)
+        return third ? big_segment<true, F64, Mixed>(N, a, st) : big_segment<false, F64, Mixed>(N, a, st);
+    return third ? big_segment<true, F64, F64>(N, a, st) : big_segment<false, F64, F64>(N, a, st);
 }
 
 }  // namespace gbgpu

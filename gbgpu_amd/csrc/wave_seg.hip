@@ -1,108 +1,27 @@
-// Batch segment kernels (gfx950): the fused likelihood over one device-side
-// segment of an uneven-N batch, for the N-bucket launches of
-// gbgpu_wave_ll_batched (batched.hip).
-#include <type_traits>
-
-#include "common.hpp"
-#include "wave_common.hpp"
-#include "wave_pass.hpp"
+// Batch segment kernels for circular binaries (templates in wave_seg.hpp),
+// and the segment dispatch of the N-bucket launches of gbgpu_wave_ll_batched.
+#include "wave_seg.hpp"
 
 namespace gbgpu {
 
-// Fused likelihood over one device-side segment of a batch: item i <
-// *a.seg_count is binary a.order[*a.seg_off + i]; the grid strides over the
-// segment.  The orbit-table and twiddle reads and the per-thread index
-// arithmetic are the same in every pass: opaque copies of the pointers and
-// of the thread index keep the compiler from hoisting all of them out of the
-// loop into registers (which would cost occupancy).
-template <int N, class P, class PK, int M, int DFT = GBGPU_TRANSFORM_LDS>
-__global__ void __launch_bounds__((Geo<N, M>::WG)) k_wave_seg(WaveArgs a) {
-    constexpr int BPW = Geo<N, M>::BPW;
-    __shared__ WaveLds<N, MODE_LL, false, P, PK, M, DFT> lds;
-    const long long count = *a.seg_count;
-    const long long first = *a.seg_off;
-    for (long long blk = blockIdx.x; blk * BPW < count; blk += gridDim.x) {
-        const double *tab = a.tab;
-        const void *twp = a.tw;
-        int tid = threadIdx.x;
-        asm volatile("" : "+s"(tab), "+s"(twp), "+v"(tid));
-        wave_pass<N, MODE_LL, false, P, PK, M, DFT>(a, lds, blk, count, first, tid, tab, twp);
-        __syncthreads();  // LDS is reused by the next pass
-    }
-}
+template int launch_ll_segment_t<false>(int, int, int, int, const WaveArgs &, hipStream_t);
+// instantiated in wave_seg3.hip
+extern template int launch_ll_segment_t<true>(int, int, int, int, const WaveArgs &, hipStream_t);
+// wave_big.hip
+int launch_big_segment(int N, int precision, int pack_precision, bool third, const WaveArgs &a, hipStream_t st);
 
-// Likelihood kernel over one device-side segment of a batch (the N-bucket
-// launches of gbgpu_wave_ll_batched): a.pack, a.order, a.seg_count and
-// a.seg_off are set by the caller; the segment's size is known only on the
-// device, so the grid covers a few passes of resident workgroups and strides.
-template <class P, class PK, int NN, int M, int DFT = GBGPU_TRANSFORM_LDS>
-static int launch_segment_n(const WaveArgs &a, hipStream_t st) {
-    // several passes' worth of resident workgroups (initialised once, thread-safe)
-    static const int grid = [] {
-        int dev = 0, ncu = 0;
-        (void)hipGetDevice(&dev);
-        (void)hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev);
-        return 16 * (ncu > 0 ? ncu : 256);
-    }();
-    long long g = (a.B + Geo<NN, M>::BPW - 1) / Geo<NN, M>::BPW;
-    if (g > grid) g = grid;
-    KernelTimer kt(st);
-    hipLaunchKernelGGL((k_wave_seg<NN, P, PK, M, DFT>), dim3((unsigned)g), dim3(Geo<NN, M>::WG), 0, st, a);
-    GBGPU_CHECK_LAUNCH("k_wave_seg");
-    return GBGPU_OK;
-}
-
-// variant 0: the waveform kernels' tile (wave_tile); variant 1: half the
-// threads per binary of that tile; VARIANT_DFT0.. (mixed N = 256 only): the
-// matrix-core transforms, one wavefront per binary.
-template <class P, class PK, int NN>
-static int launch_segment_m(int variant, const WaveArgs &a, hipStream_t st) {
-    constexpr int M0 = wave_tile<NN>();
-    constexpr int M1 = M0 == 0 ? 1 : 2;
-    if constexpr (NN == 256 && std::is_same<P, Mixed>::value && std::is_same<PK, Mixed>::value) {
-        if (variant >= VARIANT_DFT0 && !a.dftc) {
-            set_error("matrix-core transform requested without its constant table");
-            return GBGPU_E_ARG;
-        }
-        switch (variant - VARIANT_DFT0 + 1) {
-            case GBGPU_TRANSFORM_MFMA_F32:
-                return launch_segment_n<P, PK, NN, 1, GBGPU_TRANSFORM_MFMA_F32>(a, st);
-            case GBGPU_TRANSFORM_MFMA_BF16X3:
-                return launch_segment_n<P, PK, NN, 1, GBGPU_TRANSFORM_MFMA_BF16X3>(a, st);
-            case GBGPU_TRANSFORM_MFMA_BF16X6:
-                return launch_segment_n<P, PK, NN, 1, GBGPU_TRANSFORM_MFMA_BF16X6>(a, st);
-            default:
-                break;
-        }
-    }
-    return variant == 1 ? launch_segment_n<P, PK, NN, M1>(a, st) : launch_segment_n<P, PK, NN, M0>(a, st);
-}
-
-// precision: arithmetic of the pass; pack_precision: layout of a.pack;
-// variant: tile shape (0: wave_tile, 1: half the threads per binary).
-int launch_ll_segment(int N, int precision, int pack_precision, int variant, const WaveArgs &a, hipStream_t st) {
+// One N bucket's segment: N <= 2048 on the uniform kernel's tiles (variant
+// as for launch_ll_segment_t), N = 4096 and 8192 on the large-N tile.
+// precision: arithmetic of the pass; pack_precision: layout of a.pack.
+int launch_ll_segment(int N, int precision, int pack_precision, int variant, bool third, const WaveArgs &a,
+                      hipStream_t st) {
     if (precision == GBGPU_MIXED && pack_precision != GBGPU_MIXED) {
         set_error("mixed segments need mixed per-binary constants");
         return GBGPU_E_ARG;
     }
-#define GBGPU_SEG(NN)                                                                           \
-    case NN:                                                                                    \
-        if (precision == GBGPU_MIXED) return launch_segment_m<Mixed, Mixed, NN>(variant, a, st); \
-        if (pack_precision == GBGPU_MIXED) return launch_segment_m<F64, Mixed, NN>(variant, a, st); \
-        return launch_segment_m<F64, F64, NN>(variant, a, st);
-    switch (N) {
-        GBGPU_SEG(32)
-        GBGPU_SEG(64)
-        GBGPU_SEG(128)
-        GBGPU_SEG(256)
-        GBGPU_SEG(512)
-        GBGPU_SEG(1024)
-        GBGPU_SEG(2048)
-        default:
-            set_error("N must be a power of two in [32, 2048]");
-            return GBGPU_E_ARG;
-    }
-#undef GBGPU_SEG
+    if (N == 4096 || N == 8192) return launch_big_segment(N, precision, pack_precision, third, a, st);
+    return third ? launch_ll_segment_t<true>(N, precision, pack_precision, variant, a, st)
+                 : launch_ll_segment_t<false>(N, precision, pack_precision, variant, a, st);
 }
 
 }  // namespace gbgpu

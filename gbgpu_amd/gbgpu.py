@@ -465,29 +465,36 @@ class GBGPUBase(abc.ABC):
 
     def _get_ll_uneven(self, params, data, psd, N, T, dt, tdi2, precision, phase_marginalize, start_freq_ind,
                        data_index, noise_index):
-        """get_ll with one N per binary: one persistent launch over N buckets."""
+        """get_ll with one N per binary (powers of two in [32, 8192]): one
+        gbgpu_wave_ll_batched call buckets the binaries by N on the device and
+        runs each bucket's fused kernel; same arithmetic as the uniform get_ll
+        (no FP64 re-evaluation of loud binaries).  Third-body rows, tdi2 and
+        data_index / noise_index as in get_ll."""
         from .batch import BatchedLogLike
 
-        if tdi2 or data_index is not None or noise_index is not None or params.shape[0] != 9:
-            raise NotImplementedError("per-binary N supports circular binaries, TDI1 and a single data stream")
         if len(N) != params.shape[1]:
             raise ValueError("N must have one entry per binary")
+        for n in np.unique(N):
+            _check_N(int(n))
         if T > np.max(self.orbits.t_base):
             raise ValueError(
                 f"Observation time ({T}) is larger than length of time in orbital information "
                 f"({np.max(self.orbits.t_base)})"
             )
+        if params.shape[0] > 9:
+            # the extra rows through the class's own hook, as run_wave does
+            params = np.vstack([params[:9], np.array(self.prepare_additional_args(*params[9:]))])
         # 2-D streams without indices: every binary uses stream 0 (data_index
         # and noise_index default to zero, src/gbgpu/gbgpu.py:675-678), and a
         # window reaching past data_length is clipped there
-        if data[0].ndim == 2:
+        if data[0].ndim == 2 and data_index is None:
             data = [data[0][0], data[1][0]]
-        if psd[0].ndim == 2:
+        if psd[0].ndim == 2 and noise_index is None:
             psd = [psd[0][0], psd[1][0]]
-        # same arithmetic as the uniform get_ll (no FP64 re-evaluation of loud binaries)
         bl = BatchedLogLike(self, data, psd, T, dt, precision=precision or self.precision,
                             start_freq_ind=start_freq_ind, N_levels=(int(N.min()),), f_edges=(),
-                            phase_marginalize=phase_marginalize, guard_tol=None)
+                            phase_marginalize=phase_marginalize, guard_tol=None, tdi2=tdi2,
+                            data_index=data_index, noise_index=noise_index)
         ll, d_h, h_h = bl(params, N=N.astype(np.int32), d_h=True)
         self.num_bin, self.N, self.df = params.shape[1], N, 1 / bl.T
         if phase_marginalize:

@@ -229,7 +229,7 @@ typedef struct gbgpu_batch_cfg {
     double armlength;              /* metres */
     int tdi2;
     int precision;                 /* GBGPU_FP64 or GBGPU_MIXED */
-    const double *orbit_tables[7]; /* table for N = 32 << k (k < 7), NULL when no binary uses it */
+    const double *orbit_tables[9]; /* table for N = 32 << k (k < 9), NULL when no binary uses it */
     int strategy;                  /* GBGPU_BATCH_BUCKETS: one launch per N bucket (default);
                                       GBGPU_BATCH_PERSISTENT: one persistent launch */
     double guard_tol;              /* mixed precision with GBGPU_BATCH_BUCKETS: binaries whose
@@ -246,12 +246,15 @@ typedef struct gbgpu_batch_cfg {
 long long gbgpu_batch_workspace_bytes(int num_bin, int precision);
 
 /* Fused wave + log-likelihood for a batch with per-binary N (N_bin, device
- * int32, powers of two in [32, 2048]): binaries are bucketed on the device
+ * int32, powers of two in [32, 8192]): binaries are bucketed on the device
  * by N and start bin, then either each bucket runs the uniform fused kernel
  * over its device-side segment (GBGPU_BATCH_BUCKETS) or one persistent
  * launch pulls N-bucketed work items from an atomic queue, largest N first
- * (GBGPU_BATCH_PERSISTENT, switched off in this build: GBGPU_E_ARG).  The reference needs one run_wave +
- * get_ll call per distinct N (src/gbgpu/gbgpu.py:205-207, gbgpu_utils.cu:126).
+ * (GBGPU_BATCH_PERSISTENT, switched off in this build: GBGPU_E_ARG).
+ * num_params is 9 or 14 (third-body rows); cfg->tdi2 and the data's
+ * data_index / noise_index apply as in gbgpu_wave_ll.  The reference needs
+ * one run_wave + get_ll call per distinct N (src/gbgpu/gbgpu.py:205-207,
+ * gbgpu_utils.cu:126).
  * Outputs as gbgpu_wave_ll; binaries with an unsupported N or a NULL orbit
  * table get NaN.  workspace: gbgpu_batch_workspace_bytes() of device memory,
  * or NULL to allocate stream-ordered scratch.  With a workspace and after one
@@ -270,7 +273,7 @@ int gbgpu_wave_ll_batched(double *loglike, gbgpu_cmplx *d_h, gbgpu_cmplx *h_h, c
  * cfg->transform is GBGPU_TRANSFORM_LDS, also 2 and 4: the
  * GBGPU_TRANSFORM_MFMA_F32 and _BF16X6 transforms) three times, times
  * every bucket with HIP events, and keeps the fastest variant per (precision,
- * bucket) for all later calls in the process.  chosen (7 ints, N = 32 << k,
+ * bucket) for all later calls in the process.  chosen (9 ints, N = 32 << k,
  * may be NULL) receives the pick, -1 for buckets without an orbit table.
  * loglike receives the last run's results.  Synchronises `stream`; call it
  * outside graph capture.  No counterpart in the reference. */
@@ -278,13 +281,13 @@ int gbgpu_batch_autotune(double *loglike, const double *params, int num_params, 
                          const gbgpu_batch_cfg *cfg, const gbgpu_data *data, double d_d, void *workspace,
                          int *chosen, void *stream);
 
-/* Sets the per-bucket variants (7 ints: 0 or 1 as for the autotuner; for the
+/* Sets the per-bucket variants (9 ints: 0 or 1 as for the autotuner; for the
  * GBGPU_MIXED N = 256 bucket also 1 + t, the matrix-core transform t =
  * GBGPU_TRANSFORM_MFMA_*) used by GBGPU_BATCH_BUCKETS calls of the given
  * precision. */
 int gbgpu_batch_set_variants(int precision, const int *variants);
 
-/* Per-bucket counts (7 ints, N = 32 << k) of binaries the accuracy guard
+/* Per-bucket counts (9 ints, N = 32 << k) of binaries the accuracy guard
  * evaluated in FP64 (routed at set-up plus re-evaluated after the mixed
  * pass) in the last gbgpu_wave_ll_batched call that used this workspace.
  * Diagnostic: copies to host memory and synchronises `stream`. */
@@ -304,7 +307,7 @@ int gbgpu_perturb_params(double *out, const double *in, int num_params, int num_
 /* Message describing the last failure on the calling thread. */
 const char *gbgpu_last_error(void);
 
-/* ABI version of this header (5). */
+/* ABI version of this header (6). */
 int gbgpu_abi_version(void);
 
 /* Measurement hook with no counterpart in the reference: while enabled,
