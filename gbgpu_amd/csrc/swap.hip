@@ -156,18 +156,13 @@ static int launch_swap(int N, WaveArgs a, double *dll, double *terms, hipStream_
     hipLaunchKernelGGL((k_setup<P>), dim3((unsigned)((a.B + 255) / 256)), dim3(256), 0, st, a, N, pk);
     a.pack = pk;
     int rc = GBGPU_OK;
-#define GBGPU_CASE(NN)                                                                                     \
-    case NN: {                                                                                             \
-        if constexpr (!THIRD || NN <= 1024) {                                                              \
-            const long long grid = (a.B + SwapGeo<NN>::BPW - 1) / SwapGeo<NN>::BPW;                        \
-            KernelTimer kt(st);                                                                            \
-            hipLaunchKernelGGL((k_swap<NN, THIRD, P>), dim3((unsigned)grid), dim3(SwapGeo<NN>::WG), 0, st, \
-                               a, dll, terms);                                                             \
-        } else {                                                                                           \
-            set_error("third-body swap supports N <= 1024 (LDS limit)");                                   \
-            rc = GBGPU_E_ARG;                                                                              \
-        }                                                                                                  \
-        break;                                                                                             \
+#define GBGPU_CASE(NN)                                                                                 \
+    case NN: {                                                                                         \
+        const long long grid = (a.B + SwapGeo<NN>::BPW - 1) / SwapGeo<NN>::BPW;                        \
+        KernelTimer kt(st);                                                                            \
+        hipLaunchKernelGGL((k_swap<NN, THIRD, P>), dim3((unsigned)grid), dim3(SwapGeo<NN>::WG), 0, st, \
+                           a, dll, terms);                                                             \
+        break;                                                                                         \
     }
     switch (N) {
         GBGPU_CASE(32)

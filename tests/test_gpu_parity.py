@@ -496,18 +496,23 @@ def test_swap_likelihood_difference_matches_bruteforce(precision, N):
     assert np.all(np.abs(dll - ref) <= 4 * wbar * scale + 1e-9), np.max(np.abs(dll - ref) / scale)
 
 
-def test_swap_third_body_proposal():
+@pytest.mark.parametrize("N,e2", [(256, 0.0), (2048, 0.3)])
+def test_swap_third_body_proposal(N, e2):
+    """Third-body add proposals against brute force; at N = 2048 a proposal
+    pair fills a 512-thread workgroup (128 KB of transform buffers), with
+    eccentric outer orbits."""
     from gbgpu_amd import GBGPUThirdBody
 
-    rng = np.random.default_rng(77)
-    B, N = 8, 256
+    rng = np.random.default_rng(77 + N)
+    B = 8
     p = draw_params(B, rng)
     T = YR
     A, E, SA, SE, df, L = make_data(p, N, T, DT, rng)
     zeros = np.zeros(B)
     rem = np.concatenate([p, [zeros, zeros, zeros, np.ones(B), zeros]])  # A2 = 0: plain binary
-    add = np.concatenate([p, [rng.uniform(0, 100, B), rng.uniform(0, 2 * np.pi, B), zeros,
+    add = np.concatenate([p, [rng.uniform(0, 100, B), rng.uniform(0, 2 * np.pi, B), np.full(B, e2),
                               10 ** rng.uniform(0, 1, B), zeros]])
+    print(f"third-body swap N={N} e2={e2}", flush=True)
     gb = GBGPUThirdBody()
     dll = gb.swap_likelihood_difference(rem, add, [A, E], [SA, SE], N=N, T=T, dt=DT)
     o_r = F.run_wave(*rem, N=N, T=T, dt=DT)
