@@ -95,6 +95,19 @@ SIGNATURES = {
         c_int,
         [c_void_p, c_void_p, c_void_p, c_void_p, c_int, c_int, ctypes.POINTER(WaveCfg), c_void_p],
     ),
+    "gbgpu_hook_inputs": (
+        c_int,
+        [c_void_p, c_void_p, c_void_p, c_void_p, c_int, c_int, ctypes.POINTER(WaveCfg), c_void_p],
+    ),
+    "gbgpu_run_wave_ext": (
+        c_int,
+        [c_void_p, c_void_p, c_void_p, c_void_p, c_int, c_int, ctypes.POINTER(WaveCfg), c_void_p, c_void_p],
+    ),
+    "gbgpu_wave_ll_ext": (
+        c_int,
+        [c_void_p, c_void_p, c_void_p, c_void_p, c_int, c_int, ctypes.POINTER(WaveCfg), ctypes.POINTER(Data),
+         c_double, c_int, c_void_p, c_void_p],
+    ),
     "gbgpu_run_wave_ae": (
         c_int,
         [c_void_p, c_void_p, c_void_p, c_int, c_int, ctypes.POINTER(WaveCfg), c_void_p],

@@ -44,6 +44,16 @@ struct F64 {
         slow_xyz_f64<THIRD, TIER>(s, g, fstar_inv, scale, v3d, t3, X, Y, Z, F);
     }
 
+    // Derived waveforms: f_i and argS of the three spacecraft at sample n
+    // given (fi3, argS3), as the class's hooks returned them.
+    template <int TIER>
+    static __device__ __forceinline__ void sample_ext(const Src &s, const double *__restrict__ tab, int N, int n,
+                                                      double fstar_inv, double scale, const double *fi3,
+                                                      const double *argS3, CT &X, CT &Y, CT &Z, CT &F) {
+        const SampleGeom g = load_geom(s, tab, N, n);
+        slow_xyz_f64<false, TIER, true>(s, g, fstar_inv, scale, fi3, argS3, X, Y, Z, F);
+    }
+
     static __device__ __forceinline__ double2 fac(const Src &s) { return make_double2(s.tdr, s.tdi); }
     static __device__ __forceinline__ double2 up(CT v) { return v; }
     static const void *twiddles(int N) { return twiddles_f64(N); }

@@ -293,13 +293,16 @@ __device__ __forceinline__ double2 link_f64(double fonfs, double kr, double cphi
 // which is returned separately so that the caller applies it once per
 // output channel (A and E for the likelihood).
 // v3[sc] (third-body v_LOS at xi_sc) and term3[sc] (its phase integral) are
-// used only when THIRD.
-template <bool THIRD, int TIER>
+// used only when THIRD.  EXT (derived waveforms): v3[sc] and term3[sc] are
+// the frequency f_i and the phase argS of spacecraft sc at this sample as
+// the class's hooks returned them (src/gbgpu/gbgpu.py:453, :505).
+template <bool THIRD, int TIER, bool EXT = false>
 __device__ __forceinline__ void slow_xyz_f64(const Src &s, const SampleGeom &g, double fstar_inv, double scale,
                                              const double *v3, const double *term3, double2 &X, double2 &Y,
                                              double2 &Z, double2 &F) {
+    static_assert(!(THIRD && EXT), "third-body and external hooks are exclusive");
     double fonfs[3], cphi[3], sphi[3];
-    if constexpr (!THIRD && TIER < NTIER) {
+    if constexpr (!THIRD && !EXT && TIER < NTIER) {
         // Guiding-centre split: with xi_c = t - k.Pc/c and d_s = k.dP_s/c,
         //   Phi_s = Phi_c + 2 pi f(xi_c) d_s - pi f'(xi_c) d_s^2 + (pi/3) fddot d_s^3,
         // exact for the cubic argS.  Phi_c takes the one range-reduced
@@ -328,11 +331,17 @@ __device__ __forceinline__ void slow_xyz_f64(const Src &s, const SampleGeom &g, 
 #pragma unroll
         for (int sc = 0; sc < 3; sc++) {
             const double xi = g.t - g.kP[sc];
-            double fi = s.f0 + s.fdot * xi + 0.5 * s.fddot * xi * xi;
-            if (THIRD) fi *= 1.0 + v3[sc] / C_SI;
+            double fi, argS;
+            if constexpr (EXT) {
+                fi = v3[sc];
+                argS = term3[sc];
+            } else {
+                fi = s.f0 + s.fdot * xi + 0.5 * s.fddot * xi * xi;
+                if (THIRD) fi *= 1.0 + v3[sc] / C_SI;
+                argS = s.phi0 + s.omq * g.t + PI * s.fdot * (xi * xi) + (1.0 / 3.0) * PI * s.fddot * (xi * xi * xi);
+                if (THIRD) argS += term3[sc];
+            }
             fonfs[sc] = fi * fstar_inv;
-            double argS = s.phi0 + s.omq * g.t + PI * s.fdot * (xi * xi) + (1.0 / 3.0) * PI * s.fddot * (xi * xi * xi);
-            if (THIRD) argS += term3[sc];
             const double Phi = s.om * g.kP[sc] - argS;
             sincos_f64(Phi, sphi[sc], cphi[sc]);
         }

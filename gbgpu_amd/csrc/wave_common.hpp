@@ -84,6 +84,9 @@ struct WaveArgs {
     unsigned *keys;    // k_setup output: start-bin sort keys, or null
     int *vals;         // k_setup output: binary indices, or null
     const void *dftc;  // per-lane constants of the matrix-core transform (dft256_table)
+    // derived waveforms: (B, 2, 3, N) f_i then argS per binary, spacecraft and
+    // sample, as the class's hooks returned them (null otherwise)
+    const double *ext;
 };
 
 // Bound on |logL_mixed - logL_FP64| / (<h|h> + |<d|h>|) for the mixed
@@ -272,12 +275,13 @@ struct ThirdScratch {
 // register layout gives it, xr[c][i] holding sample dft256_index(lane, i) of
 // channel c, and dft256_mfma transforms them in place (same index on exit);
 // buf and tw are not touched.
-template <int N, int NCH, bool THIRD, class P, int M = 1, int DFT = GBGPU_TRANSFORM_LDS, int C0 = 0, int NCP = NCH>
+template <int N, int NCH, bool THIRD, class P, int M = 1, int DFT = GBGPU_TRANSFORM_LDS, int C0 = 0, int NCP = NCH,
+          bool EXT = false>
 __device__ __forceinline__ void binary_spectrum(const Src &s, const typename P::Aux &aux, bool valid, int lane,
                                                 const double *__restrict__ tab, double fstar_inv, double scale,
                                                 typename P::CT *buf, const typename P::CT *__restrict__ tw,
                                                 ThirdScratch ts, double *tot, typename P::CT *xr = nullptr,
-                                                const void *dftc = nullptr) {
+                                                const void *dftc = nullptr, const double *ext = nullptr) {
     using G = Geo<N, M>;
     using CT = typename P::CT;
     constexpr int TPB = G::TPB, SPT = G::SPT;
@@ -286,6 +290,8 @@ __device__ __forceinline__ void binary_spectrum(const Src &s, const typename P::
                   "the matrix-core transform covers mixed N = 256 without a third body");
     static_assert(C0 >= 0 && NCP >= 1 && C0 + NCP <= NCH && (DFT == GBGPU_TRANSFORM_LDS || NCP == NCH),
                   "channel pass out of range");
+    static_assert(!EXT || (!THIRD && DFT == GBGPU_TRANSFORM_LDS && std::is_same<P, F64>::value),
+                  "derived-waveform hooks run on the FP64 circular path with the LDS transform");
     constexpr int NT = THIRD ? SPT : 1;
     float vf[3][NT];   // v_LOS at xi_sc(t_n)
     double gc[3][NT];  // the cumulative phase term at t_n
@@ -386,7 +392,18 @@ __device__ __forceinline__ void binary_spectrum(const Src &s, const typename P::
                         t3[sc] = gc[sc][q];
                     }
                 }
-                P::template sample<THIRD, TIER>(s, aux, tab, N, n, fstar_inv, scale, v3, t3, X, Y, Z, F);
+                if constexpr (EXT) {
+                    // this binary's hook outputs: ext[(0, sc, n)] = f_i, ext[(1, sc, n)] = argS
+                    double fi3[3], ar3[3];
+#pragma unroll
+                    for (int sc = 0; sc < 3; sc++) {
+                        fi3[sc] = ext[sc * N + n];
+                        ar3[sc] = ext[(3 + sc) * N + n];
+                    }
+                    F64::template sample_ext<TIER>(s, tab, N, n, fstar_inv, scale, fi3, ar3, X, Y, Z, F);
+                } else {
+                    P::template sample<THIRD, TIER>(s, aux, tab, N, n, fstar_inv, scale, v3, t3, X, Y, Z, F);
+                }
             }
             CT out[NCH];
             if constexpr (NCH == 3) {
@@ -408,7 +425,11 @@ __device__ __forceinline__ void binary_spectrum(const Src &s, const typename P::
             }
         }
     };
-    with_tier(s.tier, samples);
+    // hooked frequencies may leave the binary's tier: reduced sine/cosine throughout
+    if constexpr (EXT)
+        samples(std::integral_constant<int, NTIER>{});
+    else
+        with_tier(s.tier, samples);
     if constexpr (DFT != GBGPU_TRANSFORM_LDS) {
         const Dft256Lane<DFT> k = load_dft256<DFT>(dftc, lane);
 #pragma unroll

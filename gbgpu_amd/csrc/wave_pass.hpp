@@ -42,7 +42,8 @@ __device__ __forceinline__ typename P::Aux aux_of(const Pack<PK> &pk) {
         return P::make_aux(pk.s);
 }
 
-template <int N, int MODE, bool THIRD, class P, class PK = P, int M = 1, int DFT = GBGPU_TRANSFORM_LDS>
+template <int N, int MODE, bool THIRD, class P, class PK = P, int M = 1, int DFT = GBGPU_TRANSFORM_LDS,
+          bool EXT = false>
 __device__ __forceinline__ void wave_pass(const WaveArgs &a, WaveLds<N, MODE, THIRD, P, PK, M, DFT> &L,
                                           long long blk, long long count, long long first, int tid,
                                           const double *tab, const void *twp) {
@@ -76,9 +77,10 @@ __device__ __forceinline__ void wave_pass(const WaveArgs &a, WaveLds<N, MODE, TH
 
     constexpr bool REG = DFT != GBGPU_TRANSFORM_LDS;
     CT xr[REG ? NCH * SPT : 1];  // register-resident spectra (matrix-core transform)
-    binary_spectrum<N, NCH, THIRD, P, M, DFT>(s, per_group<TPB>(aux_of<P, PK>(*pk)), valid, lane, tab,
-                                              a.fstar_inv, a.scale, buf, static_cast<const CT *>(twp), ts, tot,
-                                              xr, a.dftc);
+    binary_spectrum<N, NCH, THIRD, P, M, DFT, 0, NCH, EXT>(s, per_group<TPB>(aux_of<P, PK>(*pk)), valid, lane, tab,
+                                                           a.fstar_inv, a.scale, buf, static_cast<const CT *>(twp),
+                                                           ts, tot, xr, a.dftc,
+                                                           (EXT && valid) ? a.ext + b * 6 * N : nullptr);
     // bin of output position q of this thread: fftshifted position m(q),
     // spectrum value spec(c, q) of channel c
     auto pos = [&](int q) { return REG ? (dft256_index(lane, q) + N / 2) & (N - 1) : lane + q * TPB; };

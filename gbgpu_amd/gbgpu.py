@@ -202,33 +202,48 @@ class GBGPUBase(abc.ABC):
         is NaN: it cancels to O((f/f*)^2) of X, below the FP32 spectra's
         error.
         """
-        self._check_hooks()
+        if self._user_hooks:
+            params, N, T, add_args = self._prepare_hooked(amp, f0, fdot, fddot, phi0, iota, psi, lam, beta, args, N,
+                                                          T, dt, oversample)
+            self._run_wave_params(params, N, T, tdi2, precision, add_args)
+            return
         params, N, T = self._prepare(amp, f0, fdot, fddot, phi0, iota, psi, lam, beta, args, N, T, dt, oversample)
         self._run_wave_params(params, N, T, tdi2, precision)
 
-    def _run_wave_params(self, params, N, T, tdi2, precision):
+    def _run_wave_params(self, params, N, T, tdi2, precision, add_args=None):
+        """add_args: the extra arguments of a derived waveform with its own
+        hooks (None otherwise)."""
         torch = _torch()
         B = params.shape[1]
         self.num_bin = B
         self.N = N
         self.df = 1 / T
-        cfg, _tab = self._cfg(N, T, tdi2, precision, self._transform_code(N, precision, params.shape[0]))
+        hooked = add_args is not None
+        if hooked and (precision or self.precision) != "fp64":
+            raise ValueError("derived waveforms run in FP64")
+        cfg, _tab = self._cfg(N, T, tdi2, "fp64" if hooked else precision,
+                              0 if hooked else self._transform_code(N, precision, params.shape[0]))
         dev = params.device
         XYZf = torch.empty((B, 3, N), dtype=torch.complex128, device=dev)
         AETf = torch.empty((B, 3, N), dtype=torch.complex128, device=dev)
         start = torch.empty((B,), dtype=torch.int64, device=dev)
         lib = _capi.load()
-        rc = lib.gbgpu_run_wave(XYZf.data_ptr(), AETf.data_ptr(), start.data_ptr(), params.data_ptr(),
-                                params.shape[0], B, cfg, _capi.stream_handle())
-        _capi.check(rc, "gbgpu_run_wave")
+        if hooked:
+            ext = self._hook_outputs(params, N, T, add_args)
+            rc = lib.gbgpu_run_wave_ext(XYZf.data_ptr(), AETf.data_ptr(), start.data_ptr(), params.data_ptr(), 9, B,
+                                        cfg, ext.data_ptr(), _capi.stream_handle())
+            _capi.check(rc, "gbgpu_run_wave_ext")
+        else:
+            rc = lib.gbgpu_run_wave(XYZf.data_ptr(), AETf.data_ptr(), start.data_ptr(), params.data_ptr(),
+                                    params.shape[0], B, cfg, _capi.stream_handle())
+            _capi.check(rc, "gbgpu_run_wave")
         self._XYZf, self._AETf = XYZf, AETf
         self.start_inds = self.kmin = start
         self._wave_state = None
 
     def _materialize(self):
         if self._wave_state is not None:
-            params, N, T, tdi2, precision = self._wave_state
-            self._run_wave_params(params, N, T, tdi2, precision)
+            self._run_wave_params(*self._wave_state)
 
     @property
     def XYZf(self):
@@ -363,7 +378,6 @@ class GBGPUBase(abc.ABC):
                 "self.d_d attribute must be set before computing log-Likelihood. This attribute is the data "
                 "with data inner product (<d|d>)."
             )
-        self._check_hooks()
         torch = _torch()
         tdi2 = kwargs.pop("tdi2", False)
         precision = kwargs.pop("precision", None)
@@ -375,14 +389,23 @@ class GBGPUBase(abc.ABC):
             raise TypeError(f"unexpected keyword arguments {list(kwargs)}")
         params = np.asarray(params) if not torch.is_tensor(params) else params.cpu().numpy()
         self._check_data(data, data_index, noise_index)
+        hooked = self._user_hooks
         if N is not None and np.ndim(N) == 1:
+            self._check_hooks("get_ll with one N per binary")
             return self._get_ll_uneven(params, data, psd, np.asarray(N), T, dt, tdi2, precision, phase_marginalize,
                                        start_freq_ind, data_index, noise_index)
-        p, N, T = self._prepare(*params[:9], params[9:], N, T, dt, oversample)
+        if hooked:
+            if (precision or self.precision) != "fp64":
+                raise ValueError("derived waveforms run in FP64")
+            p, N, T, add_args = self._prepare_hooked(*params[:9], params[9:], N, T, dt, oversample)
+        else:
+            p, N, T = self._prepare(*params[:9], params[9:], N, T, dt, oversample)
+            add_args = None
         B = p.shape[1]
         self.num_bin, self.N, self.df = B, N, 1 / T
         dA, dE, SA, SE, di, ni, L = self._prep_data(data, psd, data_index, noise_index, B)
-        cfg, _tab = self._cfg(N, T, tdi2, precision, self._transform_code(N, precision, p.shape[0]))
+        cfg, _tab = self._cfg(N, T, tdi2, "fp64" if hooked else precision,
+                              0 if hooked else self._transform_code(N, precision, p.shape[0]))
         dev = p.device
         ll = torch.empty(B, dtype=torch.float64, device=dev)
         d_h = torch.empty(B, dtype=torch.complex128, device=dev)
@@ -391,13 +414,20 @@ class GBGPUBase(abc.ABC):
                          data_index=di.data_ptr(), noise_index=ni.data_ptr(), data_length=int(L),
                          start_freq_ind=int(start_freq_ind), df=float(1 / T))
         lib = _capi.load()
-        rc = lib.gbgpu_wave_ll(ll.data_ptr(), d_h.data_ptr(), h_h.data_ptr(), p.data_ptr(), p.shape[0], B, cfg,
-                               dat, float(np.real(self.d_d)), int(bool(phase_marginalize)), _capi.stream_handle())
-        _capi.check(rc, "gbgpu_wave_ll")
+        if hooked:
+            ext = self._hook_outputs(p, N, T, add_args)
+            rc = lib.gbgpu_wave_ll_ext(ll.data_ptr(), d_h.data_ptr(), h_h.data_ptr(), p.data_ptr(), 9, B, cfg, dat,
+                                       float(np.real(self.d_d)), int(bool(phase_marginalize)), ext.data_ptr(),
+                                       _capi.stream_handle())
+            _capi.check(rc, "gbgpu_wave_ll_ext")
+        else:
+            rc = lib.gbgpu_wave_ll(ll.data_ptr(), d_h.data_ptr(), h_h.data_ptr(), p.data_ptr(), p.shape[0], B, cfg,
+                                   dat, float(np.real(self.d_d)), int(bool(phase_marginalize)), _capi.stream_handle())
+            _capi.check(rc, "gbgpu_wave_ll")
         # start bins without materialising templates: rint(f0 T) - N/2
         f0 = p[1]
         self.start_inds = self.kmin = (torch.round(f0 * T).to(torch.int64) - N // 2)
-        self._wave_state = (p, N, T, tdi2, precision)
+        self._wave_state = (p, N, T, tdi2, precision, add_args)
         if phase_marginalize:
             self.non_marg_d_h = d_h.cpu().numpy()
             d_h = torch.abs(d_h).to(torch.complex128)
@@ -425,7 +455,7 @@ class GBGPUBase(abc.ABC):
         if phase_marginalize:
             raise NotImplementedError("phase marginalisation is not defined for a swap difference")
         torch = _torch()
-        self._check_hooks()
+        self._check_hooks("swap_likelihood_difference")
         tdi2 = kwargs.pop("tdi2", False)
         precision = kwargs.pop("precision", None)
         N = kwargs.pop("N", None)
@@ -566,7 +596,6 @@ class GBGPUBase(abc.ABC):
         ``AETf``, ``A``, ...) are computed on first access, as after get_ll.
         """
         torch = _torch()
-        self._check_hooks()
         params = np.asarray(params).T
         N = kwargs.pop("N", None)
         T = kwargs.pop("T", 4 * YRSID_SI)
@@ -576,6 +605,13 @@ class GBGPUBase(abc.ABC):
         precision = kwargs.pop("precision", None)
         if kwargs:
             raise TypeError(f"unexpected keyword arguments {list(kwargs)}")
+        if self._user_hooks:
+            # derived waveform: its hooked run_wave, then the fill
+            self.run_wave(*params[:9], *params[9:], N=N, T=T, dt=dt, oversample=oversample, tdi2=tdi2,
+                          precision=precision)
+            self.fill_global_template(group_index, templates, self.AETf[:, 0], self.AETf[:, 1], self.start_inds,
+                                      self.N, start_freq_ind=start_freq_ind)
+            return
         p, N, T = self._prepare(*params[:9], params[9:], N, T, dt, oversample)
         B = p.shape[1]
         self.num_bin, self.N, self.df = B, N, 1 / T
@@ -586,7 +622,7 @@ class GBGPUBase(abc.ABC):
                                             _capi.stream_handle())
         _capi.check(rc, "gbgpu_run_wave_ae")
         self.start_inds = self.kmin = start
-        self._wave_state = (p, N, T, tdi2, precision)
+        self._wave_state = (p, N, T, tdi2, precision, None)
         self.fill_global_template(group_index, templates, AE[:, 0], AE[:, 1], start, N,
                                   start_freq_ind=start_freq_ind)
 
@@ -630,7 +666,7 @@ class GBGPUBase(abc.ABC):
     def _deriv_setup(self, params, N, T, dt, oversample, kwargs_left):
         if kwargs_left:
             raise TypeError(f"unexpected keyword arguments {list(kwargs_left)}")
-        self._check_hooks()
+        self._check_hooks("the exact derivative kernels")
         if type(self)._kernel_variant != "circular":
             raise NotImplementedError("derivatives are implemented for circular (9-parameter) binaries")
         torch = _torch()
@@ -745,7 +781,8 @@ class GBGPUBase(abc.ABC):
         if inds is None:
             inds = np.arange(num_params)
         inds = np.asarray(inds)
-        exact_ok = method == "exact" and num_params == 9 and N <= 1024 and type(self)._kernel_variant == "circular"
+        exact_ok = (method == "exact" and num_params == 9 and N <= 1024 and type(self)._kernel_variant == "circular"
+                    and not self._user_hooks)
         if method not in ("exact", "fd"):
             raise ValueError("method must be 'exact' or 'fd'")
         if not exact_ok:
@@ -854,18 +891,63 @@ class GBGPUBase(abc.ABC):
         return info.cpu().numpy()
 
     # ------------------------------------------------------------- hooks ---
-    def _check_hooks(self):
-        variant = type(self)._kernel_variant
-        base_hooks = {"add_to_argS", "prepare_additional_args", "special_get_N", "shift_frequency"}
+    @property
+    def _user_hooks(self):
+        """True when a class outside this package defines the slow-part hooks
+        (shift_frequency / add_to_argS, src/gbgpu/gbgpu.py:1140-1229):
+        run_wave, get_ll and generate_global_template then evaluate them on
+        device arrays between gbgpu_hook_inputs and gbgpu_run_wave_ext /
+        gbgpu_wave_ll_ext.  The package's own classes (circular, third body)
+        evaluate their hooks inside the kernels."""
+        pkg = __name__.split(".")[0]
         for cls in type(self).__mro__:
-            if cls in (GBGPU, GBGPUBase, object) or getattr(cls, "_kernel_variant", None) != variant:
+            if cls is object or cls.__module__.split(".")[0] == pkg:
                 continue
-            overridden = base_hooks & set(vars(cls))
-            if overridden and cls.__module__.split(".")[0] != __name__.split(".")[0]:
-                raise NotImplementedError(
-                    f"{cls.__name__} overrides {sorted(overridden)}: the GPU kernels implement only the circular "
-                    "and third-body hook sets"
-                )
+            if {"add_to_argS", "shift_frequency"} & set(vars(cls)):
+                return True
+        return False
+
+    def _check_hooks(self, what="this method"):
+        if self._user_hooks:
+            raise NotImplementedError(
+                f"{type(self).__name__} defines its own slow-part hooks: {what} runs the kernels' own circular "
+                "and third-body hook sets only (run_wave, get_ll and generate_global_template take derived waveforms)"
+            )
+
+    def _prepare_hooked(self, amp, f0, fdot, fddot, phi0, iota, psi, lam, beta, args, N, T, dt, oversample):
+        """_prepare for a derived waveform: the 9 base rows go to the kernels;
+        the extra arguments pass through prepare_additional_args to the hooks
+        (src/gbgpu/gbgpu.py:197-207)."""
+        T_adj = int(T / dt) * dt
+        add_args = self.prepare_additional_args(*args) if len(args) else ()
+        if N is None:
+            N = int(np.max(self.special_get_N(np.atleast_1d(amp), np.atleast_1d(f0), T_adj, *args,
+                                              oversample=oversample)))
+        params, N, T = self._prepare(amp, f0, fdot, fddot, phi0, iota, psi, lam, beta, (), N, T, dt, oversample)
+        return params, N, T, add_args
+
+    def _hook_outputs(self, params, N, T, add_args):
+        """Runs the class's hooks on device arrays and returns (B, 2, 3, N):
+        the hooked f_i, then argS, for every binary, spacecraft and sample
+        (src/gbgpu/gbgpu.py:441-453, :497-505)."""
+        torch = _torch()
+        if N > 2048:
+            raise ValueError(f"N={N}: derived waveforms take N <= 2048")
+        B = params.shape[1]
+        cfg, _tab = self._cfg(N, T, False, "fp64")
+        xi, fi, argS = (torch.empty((B, 3, N), dtype=torch.float64, device=params.device) for _ in range(3))
+        rc = _capi.load().gbgpu_hook_inputs(xi.data_ptr(), fi.data_ptr(), argS.data_ptr(), params.data_ptr(), 9, B,
+                                            cfg, _capi.stream_handle())
+        _capi.check(rc, "gbgpu_hook_inputs")
+        fi = self.shift_frequency(fi, xi, *add_args)
+        argS = self.add_to_argS(argS, params[1], params[2], params[3], xi, *add_args)
+        out = []
+        for name, x in (("shift_frequency", fi), ("add_to_argS", argS)):
+            x = torch.as_tensor(x).to(device=params.device, dtype=torch.float64)
+            if tuple(x.shape) != (B, 3, N):
+                raise ValueError(f"{name} must return a ({B}, 3, {N}) array, got {tuple(x.shape)}")
+            out.append(x)
+        return torch.stack(out, dim=1).contiguous()
 
     @abc.abstractmethod
     def add_to_argS(self, argS, f0, fdot, fddot, xi, *args):

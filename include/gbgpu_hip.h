@@ -302,12 +302,36 @@ int gbgpu_perturb_params(double *out, const double *in, int num_params, int num_
                          unsigned long long seed, unsigned long long call_index, int *N_out, int n_edges,
                          const double *f_edges, const int *N_levels, void *stream);
 
+/* ---- derived waveforms ------------------------------------------------ */
+
+/* The reference's extension API (src/gbgpu/gbgpu.py:1140-1229) lets a
+ * subclass change the slow part through two hooks on (num_bin, 3, N) arrays
+ * (called at src/gbgpu/gbgpu.py:453 and :505):
+ *   fi = shift_frequency(fi, xi, *args), argS = add_to_argS(argS, f0, fdot, fddot, xi, *args).
+ * gbgpu_hook_inputs writes their inputs for a batch: xi[b, s, n] =
+ * t_n - k.P_s(t_n)/c, the unshifted fi = f0 + fdot xi + fddot xi^2/2 and the
+ * base argS = phi0 + (om - 2 pi rint(f0 T)/T) t_n + pi fdot xi^2
+ * + (pi/3) fddot xi^3, each (num_bin, 3, N) float64.  The caller runs the
+ * hooks and passes fi_argS (num_bin, 2, 3, N) float64 -- per binary the
+ * hooked fi, then argS -- to gbgpu_run_wave_ext / gbgpu_wave_ll_ext, which
+ * are gbgpu_run_wave / gbgpu_wave_ll with those values in place of the
+ * computed ones.  9 parameter rows, GBGPU_FP64, GBGPU_TRANSFORM_LDS,
+ * N <= 2048. */
+int gbgpu_hook_inputs(double *xi, double *fi, double *argS, const double *params, int num_params, int num_bin,
+                      const gbgpu_wave_cfg *cfg, void *stream);
+int gbgpu_run_wave_ext(gbgpu_cmplx *XYZf, gbgpu_cmplx *AETf, long long *start_inds, const double *params,
+                       int num_params, int num_bin, const gbgpu_wave_cfg *cfg, const double *fi_argS,
+                       void *stream);
+int gbgpu_wave_ll_ext(double *loglike, gbgpu_cmplx *d_h, gbgpu_cmplx *h_h, const double *params, int num_params,
+                      int num_bin, const gbgpu_wave_cfg *cfg, const gbgpu_data *data, double d_d,
+                      int phase_marginalize, const double *fi_argS, void *stream);
+
 /* ---- library -------------------------------------------------------- */
 
 /* Message describing the last failure on the calling thread. */
 const char *gbgpu_last_error(void);
 
-/* ABI version of this header (6). */
+/* ABI version of this header (7). */
 int gbgpu_abi_version(void);
 
 /* Measurement hook with no counterpart in the reference: while enabled,

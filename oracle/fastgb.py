@@ -180,10 +180,15 @@ def adjust_T(T, dt):
     return N_obs * dt
 
 
-def run_wave(amp, f0, fdot, fddot, phi0, iota, psi, lam, beta, *third, N, T, dt, orbits=None, tdi2=False):
+def run_wave(amp, f0, fdot, fddot, phi0, iota, psi, lam, beta, *third, N, T, dt, orbits=None, tdi2=False,
+             hooks=None):
     """Restates GBGPUBase.run_wave (src/gbgpu/gbgpu.py:108-316).
 
     ``third`` is empty or (A2, varpi, e2, P2, T2) as GBGPUThirdBody takes them.
+    ``hooks``: a derived waveform's (shift_frequency, add_to_argS), numpy
+    callables (fi, xi) -> fi and (argS, f0, fdot, fddot, xi) -> argS on
+    (B, 3, N) arrays (their extra arguments bound), applied where the
+    reference's _construct_slow_part calls them (src/gbgpu/gbgpu.py:453, :505).
     Returns dict with XYZf, AETf (B, 3, N) complex128, start_inds (B,) int64, df, T.
     """
     if orbits is None:
@@ -213,7 +218,8 @@ def run_wave(amp, f0, fdot, fddot, phi0, iota, psi, lam, beta, *third, N, T, dt,
     tm = np.linspace(0, T, num=N, endpoint=False)
     P = [np.asarray(orbits.get_pos(tm, np.full(N, s, dtype=np.int32)), dtype=np.float64) for s in (1, 2, 3)]
 
-    Gs, q = _construct_slow_part(T, L, P, tm, f0, fdot, fddot, fstar, phi0, k, DP, DC, eplus, ecross, *add_args)
+    Gs, q = _construct_slow_part(T, L, P, tm, f0, fdot, fddot, fstar, phi0, k, DP, DC, eplus, ecross, *add_args,
+                                 hooks=hooks)
     XYZf, f_min = _computeXYZ(T, Gs, f0, fdot, fddot, fstar, amp, q, tm)
     start_inds = np.round(f_min / df).astype(np.int64)
     fctr = 0.5 * T / N
@@ -227,7 +233,7 @@ def run_wave(amp, f0, fdot, fddot, phi0, iota, psi, lam, beta, *third, N, T, dt,
     return dict(XYZf=XYZf, AETf=AETf, start_inds=start_inds, df=df, T=T, N=N)
 
 
-def _construct_slow_part(T, L, P, tm, f0, fdot, fddot, fstar, phi0, k, DP, DC, eplus, ecross, *add_args):
+def _construct_slow_part(T, L, P, tm, f0, fdot, fddot, fstar, phi0, k, DP, DC, eplus, ecross, *add_args, hooks=None):
     """src/gbgpu/gbgpu.py:399-540."""
     P1, P2, P3 = P
     r = {}
@@ -244,6 +250,8 @@ def _construct_slow_part(T, L, P, tm, f0, fdot, fddot, fstar, phi0, k, DP, DC, e
     fi = f0[:, None, None] + fdot[:, None, None] * xi + 1 / 2.0 * fddot[:, None, None] * xi**2
     if add_args:
         fi = fi * (1 + get_vLOS(xi, *add_args) / C_SI)  # thirdbody.py:179
+    if hooks is not None:
+        fi = hooks[0](fi, xi)  # gbgpu.py:453
     fonfs = fi / fstar
     q = np.rint(f0 * T)
     dfq = 2.0 * np.pi * (q / T)
@@ -266,6 +274,8 @@ def _construct_slow_part(T, L, P, tm, f0, fdot, fddot, fstar, phi0, k, DP, DC, e
         term = np.zeros_like(xi)
         term[:, :, 1:] = np.cumsum(parab_step_ET(f0, fdot, fddot, A2, varpi, e2, n2, T2, xi[:, :, 1:], xi[:, :, :-1]), axis=-1)
         argS = argS + term
+    if hooks is not None:
+        argS = hooks[1](argS, f0, fdot, fddot, xi)  # gbgpu.py:505
     kdotP = om[:, None, None] * kdotP - argS
     Gs = {}
     for ij, ij_sym, s in [("12", "12", 0), ("23", "23", 1), ("31", "31", 2), ("21", "12", 1), ("32", "23", 2), ("13", "31", 0)]:

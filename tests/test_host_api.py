@@ -132,3 +132,35 @@ def test_persistent_strategy_refused_by_the_library():
                      df=1e-7)
     assert lib.gbgpu_wave_ll_batched(P, None, None, P, 9, 4, P, bcfg, dat, 0.0, 0, None, None) == -1
     assert b"persistent" in lib.gbgpu_last_error()
+
+
+def test_derived_waveform_hooks_detected_and_unsupported_paths_refused():
+    """A subclass defined outside the package with its own slow-part hooks
+    takes the derived-waveform path (gbgpu_hook_inputs -> hooks -> *_ext);
+    the package's own classes do not.  Paths the kernels cannot hook refuse
+    before any device work."""
+    import numpy as np
+    import pytest
+
+    from gbgpu_amd import GBGPU, GBGPUBase, GBGPUThirdBody
+
+    class Shifted(GBGPU):
+        def shift_frequency(self, fi, xi, *args):
+            return fi * 1.0001
+
+    class OnlyN(GBGPU):
+        def special_get_N(self, amp, f0, T, *args, oversample=1):
+            return super().special_get_N(amp, f0, T, *args, oversample=oversample)
+
+    assert Shifted()._user_hooks
+    assert not OnlyN()._user_hooks
+    assert not GBGPU()._user_hooks and not GBGPUThirdBody()._user_hooks
+    assert issubclass(Shifted, GBGPUBase)
+    gb = Shifted()
+    p = np.ones((9, 2))
+    A = np.zeros(100, dtype=np.complex128)
+    with pytest.raises(NotImplementedError, match="swap_likelihood_difference"):
+        gb.swap_likelihood_difference(p, p, [A, A], [np.ones(100)] * 2, N=64)
+    gb.d_d = 0.0
+    with pytest.raises(NotImplementedError, match="one N per binary"):
+        gb.get_ll(p, [A, A], [np.ones(100)] * 2, N=np.array([64, 64]))

@@ -112,7 +112,7 @@ def test_transform_of_a_pure_tone(monkeypatch, m0):
     q = 250_000
     f0 = (q + 0.25) / T  # rint(f0 T) = q
 
-    def tone_slow_part(T_, L_, P_, tm, f0_, fdot_, fddot_, fstar, *rest):
+    def tone_slow_part(T_, L_, P_, tm, f0_, fdot_, fddot_, fstar, *rest, hooks=None):
         z = np.zeros((1, N), np.complex128)
         Gs = {ij: z for ij in ("12", "23", "31", "13", "32")}
         Gs["21"] = np.exp(2j * np.pi * m0 * tm / T_)[None, :]
