@@ -656,6 +656,19 @@ def measure_counters(num_bin):
     return out
 
 
+def fma_probe(precision):
+    """Attainable FMA rate (TFLOP/s): the library's 8-chain FMA kernel, FP64
+    (precision 0) or FP32 without packed forms (1)."""
+    import ctypes
+
+    from gbgpu_amd import _capi
+
+    out = ctypes.c_double(0.0)
+    _capi.check(_capi.load().gbgpu_fma_probe(int(precision), ctypes.byref(out), _capi.stream_handle()),
+                "gbgpu_fma_probe")
+    return out.value
+
+
 def copy_probe():
     """Attainable HBM bandwidth: a 1 GiB device-to-device elementwise copy
     (1 GiB read + 1 GiB written), timed with HIP events."""
@@ -794,6 +807,8 @@ def main():
 
     print("[bench] copy probe", file=sys.stderr, flush=True)
     copy_gbs = copy_probe()
+    fma64 = fma_probe(0)
+    fma32 = fma_probe(1)
 
     dropin = None
     if not args.no_dropin and args.only is None:
@@ -843,11 +858,15 @@ def main():
         "flop_per_binary": flop_per_binary(N),
         "algorithmic_hbm_GBs": bytes_per_binary(N) * B / kernel_s / 1e9,
     }
+    roof["attainable_fma_TFLOPs"] = fma64
+    roof["attainable_note"] = ("FP64 FMA rate of an 8-independent-chain kernel on this GPU (gbgpu_fma_probe); "
+                               "FP32 without packed forms: %.1f TFLOP/s" % fma32)
     if counters:
         ex = counters["fp64"]["executed_fp64_flop_per_binary"]
         roof["executed_flop_per_binary"] = ex
         roof["executed_TFLOPs"] = ex * B / kernel_s / 1e12
         roof["executed_frac"] = roof["executed_TFLOPs"] / FP64_PEAK_TFLOPS
+        roof["executed_frac_of_attainable"] = roof["executed_TFLOPs"] / fma64
     if mixed is not None and counters:
         km = mixed["kernel_ms"] / 1e3
         c = counters["mixed"]
@@ -859,6 +878,7 @@ def main():
             "executed_fp64_flop_per_binary": c["executed_fp64_flop_per_binary"],
             "executed_fp32_TFLOPs": c["executed_fp32_flop_per_binary"] * B / km / 1e12,
             "executed_fp64_TFLOPs": c["executed_fp64_flop_per_binary"] * B / km / 1e12,
+            "attainable_fp32_fma_TFLOPs": fma32,
             "traffic": c["traffic_bytes"],
         }
 

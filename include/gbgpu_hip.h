@@ -346,6 +346,12 @@ int gbgpu_kernel_timing(int enable);
  * count, and drops them. */
 int gbgpu_kernel_timing_read(double *total_ms, int *count);
 
+/* Measurement hook with no counterpart in the reference: the FMA rate this
+ * GPU sustains (TFLOP/s, FMA = 2 FLOP) in a kernel of 8 independent FMA
+ * chains per thread, FP64 (GBGPU_FP64) or FP32 without packed forms
+ * (GBGPU_MIXED).  Synchronises `stream` (about 30 ms); for rooflines. */
+int gbgpu_fma_probe(int precision, double *tflops, void *stream);
+
 #ifdef __cplusplus
 }
 #endif
