@@ -59,6 +59,20 @@ __device__ __forceinline__ void sincos_f64(double x, double &s, double &c) {
     sincos_poly_f64(y, (int)k & 3, s, c);
 }
 
+// sin and cos of x + d for a large x and a small d (|d| < 2): the quadrant is
+// taken from x + d, the reduction is applied to x alone, and d is added to
+// the reduced value, so |y| <= pi/4 (up to rounding) and the rounding of the
+// large sum x + d never reaches the result.
+__device__ __forceinline__ void sincos_f64_shift(double x, double d, double &s, double &c) {
+    constexpr double TWO_OVER_PI = 0.63661977236758134308;
+    constexpr double P1 = 1.5707963267948966;
+    constexpr double P2 = 6.123233995736766e-17;
+    const double k = rint((x + d) * TWO_OVER_PI);
+    double y = fma(-k, P1, x);
+    y = fma(-k, P2, y) + d;
+    sincos_poly_f64(y, (int)k & 3, s, c);
+}
+
 // 1/x to ~1 ulp: hardware estimate refined by two Newton steps.
 __device__ __forceinline__ double rcp_f64(double x) {
     double r = __builtin_amdgcn_rcp(x);

@@ -291,7 +291,9 @@ __device__ __forceinline__ double2 link_f64(double fonfs, double kr, double cphi
 // Time-domain X, Y, Z slow part at sample n, before their common factor
 // F = fctr2 * amp * 0.5 T / N (gbgpu.py:359-375, :298; amp / amp cancels),
 // which is returned separately so that the caller applies it once per
-// output channel (A and E for the likelihood).
+// output channel (A and E for the likelihood).  On the guiding-centre path F
+// also carries the phase factor e^{-i Phi_c} shared by all six links, which
+// X, Y and Z then leave out: only the products F X, F Y, F Z are defined.
 // v3[sc] (third-body v_LOS at xi_sc) and term3[sc] (its phase integral) are
 // used only when THIRD.  EXT (derived waveforms): v3[sc] and term3[sc] are
 // the frequency f_i and the phase argS of spacecraft sc at this sample as
@@ -301,33 +303,43 @@ __device__ __forceinline__ void slow_xyz_f64(const Src &s, const SampleGeom &g, 
                                              const double *v3, const double *term3, double2 &X, double2 &Y,
                                              double2 &Z, double2 &F) {
     static_assert(!(THIRD && EXT), "third-body and external hooks are exclusive");
+    constexpr bool SPLIT = !THIRD && !EXT && TIER < NTIER;
+    // TDI transfer (gbgpu.py:352-375): omL = f(t)/f*
+    const double f = s.f0 + s.fdot * g.t + 0.5 * s.fddot * g.t * g.t;
+    const double omL = f * fstar_inv;
+    double sL, cL;
+    if constexpr (TIER < NTIER)
+        sincos_series<TierF64<TIER>::KS>(omL, sL, cL);
+    else
+        sincos_f64(omL, sL, cL);
+    const double m = omL * sL * scale;  // 4 omL sL scale, times the links' 1/4
     double fonfs[3], cphi[3], sphi[3];
-    if constexpr (!THIRD && !EXT && TIER < NTIER) {
+    if constexpr (SPLIT) {
         // Guiding-centre split: with xi_c = t - k.Pc/c and d_s = k.dP_s/c,
         //   Phi_s = Phi_c + 2 pi f(xi_c) d_s - pi f'(xi_c) d_s^2 + (pi/3) fddot d_s^3,
-        // exact for the cubic argS.  Phi_c takes the one range-reduced
-        // sine/cosine; the offsets (|.| <= 0.29 U_MAX of the tier) take short
-        // series and a rotation.
+        // exact for the cubic argS.  The offsets (|.| <= 0.29 U_MAX of the
+        // tier) take short series; the common e^{-i Phi_c} of every link is
+        // factored out of X, Y and Z into F, where it joins the TDI factor's
+        // e^{-i omL} in the one range-reduced sine/cosine of Phi_c + omL.
         const double xic = g.t - g.kPc;
         const double fc = s.f0 + s.fdot * xic + 0.5 * s.fddot * xic * xic;
         const double fdc = s.fdot + s.fddot * xic;
         const double argSc =
             s.phi0 + s.omq * g.t + PI * s.fdot * (xic * xic) + (1.0 / 3.0) * PI * s.fddot * (xic * xic * xic);
         double sC, cC;
-        sincos_f64(s.om * g.kPc - argSc, sC, cC);
+        sincos_f64_shift(s.om * g.kPc - argSc, omL, sC, cC);
+        F = make_double2(m * cC, -m * sC);  // fctr2 e^{-i Phi_c}
         const double w1 = 2.0 * PI * fc, w2 = PI * fdc, w3 = (PI / 3.0) * s.fddot;
 #pragma unroll
         for (int sc = 0; sc < 3; sc++) {
             const double dd = g.d[sc];
             const double dphi = dd * (w1 - dd * (w2 - dd * w3));
-            double sd, cd;
-            sincos_series<TierF64<TIER>::KD>(dphi, sd, cd);
-            sphi[sc] = sC * cd + cC * sd;
-            cphi[sc] = cC * cd - sC * sd;
+            sincos_series<TierF64<TIER>::KD>(dphi, sphi[sc], cphi[sc]);
             // f(xi_c - d_s), exact for the quadratic f
             fonfs[sc] = (fc - dd * (fdc - 0.5 * s.fddot * dd)) * fstar_inv;
         }
     } else {
+        F = make_double2(m * cL, -m * sL);  // fctr2
 #pragma unroll
         for (int sc = 0; sc < 3; sc++) {
             const double xi = g.t - g.kP[sc];
@@ -361,17 +373,7 @@ __device__ __forceinline__ void slow_xyz_f64(const Src &s, const SampleGeom &g, 
     const double2 G21 = link_f64<TIER>(fonfs[1], -g.kr12, cphi[1], sphi[1], A12);
     const double2 G32 = link_f64<TIER>(fonfs[2], -g.kr23, cphi[2], sphi[2], A23);
     const double2 G13 = link_f64<TIER>(fonfs[0], g.kr13, cphi[0], sphi[0], A31);
-    // TDI transfer (gbgpu.py:352-375)
-    const double f = s.f0 + s.fdot * g.t + 0.5 * s.fddot * g.t * g.t;
-    const double omL = f * fstar_inv;
-    double sL, cL;
-    if constexpr (TIER < NTIER)
-        sincos_series<TierF64<TIER>::KS>(omL, sL, cL);
-    else
-        sincos_f64(omL, sL, cL);
     const double2 fctr = make_double2(cL, -sL);
-    const double m = omL * sL * scale;  // 4 omL sL scale, times the links' 1/4
-    F = make_double2(m * cL, -m * sL);  // fctr2
     X = cadd(csub(G21, G31), cmul(csub(G12, G13), fctr));
     Y = cadd(csub(G32, G12), cmul(csub(G23, G21), fctr));
     Z = cadd(csub(G13, G23), cmul(csub(G31, G32), fctr));

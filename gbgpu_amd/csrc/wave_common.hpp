@@ -275,8 +275,13 @@ struct ThirdScratch {
 // register layout gives it, xr[c][i] holding sample dft256_index(lane, i) of
 // channel c, and dft256_mfma transforms them in place (same index on exit);
 // buf and tw are not touched.
+//
+// REGS (LDS transform, circular binaries, all channels in one pass): the
+// samples stay in registers, xr[c * SPT + q] = sample lane + q TPB, and
+// fft_lds_regs returns the spectrum there, xr[c * SPT + p] = bin lane + p TPB;
+// buf is the transform's working storage only.
 template <int N, int NCH, bool THIRD, class P, int M = 1, int DFT = GBGPU_TRANSFORM_LDS, int C0 = 0, int NCP = NCH,
-          bool EXT = false>
+          bool EXT = false, bool REGS = false>
 __device__ __forceinline__ void binary_spectrum(const Src &s, const typename P::Aux &aux, bool valid, int lane,
                                                 const double *__restrict__ tab, double fstar_inv, double scale,
                                                 typename P::CT *buf, const typename P::CT *__restrict__ tw,
@@ -292,6 +297,8 @@ __device__ __forceinline__ void binary_spectrum(const Src &s, const typename P::
                   "channel pass out of range");
     static_assert(!EXT || (!THIRD && DFT == GBGPU_TRANSFORM_LDS && std::is_same<P, F64>::value),
                   "derived-waveform hooks run on the FP64 circular path with the LDS transform");
+    static_assert(!REGS || (!THIRD && DFT == GBGPU_TRANSFORM_LDS && C0 == 0 && NCP == NCH),
+                  "register-ended transform: circular binaries, LDS transform, one pass");
     constexpr int NT = THIRD ? SPT : 1;
     float vf[3][NT];   // v_LOS at xi_sc(t_n)
     double gc[3][NT];  // the cumulative phase term at t_n
@@ -418,7 +425,7 @@ __device__ __forceinline__ void binary_spectrum(const Src &s, const typename P::
             }
 #pragma unroll
             for (int c = 0; c < NCH; c++) {
-                if constexpr (DFT != GBGPU_TRANSFORM_LDS)
+                if constexpr (DFT != GBGPU_TRANSFORM_LDS || REGS)
                     xr[c * SPT + q] = out[c];
                 else if (c >= C0 && c < C0 + NCP)
                     buf[(c - C0) * N + n] = out[c];
@@ -434,6 +441,8 @@ __device__ __forceinline__ void binary_spectrum(const Src &s, const typename P::
         const Dft256Lane<DFT> k = load_dft256<DFT>(dftc, lane);
 #pragma unroll
         for (int c = 0; c < NCH; c++) dft256_mfma<DFT>(xr + c * SPT, k);
+    } else if constexpr (REGS) {
+        fft_lds_regs<N, NCH, TPB>(xr, buf, lane, tw);
     } else {
         __syncthreads();
         fft_lds<N, NCP, TPB>(buf, lane, tw);
@@ -492,20 +501,18 @@ __device__ __forceinline__ void ll_channel(const typename P::CT *buf, int lane, 
     }
 }
 
-// ll_partial for spectra held in registers by the matrix-core transform:
-// h[c * 4 + i] is bin dft256_index(lane, i) of channel c (A, E).
-template <class P>
-__device__ __forceinline__ void ll_partial_regs(const typename P::CT *h, double2 fac, int lane, long long j0,
-                                                long long doff, long long noff, const WaveArgs &a, double &dhr,
-                                                double &dhi, double &hh) {
-    constexpr int N = 256;
+// ll_partial for spectra held in registers: spec(c, i) is channel c's (A,
+// E) value for fftshifted output position pos(i).
+template <class P, int SPT, class Pos, class Spec>
+__device__ __forceinline__ void ll_partial_regs(Spec spec, Pos pos, double2 fac, long long j0, long long doff,
+                                                long long noff, const WaveArgs &a, double &dhr, double &dhi,
+                                                double &hh) {
     double xr = 0.0, xi = 0.0, xx = 0.0;
 #pragma unroll
-    for (int i = 0; i < 4; i++) {
-        const int m = (dft256_index(lane, i) + N / 2) & (N - 1);  // fftshifted position
-        const long long j = j0 + m;
+    for (int i = 0; i < SPT; i++) {
+        const long long j = j0 + pos(i);
         if (j < 0 || j >= a.L) continue;
-        const double2 hA = P::up(h[i]), hE = P::up(h[4 + i]);
+        const double2 hA = P::up(spec(0, i)), hE = P::up(spec(1, i));
         const double2 dA = a.dA[doff + j], dE = a.dE[doff + j];
         const double iA = rcp_f64(a.SA[noff + j]), iE = rcp_f64(a.SE[noff + j]);
         xr += (dA.x * hA.x + dA.y * hA.y) * iA + (dE.x * hE.x + dE.y * hE.y) * iE;

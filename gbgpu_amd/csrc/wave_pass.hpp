@@ -75,18 +75,30 @@ __device__ __forceinline__ void wave_pass(const WaveArgs &a, WaveLds<N, MODE, TH
     double *tot = stot + slot * WPB;
     ThirdScratch ts{L.s_edge + slot * W::NEDGE};
 
-    constexpr bool REG = DFT != GBGPU_TRANSFORM_LDS;
-    CT xr[REG ? NCH * SPT : 1];  // register-resident spectra (matrix-core transform)
-    binary_spectrum<N, NCH, THIRD, P, M, DFT, 0, NCH, EXT>(s, per_group<TPB>(aux_of<P, PK>(*pk)), valid, lane, tab,
-                                                           a.fstar_inv, a.scale, buf, static_cast<const CT *>(twp),
-                                                           ts, tot, xr, a.dftc,
-                                                           (EXT && valid) ? a.ext + b * 6 * N : nullptr);
-    // bin of output position q of this thread: fftshifted position m(q),
-    // spectrum value spec(c, q) of channel c
-    auto pos = [&](int q) { return REG ? (dft256_index(lane, q) + N / 2) & (N - 1) : lane + q * TPB; };
+    // spectra in registers: the matrix-core transform's layout (MFMA), or
+    // the LDS transform with register ends (circular binaries)
+    constexpr bool MFMA = DFT != GBGPU_TRANSFORM_LDS;
+    constexpr bool REG = MFMA || !THIRD;
+    CT xr[REG ? NCH * SPT : 1];
+    binary_spectrum<N, NCH, THIRD, P, M, DFT, 0, NCH, EXT, REG && !MFMA>(
+        s, per_group<TPB>(aux_of<P, PK>(*pk)), valid, lane, tab, a.fstar_inv, a.scale, buf,
+        static_cast<const CT *>(twp), ts, tot, xr, a.dftc, (EXT && valid) ? a.ext + b * 6 * N : nullptr);
+    // output q of this thread: fftshifted position pos(q) (natural bin k sits
+    // at position (k + N/2) mod N) and its value spec(c, q) in channel c.
+    // Positions lane + q TPB keep the data reads at constant offsets; with
+    // the register-ended LDS transform their bins (lane + q TPB + N/2) mod N
+    // are registers (q + SPT/2) mod SPT.
+    auto pos = [&](int q) {
+        if constexpr (MFMA)
+            return (dft256_index(lane, q) + N / 2) & (N - 1);
+        else
+            return lane + q * TPB;
+    };
     auto spec = [&](int c, int q) {
-        if constexpr (REG)
+        if constexpr (MFMA)
             return xr[c * SPT + q];
+        else if constexpr (REG)
+            return xr[c * SPT + ((q + SPT / 2) & (SPT - 1))];
         else
             return buf[c * N + ((lane + q * TPB + N / 2) & (N - 1))];  // fftshift
     };
@@ -138,7 +150,7 @@ __device__ __forceinline__ void wave_pass(const WaveArgs &a, WaveLds<N, MODE, TH
             const long long doff = (a.di ? (long long)a.di[b] : 0LL) * a.L;
             const long long noff = (a.ni ? (long long)a.ni[b] : 0LL) * a.L;
             if constexpr (REG)
-                ll_partial_regs<P>(xr, fac, lane, s.start - a.start_freq_ind, doff, noff, a, dhr, dhi, hh);
+                ll_partial_regs<P, SPT>(spec, pos, fac, s.start - a.start_freq_ind, doff, noff, a, dhr, dhi, hh);
             else
                 ll_partial<N, P, M>(buf, fac, lane, s.start - a.start_freq_ind, doff, noff, a, dhr, dhi, hh);
         }
