@@ -4,11 +4,12 @@
 // Work decomposition: a group of TPB = min(N/4, 256) threads owns one binary
 // and a 256-thread workgroup holds BPW = 256/TPB binaries (N = 2048: one
 // 512-thread workgroup per binary).  Each thread evaluates the slow part at
-// SPT = N/TPB samples straight into LDS, the group runs the length-N
-// transform in LDS, and the epilogue either streams the fftshifted, scaled
-// spectra to HBM (run_wave) or reads the binary's data/PSD window with
-// coalesced loads and reduces <d|h>, <h|h> over the group (wave_ll); one
-// workgroup pass is wave_pass (wave_pass.hpp).  N = 4096 and 8192 run the
+// SPT = N/TPB samples, the group runs the length-N transform in LDS (for
+// circular binaries with its first and last stages in registers), and the
+// epilogue either streams the fftshifted, scaled spectra to HBM (run_wave)
+// or reads the binary's data/PSD window with coalesced loads and reduces
+// <d|h>, <h|h> over the group (wave_ll); one workgroup pass is wave_pass
+// (wave_pass.hpp).  N = 4096 and 8192 run the
 // large-N kernels of wave_big.hip; the batch segment kernels are in
 // wave_seg.hip.
 #include <hipcub/hipcub.hpp>

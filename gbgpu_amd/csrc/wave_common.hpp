@@ -449,6 +449,19 @@ __device__ __forceinline__ void binary_spectrum(const Src &s, const typename P::
     }
 }
 
+// One bin of <d|x> and <x|x> for the A and E channels: xr + i xi +=
+// conj(d) x / S, xx += |x|^2 / S, with explicit fused multiply-adds so that
+// every caller rounds alike.
+__device__ __forceinline__ void ll_bin(double2 hA, double2 hE, double2 dA, double2 dE, double iA, double iE,
+                                       double &xr, double &xi, double &xx) {
+    const double rA = fma(dA.x, hA.x, dA.y * hA.y), rE = fma(dE.x, hE.x, dE.y * hE.y);
+    const double jA = fma(dA.x, hA.y, -(dA.y * hA.x)), jE = fma(dE.x, hE.y, -(dE.y * hE.x));
+    const double nA = fma(hA.x, hA.x, hA.y * hA.y), nE = fma(hE.x, hE.x, hE.y * hE.y);
+    xr = fma(rE, iE, fma(rA, iA, xr));
+    xi = fma(jE, iE, fma(jA, iA, xi));
+    xx = fma(nE, iE, fma(nA, iA, xx));
+}
+
 // Accumulates this thread's share of <d|h> (complex) and <h|h> for a binary
 // whose A/E spectra h = fac x have x in buf (natural order), over output bins
 // m = lane + q TPB (fftshifted), data bin j = j0 + m.  The sums are formed
@@ -465,13 +478,8 @@ __device__ __forceinline__ void ll_partial(const typename P::CT *buf, double2 fa
         const int idx = (m + N / 2) & (N - 1);
         const long long j = j0 + m;
         if (j < 0 || j >= a.L) continue;
-        const double2 hA = P::up(buf[idx]), hE = P::up(buf[N + idx]);
-        const double2 dA = a.dA[doff + j], dE = a.dE[doff + j];
-        const double iA = rcp_f64(a.SA[noff + j]), iE = rcp_f64(a.SE[noff + j]);
-        // conj(d) x / S
-        xr += (dA.x * hA.x + dA.y * hA.y) * iA + (dE.x * hE.x + dE.y * hE.y) * iE;
-        xi += (dA.x * hA.y - dA.y * hA.x) * iA + (dE.x * hE.y - dE.y * hE.x) * iE;
-        xx += (hA.x * hA.x + hA.y * hA.y) * iA + (hE.x * hE.x + hE.y * hE.y) * iE;
+        ll_bin(P::up(buf[idx]), P::up(buf[N + idx]), a.dA[doff + j], a.dE[doff + j], rcp_f64(a.SA[noff + j]),
+               rcp_f64(a.SE[noff + j]), xr, xi, xx);
     }
     dhr += fac.x * xr - fac.y * xi;
     dhi += fac.x * xi + fac.y * xr;
@@ -495,14 +503,14 @@ __device__ __forceinline__ void ll_channel(const typename P::CT *buf, int lane, 
         const double2 h = P::up(buf[(m + N / 2) & (N - 1)]);
         const double2 dv = d[doff + j];
         const double iS = rcp_f64(S[noff + j]);
-        xr += (dv.x * h.x + dv.y * h.y) * iS;
-        xi += (dv.x * h.y - dv.y * h.x) * iS;
-        xx += (h.x * h.x + h.y * h.y) * iS;
+        xr = fma(fma(dv.x, h.x, dv.y * h.y), iS, xr);
+        xi = fma(fma(dv.x, h.y, -(dv.y * h.x)), iS, xi);
+        xx = fma(fma(h.x, h.x, h.y * h.y), iS, xx);
     }
 }
 
 // ll_partial for spectra held in registers: spec(c, i) is channel c's (A,
-// E) value for fftshifted output position pos(i).
+// E) value for fftshifted output position pos(i), i < SPT.
 template <class P, int SPT, class Pos, class Spec>
 __device__ __forceinline__ void ll_partial_regs(Spec spec, Pos pos, double2 fac, long long j0, long long doff,
                                                 long long noff, const WaveArgs &a, double &dhr, double &dhi,
@@ -512,12 +520,8 @@ __device__ __forceinline__ void ll_partial_regs(Spec spec, Pos pos, double2 fac,
     for (int i = 0; i < SPT; i++) {
         const long long j = j0 + pos(i);
         if (j < 0 || j >= a.L) continue;
-        const double2 hA = P::up(spec(0, i)), hE = P::up(spec(1, i));
-        const double2 dA = a.dA[doff + j], dE = a.dE[doff + j];
-        const double iA = rcp_f64(a.SA[noff + j]), iE = rcp_f64(a.SE[noff + j]);
-        xr += (dA.x * hA.x + dA.y * hA.y) * iA + (dE.x * hE.x + dE.y * hE.y) * iE;
-        xi += (dA.x * hA.y - dA.y * hA.x) * iA + (dE.x * hE.y - dE.y * hE.x) * iE;
-        xx += (hA.x * hA.x + hA.y * hA.y) * iA + (hE.x * hE.x + hE.y * hE.y) * iE;
+        ll_bin(P::up(spec(0, i)), P::up(spec(1, i)), a.dA[doff + j], a.dE[doff + j], rcp_f64(a.SA[noff + j]),
+               rcp_f64(a.SE[noff + j]), xr, xi, xx);
     }
     dhr += fac.x * xr - fac.y * xi;
     dhi += fac.x * xi + fac.y * xr;
