@@ -43,7 +43,7 @@ __device__ __forceinline__ double2 dfac_df0(const Src &s, double armlength, int 
     if (!tdi2) return make_double2(0.0, 0.0);
     const double dxdf = 4.0 * PI * armlength / C_SI;
     double s2, c2;
-    sincos(2.0 * (4.0 * PI * s.f0 * armlength / C_SI), &s2, &c2);
+    sincos_f64(2.0 * (4.0 * PI * s.f0 * armlength / C_SI), s2, c2);
     // 2i (c2 - i s2) = 2 s2 + 2i c2
     return make_double2(2.0 * s2 * dxdf, 2.0 * c2 * dxdf);
 }
@@ -51,9 +51,9 @@ __device__ __forceinline__ double2 dfac_df0(const Src &s, double armlength, int 
 // Derivatives of DP, DC with respect to cos iota (make_src):
 //   dDP/dc = 2 amp (c cos2psi + i sin2psi),  dDC/dc = 2 amp (-c sin2psi + i cos2psi)
 __device__ __forceinline__ void dpol_dcos(double amp, double iota, double psi, double2 &DPc, double2 &DCc) {
-    const double c = cos(iota);
-    double s2, c2;
-    sincos(2.0 * psi, &s2, &c2);
+    double si, c, s2, c2;
+    sincos_f64(iota, si, c);
+    sincos_f64(2.0 * psi, s2, c2);
     DPc = make_double2(2.0 * amp * c * c2, 2.0 * amp * s2);
     DCc = make_double2(-2.0 * amp * c * s2, 2.0 * amp * c2);
 }

@@ -44,8 +44,8 @@ __device__ inline SkyDeriv make_sky_deriv(double lam, double beta) {
     SkyDeriv d;
     const double theta = PI / 2 - beta;
     double sinth, costh, sinph, cosph;
-    sincos(theta, &sinth, &costh);
-    sincos(lam, &sinph, &cosph);
+    sincos_f64(theta, sinth, costh);
+    sincos_f64(lam, sinph, cosph);
     const double tanb = costh / sinth;  // sin beta / cos beta
     d.ul[0] = -costh * sinph;
     d.ul[1] = costh * cosph;

@@ -133,7 +133,7 @@ __device__ inline double kepler_u(double l, double e) {
     const double w = s - 0.078 * s * s * s * s * s / (1.0 + e);
     const double u0 = l + e * (3.0 * w - 4.0 * w * w * w);
     double su0, cu0;
-    sincos(u0, &su0, &cu0);
+    sincos_f64(u0, su0, cu0);
     const double f = u0 - e * su0 - l;
     const double f1 = 1.0 - e * cu0;
     double f2 = u0 - f - l;
@@ -166,10 +166,13 @@ __device__ inline double third_vlos(const Src &s, double t) {
     if (s.e2 > 1e-6) {
         const double bb = (1.0 - sqrt(1.0 - s.e2 * s.e2)) / s.e2;
         double su, cu;
-        sincos(u, &su, &cu);
+        sincos_f64(u, su, cu);
         u += 2.0 * atan2(bb * su, 1.0 - bb * cu);
     }
-    return s.A2 * (sin(u + s.varpi) + s.e2 * sin(s.varpi));
+    double sn, cs, sw, cw;
+    sincos_f64(u + s.varpi, sn, cs);
+    sincos_f64(s.varpi, sw, cw);
+    return s.A2 * (sn + s.e2 * sw);
 }
 
 // ------------------------------------------------------ FP64 slow part ---
