@@ -82,7 +82,9 @@ __device__ __forceinline__ float2 transfer_general(float u) {
 //   Phi_s - Phi_c = 2 pi f(xi_c) d_s - pi (fdot + fddot xi_c) d_s^2
 //                   + (pi/3) fddot d_s^3          (exact for the cubic argS)
 // are FP32.  The third-body variant keeps one FP64 phase per spacecraft
-// because its phase integral differs per spacecraft.
+// because its phase integral differs per spacecraft.  As in slow_xyz_f64,
+// the circular path returns X, Y, Z without the links' common e^{-i Phi_c},
+// which F carries: only F X, F Y, F Z are defined.
 template <bool THIRD, int TIER>
 __device__ __forceinline__ void slow_xyz_mixed(const Src &s, const SrcF &sf, const double *__restrict__ tab, int N,
                                                int n, double fstar_inv_d, float scale, const float *v3,
@@ -110,22 +112,20 @@ __device__ __forceinline__ void slow_xyz_mixed(const Src &s, const SrcF &sf, con
     const float fc = sf.f0 + sf.fdot * xcf + 0.5f * sf.fddot * xcf * xcf;
     float fonfs[3], cphi[3], sphi[3];
     if constexpr (!THIRD) {
-        const double argSc = s.phi0 + s.omq * t + PI * s.fdot * (xic * xic) + (1.0 / 3.0) * PI * s.fddot * (xic * xic * xic);
-        const double Phic = s.om * kPc - argSc;
-        float sC, cC;
-        sincos_rev(Phic * INV_TWO_PI, sC, cC);
+        // The links' common e^{-i Phi_c} is left out of X, Y, Z and carried by
+        // F, joined with the TDI factor's e^{-i omL} in one FP64-reduced
+        // sine/cosine of Phi_c + omL (formed after the links, so that no
+        // FP64 phase stays live through them); the links keep
+        // e^{-i (Phi_s - Phi_c)}.
         const float w1 = 2.0f * PIF * fc, w2 = PIF * fdc, w3 = (PIF / 3.0f) * sf.fddot;
 #pragma unroll
         for (int sc = 0; sc < 3; sc++) {
             const float dd = d[sc];
             const float dphi = dd * (w1 - dd * (w2 - dd * w3));
-            float sd, cd;
             if constexpr (TIER < NTIER)
-                sincos_series<TierF32<TIER>::KD>(dphi, sd, cd);
+                sincos_series<TierF32<TIER>::KD>(dphi, sphi[sc], cphi[sc]);
             else
-                sincos_f32(dphi, sd, cd);
-            sphi[sc] = sC * cd + cC * sd;
-            cphi[sc] = cC * cd - sC * sd;
+                sincos_f32(dphi, sphi[sc], cphi[sc]);
             fonfs[sc] = (fc - dd * fdc) * fstar_inv;  // f(xi_c - d), dropping fddot d^2/2 (< 1e-22 Hz)
         }
     } else {
@@ -176,23 +176,35 @@ __device__ __forceinline__ void slow_xyz_mixed(const Src &s, const SrcF &sf, con
     const float2 G21 = cmul(Tf(fonfs[1] * (1.0f - kr12)), cmul(E1, A12));
     const float2 G32 = cmul(Tf(fonfs[2] * (1.0f - kr23)), cmul(E2, A23));
     const float2 G13 = cmul(Tf(fonfs[0] * (1.0f + kr13)), cmul(E0, A31));
+    // TDI factor: omL = f(t)/f*
     float omL, sL, cL;
+    double omLd;  // the same omL in FP64, for the centre phase
     if constexpr (TIER < NTIER) {
         const float tf = (float)t;
         const float f = sf.f0 + sf.fdot * tf + 0.5f * sf.fddot * tf * tf;
         omL = f * fstar_inv;
+        omLd = (double)omL;
         sincos_series<TierF32<TIER>::KS>(omL, sL, cL);
     } else {
         // omL can sit near a null of sin(omL) (f ~ pi f*), where an FP32
         // argument would cost relative accuracy: reduce it in FP64.
         const double fd = s.f0 + s.fdot * t + 0.5 * s.fddot * t * t;
-        const double omLd = fd * fstar_inv_d;
+        omLd = fd * fstar_inv_d;
         omL = (float)omLd;
         sincos_rev(omLd * INV_TWO_PI, sL, cL);
     }
-    const float2 fctr = make_float2(cL, -sL);
     const float m = 4.0f * omL * sL * scale;
-    F = make_float2(m * cL, -m * sL);  // fctr2
+    if constexpr (!THIRD) {
+        const double kPc2 = (s.k[0] * T(ROW_PC, n) + s.k[1] * T(ROW_PC + 1, n) + s.k[2] * T(ROW_PC + 2, n)) * INV_C;
+        const double xc = t - kPc2;
+        const double argSc = s.phi0 + s.omq * t + PI * s.fdot * (xc * xc) + (1.0 / 3.0) * PI * s.fddot * (xc * xc * xc);
+        float sC, cC;
+        sincos_rev((s.om * kPc2 - argSc + omLd) * INV_TWO_PI, sC, cC);
+        F = make_float2(m * cC, -m * sC);  // fctr2 e^{-i Phi_c}
+    } else {
+        F = make_float2(m * cL, -m * sL);  // fctr2
+    }
+    const float2 fctr = make_float2(cL, -sL);
     X = cadd(csub(G21, G31), cmul(csub(G12, G13), fctr));
     Y = cadd(csub(G32, G12), cmul(csub(G23, G21), fctr));
     Z = cadd(csub(G13, G23), cmul(csub(G31, G32), fctr));
