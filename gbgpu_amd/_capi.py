@@ -96,6 +96,7 @@ SIGNATURES = {
         [c_void_p, c_void_p, c_void_p, c_void_p, c_int, c_int, ctypes.POINTER(WaveCfg), c_void_p],
     ),
     "gbgpu_fma_probe": (c_int, [c_int, ctypes.POINTER(c_double), c_void_p]),
+    "gbgpu_bw_probe": (c_int, [c_int, ctypes.POINTER(c_double), c_void_p]),
     "gbgpu_hook_inputs": (
         c_int,
         [c_void_p, c_void_p, c_void_p, c_void_p, c_int, c_int, ctypes.POINTER(WaveCfg), c_void_p],

@@ -352,6 +352,14 @@ int gbgpu_kernel_timing_read(double *total_ms, int *count);
  * (GBGPU_MIXED).  Synchronises `stream` (about 30 ms); for rooflines. */
 int gbgpu_fma_probe(int precision, double *tflops, void *stream);
 
+/* Measurement hook with no counterpart in the reference: the HBM streaming
+ * rate (GB/s) of a 1 GiB buffer, one 16-byte non-temporal access per thread
+ * over contiguous 4 KiB chunks per workgroup; mode 0 reads it, mode 1 copies
+ * it (read + write bytes counted).
+ * Allocates 1-2 GiB stream-ordered and synchronises `stream`; for
+ * rooflines. */
+int gbgpu_bw_probe(int mode, double *gbps, void *stream);
+
 #ifdef __cplusplus
 }
 #endif
