@@ -74,6 +74,7 @@ class BatchCfg(ctypes.Structure):
         ("strategy", c_int),
         ("guard_tol", c_double),
         ("transform", c_int),
+        ("bucket_capacity", c_int * NBUCKET),
     ]
 
 

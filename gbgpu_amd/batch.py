@@ -154,6 +154,21 @@ class BatchedLogLike:
             self._tabs[N] = self.gb._table(self.T, N)
             self.cfg.orbit_tables[k] = self._tabs[N].data_ptr()
 
+    def _set_capacity(self, Nn, slack=False):
+        """Per-bucket size bounds for the launches (``bucket_capacity``):
+        the exact bucket sizes of the host-side N array Nn, or with ``slack``
+        (N recomputed on the device between calls, as in the sampler loop)
+        those plus 1% and 1024 for binaries that cross a bucket edge.  A
+        bound that a call exceeds only costs speed: the binaries beyond it are
+        evaluated by the strided kernel."""
+        counts = np.bincount(np.log2(np.asarray(Nn)).astype(np.int64) - 5, minlength=_capi.NBUCKET)
+        B = len(Nn)
+        for k in range(_capi.NBUCKET):
+            c = int(counts[k])
+            if slack:
+                c = min(B, c + c // 100 + 1024)
+            self.cfg.bucket_capacity[k] = c
+
     def _workspace(self, B):
         """Workspace for B binaries.  A buffer whose address a captured graph
         holds is never replaced: a larger batch after capture gets a new
@@ -182,6 +197,7 @@ class BatchedLogLike:
         for n in np.unique(Nn):
             self._add_table(int(n))
         Nt = torch.from_numpy(Nn).to(self.dev)
+        self._set_capacity(Nn)
         ll = torch.empty(B, dtype=torch.float64, device=self.dev)
         ws = self._workspace(B)
         chosen = np.zeros(_capi.NBUCKET, dtype=np.int32)
@@ -231,6 +247,7 @@ class BatchedLogLike:
         for n in np.unique(Nn):
             self._add_table(int(n))
         Nt = torch.from_numpy(Nn).to(self.dev) if not torch.is_tensor(N) else N.to(self.dev, torch.int32)
+        self._set_capacity(Nn)
         ll = torch.empty(B, dtype=torch.float64, device=self.dev)
         dh = torch.empty(B, dtype=torch.complex128, device=self.dev) if d_h else None
         hh = torch.empty(B, dtype=torch.complex128, device=self.dev) if d_h else None
@@ -258,6 +275,8 @@ class BatchedLogLike:
         Nj = torch.empty(B, dtype=torch.int32, device=self.dev)
         edges = np.ascontiguousarray(self.f_edges)
         levels = np.ascontiguousarray(self.N_levels)
+        # the jumps move a few binaries across bucket edges: bounds with slack
+        self._set_capacity(self.N_of(base[1].cpu().numpy()), slack=True)
         return base, out, pj, Nj, edges, levels
 
     def _loop_body(self, base, out, pj, Nj, edges, levels, n_calls, sigma, seed):

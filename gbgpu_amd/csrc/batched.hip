@@ -71,6 +71,7 @@ struct BatchArgs {
     int *cell;  // histogram cell per binary (-1: not evaluated)
     int *redo;  // accuracy-guard list (same per-bucket offsets as `list`)
     int *ctl;
+    long long cap[NBUCKET];  // host-side bound on each bucket's size (0: none)
     double guard_tol;
     double route_tol;  // > 0: route binaries whose estimated error bound exceeds it to FP64
     int transform;     // GBGPU_TRANSFORM_* requested for the mixed N = 256 bucket
@@ -390,6 +391,7 @@ static int launch_buckets(const BatchArgs &ba, int precision, hipStream_t st, co
         a.order = ba.list;
         a.seg_count = ba.ctl + CTL_COUNT + k;
         a.seg_off = ba.ctl + CTL_LISTOFF + k;
+        a.seg_cap = ba.cap[k];  // also bounds the guard's and the routed FP64 passes
         if (guard) {
             a.redo_list = ba.redo;
             a.redo_count = ba.ctl + CTL_REDO + k;
@@ -565,6 +567,13 @@ static int make_batch_args(BatchArgs &ba, double *loglike, gbgpu_cmplx *d_h, gbg
         // the N = 256 bucket may run a matrix-core transform (requested or tuned)
         a.dftc = dft256_table();
         if (!a.dftc) return hip_fail(hipErrorOutOfMemory, "transform constants");
+    }
+    for (int k = 0; k < NBUCKET; k++) {
+        if (cfg->bucket_capacity[k] < 0) {
+            set_error("bucket_capacity must be >= 0");
+            return GBGPU_E_ARG;
+        }
+        ba.cap[k] = cfg->bucket_capacity[k];
     }
     ba.guard_tol = cfg->guard_tol;
     ba.route_tol = (cfg->precision == GBGPU_MIXED && cfg->guard_tol > 0.0) ? cfg->guard_tol : 0.0;

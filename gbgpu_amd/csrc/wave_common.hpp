@@ -75,6 +75,11 @@ struct WaveArgs {
     // is binary order[*seg_off + i].  Null: items 0..B-1 (one grid pass).
     const int *seg_count;
     const int *seg_off;
+    // Host-side bound on *seg_count, or 0 when none is known.  With a bound
+    // the segment runs as one pass of workgroups sized for it (no loop, so
+    // the kernel keeps the uniform kernel's registers) followed by the
+    // strided kernel, which takes any items beyond the bound.
+    long long seg_cap;
     // Accuracy guard of the mixed likelihood: binaries whose error bound
     // GUARD_REL * (<h|h> + |<d|h>|) exceeds redo_tol are appended to
     // redo_list[*seg_off + i] (i from atomicAdd(redo_count)) for an FP64 pass.

@@ -5,6 +5,7 @@
 // wave_seg3.hip (two translation units that compile in parallel).
 #pragma once
 
+#include <algorithm>
 #include <type_traits>
 
 #include "common.hpp"
@@ -15,17 +16,19 @@ namespace gbgpu {
 
 // Fused likelihood over one device-side segment of a batch: item i <
 // *a.seg_count is binary a.order[*a.seg_off + i]; the grid strides over the
-// segment.  The orbit-table and twiddle reads and the per-thread index
-// arithmetic are the same in every pass: opaque copies of the pointers and
-// of the thread index keep the compiler from hoisting all of them out of the
-// loop into registers (which would cost occupancy).
+// workgroup passes blk0, blk0 + 1, ... of the segment.  The orbit-table and
+// twiddle reads and the per-thread index arithmetic are the same in every
+// pass: opaque copies of the pointers and of the thread index keep the
+// compiler from hoisting all of them out of the loop into registers.  The
+// loop still keeps more values live than one pass does (146 against 120
+// VGPRs for the FP64 N = 256 tile), hence k_wave_seg1 below.
 template <int N, bool THIRD, class P, class PK, int M, int DFT = GBGPU_TRANSFORM_LDS>
-__global__ void __launch_bounds__((Geo<N, M>::WG)) k_wave_seg(WaveArgs a) {
+__global__ void __launch_bounds__((Geo<N, M>::WG)) k_wave_seg(WaveArgs a, long long blk0) {
     constexpr int BPW = Geo<N, M>::BPW;
     __shared__ WaveLds<N, MODE_LL, THIRD, P, PK, M, DFT> lds;
     const long long count = *a.seg_count;
     const long long first = *a.seg_off;
-    for (long long blk = blockIdx.x; blk * BPW < count; blk += gridDim.x) {
+    for (long long blk = blk0 + blockIdx.x; blk * BPW < count; blk += gridDim.x) {
         const double *tab = a.tab;
         const void *twp = a.tw;
         int tid = threadIdx.x;
@@ -35,12 +38,29 @@ __global__ void __launch_bounds__((Geo<N, M>::WG)) k_wave_seg(WaveArgs a) {
     }
 }
 
+// One workgroup pass per workgroup: pass blockIdx.x of the segment, for a
+// grid sized from a host-side bound on the segment (workgroups past the
+// device-side count leave at once, about 0.2 ns each).
+template <int N, bool THIRD, class P, class PK, int M, int DFT = GBGPU_TRANSFORM_LDS>
+__global__ void __launch_bounds__((Geo<N, M>::WG)) k_wave_seg1(WaveArgs a) {
+    constexpr int BPW = Geo<N, M>::BPW;
+    __shared__ WaveLds<N, MODE_LL, THIRD, P, PK, M, DFT> lds;
+    const long long count = *a.seg_count;
+    const long long blk = blockIdx.x;
+    if (blk * BPW >= count) return;  // uniform over the workgroup
+    wave_pass<N, MODE_LL, THIRD, P, PK, M, DFT>(a, lds, blk, count, *a.seg_off, threadIdx.x, a.tab, a.tw);
+}
+
 // Likelihood kernel over one device-side segment of a batch (the N-bucket
 // launches of gbgpu_wave_ll_batched): a.pack, a.order, a.seg_count and
-// a.seg_off are set by the caller; the segment's size is known only on the
-// device, so the grid covers a few passes of resident workgroups and strides.
+// a.seg_off are set by the caller, and the segment's size is known only on
+// the device.  With a host-side bound a.seg_cap, one k_wave_seg1 pass covers
+// the first ceil(seg_cap / BPW) workgroup passes; the strided k_wave_seg
+// (a few passes' worth of resident workgroups) covers the rest, and leaves at
+// once when the bound held.  Without a bound the strided kernel does it all.
 template <bool THIRD, class P, class PK, int NN, int M, int DFT = GBGPU_TRANSFORM_LDS>
 int launch_segment_n(const WaveArgs &a, hipStream_t st) {
+    constexpr long long BPW = Geo<NN, M>::BPW;
     // several passes' worth of resident workgroups (initialised once, thread-safe)
     static const int grid = [] {
         int dev = 0, ncu = 0;
@@ -48,10 +68,19 @@ int launch_segment_n(const WaveArgs &a, hipStream_t st) {
         (void)hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev);
         return 16 * (ncu > 0 ? ncu : 256);
     }();
-    long long g = (a.B + Geo<NN, M>::BPW - 1) / Geo<NN, M>::BPW;
-    if (g > grid) g = grid;
     KernelTimer kt(st);
-    hipLaunchKernelGGL((k_wave_seg<NN, THIRD, P, PK, M, DFT>), dim3((unsigned)g), dim3(Geo<NN, M>::WG), 0, st, a);
+    const long long passes = (a.B + BPW - 1) / BPW;  // every binary of the batch
+    long long done = 0;
+    if (a.seg_cap > 0) {
+        done = (std::min<long long>(a.seg_cap, a.B) + BPW - 1) / BPW;
+        hipLaunchKernelGGL((k_wave_seg1<NN, THIRD, P, PK, M, DFT>), dim3((unsigned)done), dim3(Geo<NN, M>::WG), 0,
+                           st, a);
+        GBGPU_CHECK_LAUNCH("k_wave_seg1");
+        if (done >= passes) return GBGPU_OK;
+    }
+    const long long g = std::min<long long>(passes - done, grid);
+    hipLaunchKernelGGL((k_wave_seg<NN, THIRD, P, PK, M, DFT>), dim3((unsigned)g), dim3(Geo<NN, M>::WG), 0, st, a,
+                       done);
     GBGPU_CHECK_LAUNCH("k_wave_seg");
     return GBGPU_OK;
 }

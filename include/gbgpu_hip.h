@@ -240,6 +240,12 @@ typedef struct gbgpu_batch_cfg {
                                       pass when its own result does; <= 0 disables */
     int transform;                 /* GBGPU_TRANSFORM_* for the mixed N = 256 bucket (every
                                       other bucket and every FP64 pass uses the LDS transform) */
+    int bucket_capacity[9];        /* optional bound on the number of binaries with N = 32 << k
+                                      (0: none).  A bucket with a bound runs as one pass of
+                                      workgroups sized for it, with the registers of the uniform
+                                      kernel; binaries beyond the bound are still evaluated, by
+                                      the strided kernel that every bucket without a bound uses.
+                                      Results do not depend on it. */
 } gbgpu_batch_cfg;
 
 /* Workspace bytes gbgpu_wave_ll_batched needs for num_bin binaries. */
@@ -331,7 +337,7 @@ int gbgpu_wave_ll_ext(double *loglike, gbgpu_cmplx *d_h, gbgpu_cmplx *h_h, const
 /* Message describing the last failure on the calling thread. */
 const char *gbgpu_last_error(void);
 
-/* ABI version of this header (7). */
+/* ABI version of this header (8). */
 int gbgpu_abi_version(void);
 
 /* Measurement hook with no counterpart in the reference: while enabled,

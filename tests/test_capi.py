@@ -32,7 +32,7 @@ def test_library_exports_every_declared_symbol():
         assert hasattr(lib, n), n
     # the ctypes table covers exactly the declared functions
     assert set(_capi.SIGNATURES) == set(declared_functions())
-    assert lib.gbgpu_abi_version() == 7
+    assert lib.gbgpu_abi_version() == 8
 
 
 def test_invalid_arguments_return_errors_without_gpu_work():

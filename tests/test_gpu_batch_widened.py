@@ -164,3 +164,41 @@ def test_captured_loop_with_third_body_tdi2_and_streams():
         assert np.array_equal(out.cpu().numpy(), eager)
         bl(kp)  # eager work between replays
         torch.cuda.synchronize()
+
+
+def test_bucket_capacity_bounds_do_not_change_results():
+    """bucket_capacity only selects kernels: no bounds (the strided kernel
+    alone), the exact bucket sizes (one pass per workgroup), halved sizes and
+    one binary per bucket (nearly every binary past its bound, taken by the
+    strided kernel after the single pass) give the same logL bit for bit, in
+    FP64 and in guarded mixed (whose FP64 passes take the same bounds)."""
+    from gbgpu_amd import GBGPU, _capi
+    from gbgpu_amd.batch import BatchedLogLike
+
+    rng = np.random.default_rng(5400)
+    A, E, SA, SE, df, L = _streams(rng)
+    B = 300
+    p = draw_params(B, rng, f_lo=1e-3, f_hi=4e-3)
+    p[0, :40] *= 300.0  # loud: routed to FP64 or re-evaluated in the mixed batch
+    N = rng.choice([64, 128, 256, 512, 1024, 2048], B).astype(np.int32)
+    counts = np.bincount(np.log2(N).astype(int) - 5, minlength=_capi.NBUCKET)
+    gb = GBGPU()
+    gb.d_d = 0.0
+    for prec, guard in (("fp64", None), ("mixed", 1e-3)):
+        bl = BatchedLogLike(gb, [A[0], E[0]], [SA[0], SE[0]], YR, DT, precision=prec, guard_tol=guard)
+        kp = bl.kernel_params(p)
+        print(f"capacity bounds {prec}: exact", flush=True)
+        ref = bl(kp, N=N).cpu().numpy()
+        assert list(bl.cfg.bucket_capacity) == [int(c) for c in counts]
+        assert np.all(np.isfinite(ref))
+        for name, caps in (("none", np.zeros_like(counts)), ("halved", counts // 2),
+                           ("one", np.minimum(counts, 1))):
+
+            def fixed(Nn, slack=False, caps=caps):
+                for k in range(_capi.NBUCKET):
+                    bl.cfg.bucket_capacity[k] = int(caps[k])
+
+            bl._set_capacity = fixed
+            print(f"capacity bounds {prec}: {name}", flush=True)
+            got = bl(kp, N=N).cpu().numpy()
+            assert np.array_equal(got, ref), (prec, name, np.abs(got - ref).max())
