@@ -255,10 +255,7 @@ k_fisher(WaveArgs a, double *__restrict__ fisher, double *__restrict__ grad) {
     const long long j0 = s.start - a.start_freq_ind;
     const double2 *tw = static_cast<const double2 *>(a.tw);
 
-    double acc[FG::NRED];
-#pragma unroll
-    for (int i = 0; i < FG::NRED; i++) acc[i] = 0.0;
-
+    const int w = tid / 64;
     for (int pass = 0; pass < FG::NPASS; pass++) {
         // tangents of this pass's channel(s) into the slots
         for (int q = 0; q < FG::NSPT; q++) {
@@ -283,7 +280,11 @@ k_fisher(WaveArgs a, double *__restrict__ fisher, double *__restrict__ grad) {
             const int sg = tid / FG::TPBF, lane = tid % FG::TPBF;
             fft_lds<N, FG::SPS, FG::TPBF>(buf + sg * FG::SPS * N, lane, tw);
         }
-        // Gram and residual sums over this pass's bins
+        // Gram and residual sums over this pass's bins, in registers that
+        // live only here (not across the tangents and the transform)
+        double acc[FG::NRED];
+#pragma unroll
+        for (int i = 0; i < FG::NRED; i++) acc[i] = 0.0;
         for (int q = 0; q < FG::NSPT; q++) {
             const int m = tid + q * FG::WG;
             if (m >= N) break;
@@ -324,16 +325,15 @@ k_fisher(WaveArgs a, double *__restrict__ fisher, double *__restrict__ grad) {
                 }
             }
         }
+        // wavefront sums, added to the wavefront's row of red in pass order
+#pragma unroll
+        for (int i = 0; i < FG::NRED; i++) {
+            const double v = group_sum<64>(acc[i]);
+            if ((tid & 63) == 0) red[w * FG::NRED + i] = pass == 0 ? v : red[w * FG::NRED + i] + v;
+        }
         __syncthreads();  // the next pass overwrites the slots
     }
     // workgroup reduction
-    const int w = tid / 64;
-#pragma unroll
-    for (int i = 0; i < FG::NRED; i++) {
-        const double v = group_sum<64>(acc[i]);
-        if ((tid & 63) == 0) red[w * FG::NRED + i] = v;
-    }
-    __syncthreads();
     if (tid < FG::NRED) {
         double v = 0.0;
         for (int i = 0; i < FG::WG / 64; i++) v += red[i * FG::NRED + tid];
