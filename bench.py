@@ -759,6 +759,25 @@ def main():
     flops = flop_per_binary(N) * B
     achieved_tflops = flops / kernel_s / 1e12
 
+    # the FP64 policy's matrix-core transform (same workload, same arithmetic
+    # type), against the LDS transform timed above
+    fp64_transforms = None
+    if args.only != "mixed" and not args.no_transforms:
+        print("[bench] timing fp64 transform=mfma_f64", file=sys.stderr, flush=True)
+        call64()
+        torch.cuda.synchronize()
+        ll_lds = ll.cpu().numpy()
+        callf, _tf = runner("fp64", "mfma_f64")
+        for _ in range(max(1, args.warmup)):
+            callf()
+        torch.cuda.synchronize()
+        with _capi.KernelTimer() as ktf:
+            secf, _ = time_calls(callf, args.steps, 0)
+        fp64_transforms = {"mfma_f64": dict(
+            value=B / secf, ms_per_step=secf * 1e3, kernel_ms=ktf.mean_ms, speedup_vs_lds_transform=sec64 / secf,
+            max_abs_logL_diff_vs_lds=float(np.max(np.abs(ll.cpu().numpy() - ll_lds))),
+            kernel="k_wave<256, MODE_LL, circular, F64, GBGPU_TRANSFORM_MFMA_F64>")}
+
     mixed = None
     if args.only != "fp64":
         callm, _tm = runner("mixed")
@@ -903,6 +922,7 @@ def main():
         "parity": "HIP vs oracle: tests/test_gpu_parity.py, tests/test_gpu_golden.py, tests/test_gpu_configs_4yr.py",
         "attainable_hbm_GB_per_s": {"read": read_gbs, "copy": copy_gbs,
                                     "note": "gbgpu_bw_probe: 1 GiB, one 16-byte non-temporal access per thread"},
+        "fp64_transforms": fp64_transforms,
         "mixed": mixed,
         "uneven_batch": uneven,
         "dropin_get_ll": dropin,

@@ -18,7 +18,9 @@ LIB_PATH = Path(__file__).resolve().parent / "libgbgpu_hip.so"
 GBGPU_FP64 = 0
 GBGPU_MIXED = 1
 # transform of the slow part (gbgpu_wave_cfg.transform, gbgpu_batch_cfg.transform)
-TRANSFORMS = {"lds": 0, "mfma_f32": 1, "mfma_bf16x3": 2, "mfma_bf16x6": 3}
+TRANSFORMS = {"lds": 0, "mfma_f32": 1, "mfma_bf16x3": 2, "mfma_bf16x6": 3, "mfma_f64": 4}
+# the precision policy each matrix-core transform belongs to
+TRANSFORM_PRECISION = {"mfma_f32": "mixed", "mfma_bf16x3": "mixed", "mfma_bf16x6": "mixed", "mfma_f64": "fp64"}
 GBGPU_BATCH_BUCKETS = 0
 GBGPU_BATCH_PERSISTENT = 1
 ORBIT_ROWS = 22

@@ -106,7 +106,9 @@ class BatchedLogLike:
         self.cfg = _capi.BatchCfg(T=float(self.T), armlength=float(gb.orbits.armlength), tdi2=int(bool(tdi2)),
                                   precision=1 if precision == "mixed" else 0, strategy=self.STRATEGIES[strategy],
                                   guard_tol=self.guard_tol,
-                                  transform=_capi.TRANSFORMS[transform] if precision == "mixed" else 0)
+                                  transform=_capi.TRANSFORMS[transform]
+                                  if _capi.TRANSFORM_PRECISION.get(transform) == "mixed" and precision == "mixed"
+                                  else 0)
         self._tabs = {}
         for N in self.N_levels:
             self._add_table(int(N))

@@ -21,13 +21,15 @@
 //   Cim[j] = j < 4 ? Im F[4g+j, c] :  Re F[4g+j-4, c]   (gives Im of the product)
 //
 // Variants (GBGPU_TRANSFORM_*):
+//   MFMA_F64     v_mfma_f64_16x16x4_f64, 8 per real output tile: FP64
+//                products and accumulation (the FP64 policy).
 //   MFMA_F32     v_mfma_f32_16x16x4_f32, 8 per real output tile: FP32 products.
 //   MFMA_BF16X3  v_mfma_f32_16x16x32_bf16 on a two-piece bf16 split of data
 //                and constants (v = hi + lo), three products hi.hi + hi.lo +
 //                lo.hi: about 16 significant bits.
 //   MFMA_BF16X6  three-piece split (hi + mid + lo), six products down to the
 //                2^-16-relative terms: FP32-level accuracy.
-// FP32 accumulation throughout, smallest products first.
+// FP32 accumulation in the FP32 and bf16 variants, smallest products first.
 #pragma once
 
 #include "common.hpp"
@@ -36,6 +38,7 @@ namespace gbgpu {
 
 typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
 typedef float f32x4 __attribute__((ext_vector_type(4)));
+typedef double f64x4 __attribute__((ext_vector_type(4)));
 typedef unsigned u32x4 __attribute__((ext_vector_type(4)));
 
 // Per-lane constant table: for variant v, QN[v] rows of 64 lanes x 16 bytes
@@ -70,6 +73,38 @@ struct Dft256Lane<GBGPU_TRANSFORM_MFMA_BF16X6> {
     bf16x8 cre[3], cim[3];  // hi, mid, lo
     float2 tw[4];
 };
+
+// FP64 (v_mfma_f64_16x16x4f64).  Its A and B operand layouts are those of
+// the FP32 16 x 16 x 4 instruction, but lane c + 16 g holds rows g + 4 r of
+// its 16 x 16 result (r = 0..3; rows 4 g + r for FP32; measured).  With the
+// element index e = g + 4 t for K slot t (t < 4: Re, else Im of element
+// e(g, t - 4)) in both steps, lane l then holds sample l + 64 i in register
+// i before the transform and bin l + 64 i after it: the natural strided
+// order of the LDS transform's register ends.  The constants come straight
+// from the FP64 twiddle table of N = 256, w[k] = W256^k, since
+// F[a, b] = W16^(a b) = w[16 a b mod 256], and the step twiddles are
+// W256^(c (g + 4 i)).
+template <>
+struct Dft256Lane<GBGPU_TRANSFORM_MFMA_F64> {
+    double cre[8], cim[8];
+    double2 tw[4];
+};
+
+__device__ __forceinline__ Dft256Lane<GBGPU_TRANSFORM_MFMA_F64> load_dft256_f64(const double2 *__restrict__ w,
+                                                                              int lane) {
+    Dft256Lane<GBGPU_TRANSFORM_MFMA_F64> k;
+    const int c = lane & 15, g = lane >> 4;
+#pragma unroll
+    for (int j = 0; j < 4; j++) {
+        const double2 f = w[(16 * (g + 4 * j) * c) & 255];  // F[g+4j, c]
+        k.cre[j] = f.x;
+        k.cim[j] = f.y;
+        k.cre[4 + j] = -f.y;
+        k.cim[4 + j] = f.x;
+        k.tw[j] = w[(c * (g + 4 * j)) & 255];  // W256^(c (g+4j))
+    }
+    return k;
+}
 
 template <int V>
 __device__ __forceinline__ Dft256Lane<V> load_dft256(const void *table, int lane) {
@@ -174,6 +209,48 @@ __device__ __forceinline__ void cgemm16(const float (&v)[8], const Dft256Lane<V>
     }
 }
 
+// FP64 products, FP64 accumulation.
+template <bool DATA_IS_A>
+__device__ __forceinline__ void cgemm16_f64(const double (&v)[8], const Dft256Lane<GBGPU_TRANSFORM_MFMA_F64> &k,
+                                            f64x4 &re, f64x4 &im) {
+    re = f64x4{0.0, 0.0, 0.0, 0.0};
+    im = f64x4{0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+    for (int t = 0; t < 8; t++) {
+        if constexpr (DATA_IS_A) {
+            re = __builtin_amdgcn_mfma_f64_16x16x4f64(v[t], k.cre[t], re, 0, 0, 0);
+            im = __builtin_amdgcn_mfma_f64_16x16x4f64(v[t], k.cim[t], im, 0, 0, 0);
+        } else {
+            re = __builtin_amdgcn_mfma_f64_16x16x4f64(k.cre[t], v[t], re, 0, 0, 0);
+            im = __builtin_amdgcn_mfma_f64_16x16x4f64(k.cim[t], v[t], im, 0, 0, 0);
+        }
+    }
+}
+
+// In place: x[i] = element lane + 64 i of the binary's 256 samples on entry,
+// of its DFT on exit.  Every lane of the wavefront must call this together.
+template <int V>
+__device__ __forceinline__ void dft256_mfma(double2 *x, const Dft256Lane<V> &k) {
+    static_assert(V == GBGPU_TRANSFORM_MFMA_F64, "FP64 spectra take the FP64 matrix transform");
+    double v[8];
+#pragma unroll
+    for (int i = 0; i < 4; i++) {
+        v[i] = x[i].x;
+        v[4 + i] = x[i].y;
+    }
+    f64x4 re, im;
+    cgemm16_f64<true>(v, k, re, im);
+#pragma unroll
+    for (int i = 0; i < 4; i++) {
+        const double2 y = cmul(make_double2(re[i], im[i]), k.tw[i]);
+        v[i] = y.x;
+        v[4 + i] = y.y;
+    }
+    cgemm16_f64<false>(v, k, re, im);
+#pragma unroll
+    for (int i = 0; i < 4; i++) x[i] = make_double2(re[i], im[i]);
+}
+
 // In place: x[i] = element c + 64 g + 16 i of the binary's 256 samples on
 // entry, of its DFT on exit (lane = c + 16 g).  Every lane of the wavefront
 // must call this together.
@@ -198,7 +275,8 @@ __device__ __forceinline__ void dft256_mfma(float2 *x, const Dft256Lane<V> &k) {
     for (int i = 0; i < 4; i++) x[i] = make_float2(re[i], im[i]);
 }
 
-// Sample / output index held in register i of lane `lane`.
+// Sample / output index held in register i of lane `lane` (FP32 and bf16
+// variants; the FP64 variant holds lane + 64 i).
 __device__ __forceinline__ int dft256_index(int lane, int i) { return (lane & 15) + 64 * (lane >> 4) + 16 * i; }
 
 }  // namespace gbgpu

@@ -3,9 +3,8 @@
 // the caller's launch, then release of the workspace on the same stream.
 #pragma once
 
-#include <hipcub/hipcub.hpp>
-
 #include "common.hpp"
+#include "sort.hpp"
 #include "wave_common.hpp"
 
 namespace gbgpu {
@@ -17,14 +16,15 @@ inline size_t align256(size_t n) { return (n + 255) & ~(size_t)255; }
 
 // Bytes of the per-binary workspace of with_pack for B binaries: the Pack
 // array and, for sorted batches, keys, values and the radix sort's temporary
-// storage.  The size query of the sort allocates nothing.
+// storage (for the sort with_pack will run).  The size query of the sort
+// allocates nothing.
 template <class P>
-inline int pack_workspace_bytes(long long B, hipStream_t st, size_t *total, size_t *sort_bytes) {
+inline int pack_workspace_bytes(long long B, hipStream_t st, size_t *total, size_t *sort_bytes,
+                                bool allow_onesweep) {
     *sort_bytes = 0;
     if (B >= SORT_MIN_BATCH) {
-        hipError_t e = hipcub::DeviceRadixSort::SortPairs(nullptr, *sort_bytes, (unsigned *)nullptr,
-                                                          (unsigned *)nullptr, (int *)nullptr, (int *)nullptr,
-                                                          (int)B, 0, SORT_KEY_BITS, st);
+        hipError_t e = sort_pairs<unsigned>(allow_onesweep, nullptr, *sort_bytes, nullptr, nullptr, nullptr, nullptr,
+                                            (int)B, SORT_KEY_BITS, st);
         if (e != hipSuccess) return hip_fail(e, "SortPairs(size)");
     }
     *total = align256(sizeof(Pack<P>) * (size_t)B) +
@@ -35,14 +35,17 @@ inline int pack_workspace_bytes(long long B, hipStream_t st, size_t *total, size
 // Per-binary set-up (k_setup) into a workspace, the start-bin sort for large
 // batches, then `launch(args)`.  With ext == nullptr the workspace is
 // stream-ordered scratch (hipMallocAsync, freed with hipFreeAsync after the
-// launch); otherwise it is carved from the caller's ext_bytes at ext and
-// nothing is allocated.
+// launch) and large batches may use the onesweep radix sort; otherwise the
+// workspace is carved from the caller's ext_bytes at ext, nothing is
+// allocated and the sort is the merge sort, which issues no memset, so the
+// call can be captured in a graph.
 template <class P, class Launch>
 inline int with_pack(WaveArgs a, int N, hipStream_t st, Launch launch, char *ext = nullptr, size_t ext_bytes = 0) {
     const size_t B = (size_t)a.B;
     const bool sort = a.B >= SORT_MIN_BATCH;
+    const bool allow_onesweep = ext == nullptr;
     size_t bytes = 0, sort_bytes = 0;
-    int rc = pack_workspace_bytes<P>(a.B, st, &bytes, &sort_bytes);
+    int rc = pack_workspace_bytes<P>(a.B, st, &bytes, &sort_bytes, allow_onesweep);
     if (rc) return rc;
     char *ws = ext;
     if (ext) {
@@ -77,8 +80,8 @@ inline int with_pack(WaveArgs a, int N, hipStream_t st, Launch launch, char *ext
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) rc = hip_fail(e, "k_setup");
     if (!rc && sort) {
-        e = hipcub::DeviceRadixSort::SortPairs(tmp, sort_bytes, keys, keys_out, vals, order, (int)a.B, 0,
-                                               SORT_KEY_BITS, st);
+        e = sort_pairs<unsigned>(allow_onesweep, tmp, sort_bytes, keys, keys_out, vals, order, (int)a.B, SORT_KEY_BITS,
+                                 st);
         if (e != hipSuccess) rc = hip_fail(e, "SortPairs");
     }
     a.keys = nullptr;

@@ -14,10 +14,10 @@
 // search, splits it over its wavefronts and sums their samples in sorted
 // order -- one lane per output bin, fixed summation order, so results are
 // bit-identical run to run.
-#include <hipcub/hipcub.hpp>
 
 #include "common.hpp"
 #include "fastmath.hpp"
+#include "sort.hpp"
 
 namespace gbgpu {
 
@@ -267,8 +267,7 @@ static int fill_rows_impl(double2 *Aglob, double2 *Eglob, const double2 *rowsA, 
     const unsigned long long span = (unsigned long long)G * (unsigned long long)(L + M);
     int end_bit = 1;
     while (end_bit < 64 && (1ULL << end_bit) <= span) end_bit++;
-    e = hipcub::DeviceRadixSort::SortPairs(nullptr, tmp_bytes, keys, keys_sorted, vals, perm, (int)B, 0, end_bit,
-                                           st);
+    e = sort_pairs<unsigned long long>(true, nullptr, tmp_bytes, keys, keys_sorted, vals, perm, (int)B, end_bit, st);
     if (e != hipSuccess) return hip_fail(e, "SortPairs(size)");
     const size_t kb = sizeof(unsigned long long) * (size_t)B, vb = sizeof(int) * (size_t)B;
     char *ws = nullptr;
@@ -288,7 +287,7 @@ static int fill_rows_impl(double2 *Aglob, double2 *Eglob, const double2 *rowsA, 
     tmp = carve(tmp_bytes);
     hipLaunchKernelGGL(k_fill_keys, dim3((unsigned)((B + 255) / 256)), dim3(256), 0, st, keys, vals, start, start64,
                        group, B, M, L, G, start_shift);
-    e = hipcub::DeviceRadixSort::SortPairs(tmp, tmp_bytes, keys, keys_sorted, vals, perm, (int)B, 0, end_bit, st);
+    e = sort_pairs<unsigned long long>(true, tmp, tmp_bytes, keys, keys_sorted, vals, perm, (int)B, end_bit, st);
     if (e != hipSuccess) {
         (void)hipFreeAsync(ws, st);
         return hip_fail(e, "SortPairs");

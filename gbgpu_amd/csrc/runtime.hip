@@ -207,4 +207,4 @@ extern "C" int gbgpu_kernel_timing_read(double *total_ms, int *count) {
 
 extern "C" const char *gbgpu_last_error(void) { return g_last_error.c_str(); }
 
-extern "C" int gbgpu_abi_version(void) { return 8; }
+extern "C" int gbgpu_abi_version(void) { return 9; }

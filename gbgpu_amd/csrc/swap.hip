@@ -11,7 +11,6 @@
 // (same N), both spectra stay in LDS, the remove group sums over the remove
 // window (including the overlap term, reading h_a from its neighbour's
 // buffer) and the add group over the add window.
-#include <hipcub/hipcub.hpp>
 
 #include "common.hpp"
 #include "wave_common.hpp"

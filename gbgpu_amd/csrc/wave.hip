@@ -12,7 +12,6 @@
 // (wave_pass.hpp).  N = 4096 and 8192 run the
 // large-N kernels of wave_big.hip; the batch segment kernels are in
 // wave_seg.hip.
-#include <hipcub/hipcub.hpp>
 #include <map>
 #include <mutex>
 #include <type_traits>
@@ -35,12 +34,19 @@ __global__ void __launch_bounds__((Geo<N, wave_tile<N>()>::WG)) k_wave(WaveArgs 
     wave_pass<N, MODE, THIRD, P, P, M, DFT>(a, lds, blockIdx.x, a.B, 0, threadIdx.x, a.tab, a.tw);
 }
 
-// k_wave with the requested transform; the matrix-core variants exist for
-// the mixed policy at N = 256 without a third body (check_cfg has rejected
-// every other combination).
+// k_wave with the requested transform; the matrix-core variants exist at
+// N = 256 without a third body, FP64 for the FP64 policy and the others for
+// mixed (check_cfg has rejected every other combination).
 template <int NN, int MODE, bool THIRD, class P>
 static void launch_k_wave(int transform, long long grid, hipStream_t st, const WaveArgs &aa) {
     constexpr int WG = Geo<NN, wave_tile<NN>()>::WG;
+    if constexpr (NN == 256 && !THIRD && std::is_same<P, F64>::value) {
+        if (transform == GBGPU_TRANSFORM_MFMA_F64) {
+            hipLaunchKernelGGL((k_wave<NN, MODE, THIRD, P, GBGPU_TRANSFORM_MFMA_F64>), dim3((unsigned)grid), dim3(WG),
+                               0, st, aa);
+            return;
+        }
+    }
     if constexpr (NN == 256 && !THIRD && std::is_same<P, Mixed>::value) {
         switch (transform) {
             case GBGPU_TRANSFORM_MFMA_F32:
@@ -107,13 +113,14 @@ static int launch_any(int N, bool third, int precision, const WaveArgs &a, hipSt
         return third ? launch_wave<MODE, true, Mixed>(N, a, st, ws, ws_bytes)
                      : launch_wave<MODE, false, Mixed>(N, a, st, ws, ws_bytes, transform);
     return third ? launch_wave<MODE, true, F64>(N, a, st, ws, ws_bytes)
-                 : launch_wave<MODE, false, F64>(N, a, st, ws, ws_bytes);
+                 : launch_wave<MODE, false, F64>(N, a, st, ws, ws_bytes, transform);
 }
 
 // a.dftc from the per-device constant cache when a matrix-core transform is
 // requested (check_cfg has validated the combination).
 static int set_transform(const gbgpu_wave_cfg *cfg, WaveArgs &a) {
-    if (cfg->transform == GBGPU_TRANSFORM_LDS) return GBGPU_OK;
+    // the FP64 variant takes its constants from the FP64 twiddles (a.tw)
+    if (cfg->transform == GBGPU_TRANSFORM_LDS || cfg->transform == GBGPU_TRANSFORM_MFMA_F64) return GBGPU_OK;
     a.dftc = dft256_table();
     if (!a.dftc) return hip_fail(hipErrorOutOfMemory, "transform constants");
     return GBGPU_OK;
@@ -140,13 +147,18 @@ int check_cfg(const double *params, int num_params, int num_bin, const gbgpu_wav
         set_error("unknown precision");
         return GBGPU_E_ARG;
     }
-    if (cfg->transform < GBGPU_TRANSFORM_LDS || cfg->transform > GBGPU_TRANSFORM_MFMA_BF16X6) {
+    if (cfg->transform < GBGPU_TRANSFORM_LDS || cfg->transform > GBGPU_TRANSFORM_MFMA_F64) {
         set_error("unknown transform");
         return GBGPU_E_ARG;
     }
+    if (cfg->transform != GBGPU_TRANSFORM_LDS && (cfg->N != 256 || num_params != GBGPU_NPARAMS)) {
+        set_error("the matrix-core transforms need N = 256 and no third body");
+        return GBGPU_E_ARG;
+    }
     if (cfg->transform != GBGPU_TRANSFORM_LDS &&
-        (cfg->precision != GBGPU_MIXED || cfg->N != 256 || num_params != GBGPU_NPARAMS)) {
-        set_error("the matrix-core transforms need precision GBGPU_MIXED, N = 256 and no third body");
+        (cfg->transform == GBGPU_TRANSFORM_MFMA_F64) != (cfg->precision == GBGPU_FP64)) {
+        set_error("GBGPU_TRANSFORM_MFMA_F64 goes with precision GBGPU_FP64, the other matrix-core transforms "
+                  "with GBGPU_MIXED");
         return GBGPU_E_ARG;
     }
     return GBGPU_OK;
@@ -245,8 +257,8 @@ int noalloc_base(char **out) {
     for (int prec = 0; prec < 2; prec++) {
         size_t total = 0, sort_bytes = 0;
         int rc = prec == GBGPU_MIXED
-                     ? pack_workspace_bytes<Mixed>(GBGPU_NOALLOC_MAX_BIN, nullptr, &total, &sort_bytes)
-                     : pack_workspace_bytes<F64>(GBGPU_NOALLOC_MAX_BIN, nullptr, &total, &sort_bytes);
+                     ? pack_workspace_bytes<Mixed>(GBGPU_NOALLOC_MAX_BIN, nullptr, &total, &sort_bytes, false)
+                     : pack_workspace_bytes<F64>(GBGPU_NOALLOC_MAX_BIN, nullptr, &total, &sort_bytes, false);
         if (rc) return rc;
         if (total > NOALLOC_WS_BYTES) {
             set_error("allocation-free scratch is smaller than the workspace of GBGPU_NOALLOC_MAX_BIN binaries");

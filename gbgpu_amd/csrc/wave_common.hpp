@@ -275,11 +275,13 @@ struct ThirdScratch {
 // comes from the previous lane (a shuffle) or the previous wavefront
 // (ThirdScratch).
 //
-// DFT != GBGPU_TRANSFORM_LDS (mixed policy, N = 256, one wavefront per
-// binary, no third body): each thread evaluates the samples the transform's
-// register layout gives it, xr[c][i] holding sample dft256_index(lane, i) of
-// channel c, and dft256_mfma transforms them in place (same index on exit);
-// buf and tw are not touched.
+// DFT != GBGPU_TRANSFORM_LDS (N = 256, one wavefront per binary, no third
+// body; MFMA_F64 for the FP64 policy, the others for mixed): each thread
+// evaluates the samples the transform's register layout gives it, xr[c][i]
+// holding sample dft256_index(lane, i) of channel c (lane + 64 i for
+// MFMA_F64, as with REGS), and dft256_mfma transforms them in place (same
+// index on exit); buf is not touched, and tw only supplies the FP64
+// variant's constants.
 //
 // REGS (LDS transform, circular binaries, all channels in one pass): the
 // samples stay in registers, xr[c * SPT + q] = sample lane + q TPB, and
@@ -296,8 +298,10 @@ __device__ __forceinline__ void binary_spectrum(const Src &s, const typename P::
     using CT = typename P::CT;
     constexpr int TPB = G::TPB, SPT = G::SPT;
     static_assert(DFT == GBGPU_TRANSFORM_LDS ||
-                      (N == 256 && TPB == 64 && !THIRD && std::is_same<CT, float2>::value),
-                  "the matrix-core transform covers mixed N = 256 without a third body");
+                      (N == 256 && TPB == 64 && !THIRD &&
+                       (DFT == GBGPU_TRANSFORM_MFMA_F64) == std::is_same<CT, double2>::value),
+                  "the matrix-core transforms cover N = 256 without a third body: the FP64 variant for FP64 "
+                  "spectra, the others for mixed");
     static_assert(C0 >= 0 && NCP >= 1 && C0 + NCP <= NCH && (DFT == GBGPU_TRANSFORM_LDS || NCP == NCH),
                   "channel pass out of range");
     static_assert(!EXT || (!THIRD && DFT == GBGPU_TRANSFORM_LDS && std::is_same<P, F64>::value),
@@ -392,7 +396,9 @@ __device__ __forceinline__ void binary_spectrum(const Src &s, const typename P::
         for (int q = 0; q < SPT; q++) {
             // third body: the thread's own consecutive samples (pre-pass values in
             // registers); otherwise strided, so orbit-table loads coalesce
-            const int n = THIRD ? SPT * lane + q : (DFT != GBGPU_TRANSFORM_LDS ? dft256_index(lane, q) : lane + q * TPB);
+            const int n = THIRD ? SPT * lane + q
+                                : (DFT != GBGPU_TRANSFORM_LDS && DFT != GBGPU_TRANSFORM_MFMA_F64 ? dft256_index(lane, q)
+                                                                                                 : lane + q * TPB);
             CT X{}, Y{}, Z{}, F{};
             if (valid) {
                 float v3[3] = {0.f, 0.f, 0.f};
@@ -442,7 +448,11 @@ __device__ __forceinline__ void binary_spectrum(const Src &s, const typename P::
         samples(std::integral_constant<int, NTIER>{});
     else
         with_tier(s.tier, samples);
-    if constexpr (DFT != GBGPU_TRANSFORM_LDS) {
+    if constexpr (DFT == GBGPU_TRANSFORM_MFMA_F64) {
+        const Dft256Lane<DFT> k = load_dft256_f64(tw, lane);
+#pragma unroll
+        for (int c = 0; c < NCH; c++) dft256_mfma<DFT>(xr + c * SPT, k);
+    } else if constexpr (DFT != GBGPU_TRANSFORM_LDS) {
         const Dft256Lane<DFT> k = load_dft256<DFT>(dftc, lane);
 #pragma unroll
         for (int c = 0; c < NCH; c++) dft256_mfma<DFT>(xr + c * SPT, k);

@@ -75,19 +75,20 @@ __device__ __forceinline__ void wave_pass(const WaveArgs &a, WaveLds<N, MODE, TH
     double *tot = stot + slot * WPB;
     ThirdScratch ts{L.s_edge + slot * W::NEDGE};
 
-    // spectra in registers: the matrix-core transform's layout (MFMA), or
-    // the LDS transform with register ends (circular binaries)
-    constexpr bool MFMA = DFT != GBGPU_TRANSFORM_LDS;
-    constexpr bool REG = MFMA || !THIRD;
+    // spectra in registers: the FP32/bf16 matrix-core transforms' layout
+    // (MFMA), or bin lane + q TPB in register q (the LDS transform with
+    // register ends for circular binaries, and the FP64 matrix-core transform)
+    constexpr bool MFMA = DFT != GBGPU_TRANSFORM_LDS && DFT != GBGPU_TRANSFORM_MFMA_F64;
+    constexpr bool REG = DFT != GBGPU_TRANSFORM_LDS || !THIRD;
     CT xr[REG ? NCH * SPT : 1];
-    binary_spectrum<N, NCH, THIRD, P, M, DFT, 0, NCH, EXT, REG && !MFMA>(
+    binary_spectrum<N, NCH, THIRD, P, M, DFT, 0, NCH, EXT, DFT == GBGPU_TRANSFORM_LDS && REG>(
         s, per_group<TPB>(aux_of<P, PK>(*pk)), valid, lane, tab, a.fstar_inv, a.scale, buf,
         static_cast<const CT *>(twp), ts, tot, xr, a.dftc, (EXT && valid) ? a.ext + b * 6 * N : nullptr);
     // output q of this thread: fftshifted position pos(q) (natural bin k sits
     // at position (k + N/2) mod N) and its value spec(c, q) in channel c.
     // Positions lane + q TPB keep the data reads at constant offsets; with
-    // the register-ended LDS transform their bins (lane + q TPB + N/2) mod N
-    // are registers (q + SPT/2) mod SPT.
+    // the register-ended LDS transform and the FP64 matrix-core transform
+    // their bins (lane + q TPB + N/2) mod N are registers (q + SPT/2) mod SPT.
     auto pos = [&](int q) {
         if constexpr (MFMA)
             return (dft256_index(lane, q) + N / 2) & (N - 1);

@@ -103,12 +103,14 @@ class GBGPUBase(abc.ABC):
 
     @property
     def transform(self):
-        """Transform of the slow part for mixed-precision N = 256 calls of
-        run_wave, run_wave_ae-style templates and get_ll without a third body:
-        "lds" (radix-4 transform in LDS, default), "mfma_f32", "mfma_bf16x3"
-        or "mfma_bf16x6" (two 16 x 16 complex matrix products on the matrix
-        cores; bf16x3 carries about 16 significant bits, the others FP32
-        accuracy).  Every other call uses "lds"."""
+        """Transform of the slow part for N = 256 calls of run_wave,
+        run_wave_ae-style templates and get_ll without a third body: "lds"
+        (radix-4 transform in LDS, default), or two 16 x 16 complex matrix
+        products on the matrix cores: "mfma_f64" for FP64 calls (FP64
+        products and sums), "mfma_f32", "mfma_bf16x3" or "mfma_bf16x6" for
+        mixed-precision calls (bf16x3 carries about 16 significant bits, the
+        others FP32 accuracy).  A call of the other precision, or any other
+        call, uses "lds"."""
         return self._transform
 
     @transform.setter
@@ -119,7 +121,7 @@ class GBGPUBase(abc.ABC):
 
     def _transform_code(self, N, precision, num_params):
         p = precision or self.precision
-        if p == "mixed" and int(N) == 256 and num_params == 9:
+        if int(N) == 256 and num_params == 9 and _capi.TRANSFORM_PRECISION.get(self._transform) == p:
             return _capi.TRANSFORMS[self._transform]
         return 0
 

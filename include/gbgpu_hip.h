@@ -36,17 +36,24 @@ enum { GBGPU_FP64 = 0, GBGPU_MIXED = 1 };
 /* Length-N transform of the slow part (the reference's xp.fft.fft,
  * src/gbgpu/gbgpu.py:378).  GBGPU_TRANSFORM_LDS: radix-4 Stockham transform
  * in LDS, every N and precision.  The MFMA variants evaluate the transform as
- * two 16 x 16 complex matrix products on the matrix cores, in registers, and
- * exist for GBGPU_MIXED at N = 256 without a third body only (other
- * combinations return GBGPU_E_ARG): _F32 with FP32 matrix instructions,
+ * two 16 x 16 complex matrix products on the matrix cores, in registers, at
+ * N = 256 without a third body only (other combinations return
+ * GBGPU_E_ARG).  For GBGPU_MIXED: _F32 with FP32 matrix instructions,
  * _BF16X3 / _BF16X6 with bf16 matrix instructions on a two-piece (three
  * products, about 16 significant bits) or three-piece (six products, FP32
- * level) split of each operand. */
+ * level) split of each operand.  For GBGPU_FP64: _F64 with FP64 matrix
+ * instructions (FP64 products and sums; results agree with the LDS
+ * transform to rounding).  MI355X's FP64 matrix and vector peaks are equal
+ * and the 16 x 16 products take about four times the arithmetic of the
+ * radix-4 transform, so _F64 is slower than the LDS transform there; it is
+ * kept selectable for GPUs whose FP64 matrix rate exceeds their vector
+ * rate. */
 enum {
     GBGPU_TRANSFORM_LDS = 0,
     GBGPU_TRANSFORM_MFMA_F32 = 1,
     GBGPU_TRANSFORM_MFMA_BF16X3 = 2,
-    GBGPU_TRANSFORM_MFMA_BF16X6 = 3
+    GBGPU_TRANSFORM_MFMA_BF16X6 = 3,
+    GBGPU_TRANSFORM_MFMA_F64 = 4
 };
 
 /* Parameter rows of the (num_params, num_bin) float64 parameter array:
@@ -337,7 +344,7 @@ int gbgpu_wave_ll_ext(double *loglike, gbgpu_cmplx *d_h, gbgpu_cmplx *h_h, const
 /* Message describing the last failure on the calling thread. */
 const char *gbgpu_last_error(void);
 
-/* ABI version of this header (8). */
+/* ABI version of this header (9). */
 int gbgpu_abi_version(void);
 
 /* Measurement hook with no counterpart in the reference: while enabled,

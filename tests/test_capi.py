@@ -32,7 +32,7 @@ def test_library_exports_every_declared_symbol():
         assert hasattr(lib, n), n
     # the ctypes table covers exactly the declared functions
     assert set(_capi.SIGNATURES) == set(declared_functions())
-    assert lib.gbgpu_abi_version() == 8
+    assert lib.gbgpu_abi_version() == 9
 
 
 def test_invalid_arguments_return_errors_without_gpu_work():
@@ -145,6 +145,32 @@ def test_N_limits_checked_before_gpu_work():
         assert lib.gbgpu_wave_ll_grad(P, P, None, None, P, 9, 4, cfg, dat, 0.0, 0, None) == -1
         assert b"2048" in lib.gbgpu_last_error()
         assert lib.gbgpu_fisher(P, None, None, P, 9, 4, cfg, dat, 0.0, None) == -1
+
+
+def test_matrix_core_transforms_checked_against_precision():
+    """GBGPU_TRANSFORM_MFMA_F64 goes with FP64 only and the FP32/bf16
+    variants with mixed only, all at N = 256 without a third body: other
+    pairs return GBGPU_E_ARG before any HIP call (dummy pointers)."""
+    from gbgpu_amd import _capi
+
+    lib = _capi.load()
+    P = 8
+    dat = _capi.Data(A=P, E=P, psdA=P, psdE=P, data_index=None, noise_index=None, data_length=10,
+                     start_freq_ind=0, df=1e-7)
+    T = _capi.TRANSFORMS
+    for prec, tr, ok in ((0, T["mfma_f64"], True), (1, T["mfma_f64"], False), (0, T["mfma_f32"], False),
+                         (1, T["mfma_bf16x6"], True), (0, 5, False)):
+        cfg = _capi.WaveCfg(N=256, T=1e7, armlength=2.5e9, orbit_table=P, tdi2=0, precision=prec, transform=tr)
+        # an empty batch runs nothing, so accepted pairs return 0
+        rc = lib.gbgpu_wave_ll(P, None, None, P, 9, 0, cfg, dat, 0.0, 0, None)
+        assert (rc == 0) == ok, (prec, tr, rc, lib.gbgpu_last_error())
+        if not ok:
+            assert b"transform" in lib.gbgpu_last_error()
+    cfg = _capi.WaveCfg(N=512, T=1e7, armlength=2.5e9, orbit_table=P, tdi2=0, precision=0,
+                        transform=T["mfma_f64"])
+    assert lib.gbgpu_run_wave(P, P, P, P, 9, 0, cfg, None) == -1
+    assert b"N = 256" in lib.gbgpu_last_error()
+    assert lib.gbgpu_run_wave(P, P, P, P, 14, 0, cfg, None) == -1
 
 
 def test_missing_library_fails_loudly(tmp_path):
