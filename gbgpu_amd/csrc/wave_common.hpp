@@ -112,15 +112,29 @@ struct Pack {
     typename P::Aux aux;
 };
 
+// One thread per binary, 256 per workgroup.  The workgroup's packs are
+// staged in LDS and written out as consecutive 8-byte words, so that each
+// store instruction covers 512 contiguous bytes rather than 64 lines one
+// pack (240 or 304 bytes) apart.
 template <class P>
 __global__ void __launch_bounds__(256) k_setup(WaveArgs a, int N, Pack<P> *__restrict__ out) {
-    const long long b = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (b >= a.B) return;
+    static_assert(sizeof(Pack<P>) % 8 == 0 && alignof(Pack<P>) <= 8, "packs are whole 8-byte words");
+    constexpr int WORDS = (int)(sizeof(Pack<P>) / 8);
+    __shared__ unsigned long long stage[256 * WORDS];
+    const long long b0 = (long long)blockIdx.x * 256;
+    const long long b = b0 + threadIdx.x;
+    const bool valid = b < a.B;
     Pack<P> pk;
-    pk.s = make_src(a.params, a.num_params, a.B, b, N, a.T, a.tdi2, a.armlength);
-    pk.aux = P::make_aux(pk.s);
-    out[b] = pk;
-    if (a.keys) {
+    if (valid) {
+        pk.s = make_src(a.params, a.num_params, a.B, b, N, a.T, a.tdi2, a.armlength);
+        pk.aux = P::make_aux(pk.s);
+        reinterpret_cast<Pack<P> *>(stage)[threadIdx.x] = pk;
+    }
+    __syncthreads();
+    const int words = (int)(a.B - b0 < 256 ? a.B - b0 : 256) * WORDS;
+    unsigned long long *dst = reinterpret_cast<unsigned long long *>(out + b0);
+    for (int i = threadIdx.x; i < words; i += 256) dst[i] = stage[i];
+    if (valid && a.keys) {
         // order by start bin: neighbouring workgroups then read overlapping
         // data/PSD windows, which stay resident in L2.  24-bit keys (start
         // bins within +-2^23, f0 up to 66 mHz at 4 yr; others clamped, which
