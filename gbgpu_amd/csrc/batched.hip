@@ -5,15 +5,16 @@
 // uneven batch costs one call per N.  Here every binary carries its own N
 // (a power of two in [32, 8192]) and one call does, on the device:
 //   k_batch_reset    clear the control block (a fill kernel, no memset)
-//   k_batch_setup    per-binary constants (Pack) and a histogram over
-//                    (N bucket, route, log-spaced start-bin cell); with the
-//                    mixed accuracy guard, binaries whose estimated <h|h>
-//                    already fails it are routed to an FP64 segment
-//   k_batch_scan     offsets of every histogram cell (one workgroup), and
-//                    per-bucket counts and list offsets
-//   k_batch_scatter  binaries into one list ordered by bucket, then by
-//                    start-bin cell (neighbouring binaries read
-//                    neighbouring data windows, which stay in L2)
+//   k_batch_setup    per-binary constants (Pack) and a sort key (segment:
+//                    N bucket and route, then log-spaced start-bin cell);
+//                    with the mixed accuracy guard, binaries whose
+//                    estimated <h|h> already fails it are routed to an FP64
+//                    segment
+//   lsd_sort         the binaries into one list ordered by that key
+//                    (neighbouring binaries read neighbouring data windows,
+//                    which stay in L2); stable, so the list is deterministic
+//   k_batch_bounds   each segment's first and last list position
+//   k_batch_table    per-segment counts and the persistent work table
 // and then either
 //   strategy 0: one launch of the uniform fused kernel per N bucket, each a
 //               grid-stride loop over its segment of the list, reading the
@@ -28,6 +29,7 @@
 #include <atomic>
 
 #include "common.hpp"
+#include "sort.hpp"
 #include "wave_common.hpp"
 
 namespace gbgpu {
@@ -47,11 +49,16 @@ constexpr int NSEG = 2 * NBUCKET;
 constexpr int NCB = 4096;  // start-bin cells per segment (log-spaced)
 constexpr int NCELL = NSEG * NCB;
 enum { CTL_COUNT = 0, CTL_LISTOFF = 32, CTL_ITEMOFF = 64, CTL_ITEMS = 80, CTL_TOTAL = 96, CTL_WORK = 97,
-       CTL_REDO = 104, CTL_HIST = 128, CTL_CUR = CTL_HIST + NCELL, CTL_WORDS = CTL_CUR + NCELL };
+       CTL_REDO = 104, CTL_WORDS = 128 };
 static_assert(CTL_LISTOFF - CTL_COUNT >= NSEG && CTL_ITEMOFF - CTL_LISTOFF >= NSEG &&
                   CTL_ITEMS - CTL_ITEMOFF >= NBUCKET && CTL_TOTAL - CTL_ITEMS >= NBUCKET &&
-                  CTL_HIST - CTL_REDO >= NBUCKET,
+                  CTL_WORDS - CTL_REDO >= NBUCKET,
               "control block layout");
+// Sort keys: segment * NCB + cell, below 2^BATCH_KEY_BITS; binaries that are
+// not evaluated get BATCH_KEY_NONE, which sorts last and lies in no segment.
+constexpr int BATCH_KEY_BITS = 17;
+constexpr unsigned BATCH_KEY_NONE = (1u << BATCH_KEY_BITS) - 1u;
+static_assert(NCELL <= (int)BATCH_KEY_NONE && BATCH_KEY_NONE / NCB >= (unsigned)NSEG, "sort key range");
 
 __host__ __device__ constexpr int bucket_bpw(int k) {
     return WG_SIZE / (((32 << k) / 4 < WG_SIZE) ? (32 << k) / 4 : WG_SIZE);
@@ -68,7 +75,8 @@ struct BatchArgs {
     const void *tws[NBUCKET];
     const int *Nb;
     int *list;
-    int *cell;  // histogram cell per binary (-1: not evaluated)
+    unsigned *key;  // sort key per binary (BATCH_KEY_NONE: not evaluated)
+    int *idx;       // binary indices 0 .. B-1, the values the sort carries
     int *redo;  // accuracy-guard list (same per-bucket offsets as `list`)
     int *ctl;
     long long cap[NBUCKET];  // host-side bound on each bucket's size (0: none)
@@ -99,11 +107,10 @@ __device__ __forceinline__ double hh_estimate(const Src &s, long long b, int N, 
     return 1.1 * s.amp * s.amp * a.T * r * r * (s.tdr * s.tdr + s.tdi * s.tdi) / a.SA[noff + j];
 }
 
-// Histogram cell of a binary: segment k, then its start bin within the data
-// on a logarithmic grid of NCB cells.  Galactic binaries crowd at low
-// frequency; log-spaced cells keep the per-cell counts (and so the atomic
-// contention on each counter) even, while neighbouring binaries still land
-// in neighbouring cells.
+// Sort key of a binary: segment k, then its start bin within the data on a
+// logarithmic grid of NCB cells.  Galactic binaries crowd at low frequency;
+// log-spaced cells resolve the crowded band finely, and neighbouring
+// binaries land in neighbouring cells.
 __device__ __forceinline__ int batch_cell(int k, long long start, const WaveArgs &a) {
     long long j = start - a.start_freq_ind;
     j = j < 0 ? 0 : (j >= a.L ? a.L - 1 : j);
@@ -125,7 +132,8 @@ __global__ void __launch_bounds__(256) k_batch_setup(BatchArgs ba, Pack<P> *__re
         ba.a.loglike[b] = nan;
         if (ba.a.d_h) ba.a.d_h[b] = make_double2(nan, nan);
         if (ba.a.h_h) ba.a.h_h[b] = make_double2(nan, nan);
-        ba.cell[b] = -1;
+        ba.key[b] = BATCH_KEY_NONE;
+        ba.idx[b] = (int)b;
         return;
     }
     Pack<P> pk;
@@ -134,68 +142,37 @@ __global__ void __launch_bounds__(256) k_batch_setup(BatchArgs ba, Pack<P> *__re
     pack[b] = pk;
     const bool routed = ba.route_tol > 0.0 && GUARD_REL * hh_estimate(pk.s, b, N, ba.a) > ba.route_tol;
     const int cell = batch_cell(routed ? NBUCKET + k : k, pk.s.start, ba.a);
-    ba.cell[b] = cell;
-    atomicAdd(&ba.ctl[CTL_HIST + cell], 1);
+    ba.key[b] = (unsigned)cell;
+    ba.idx[b] = (int)b;
 }
 
-// Exclusive scan of the histogram (one workgroup of 1024 threads), the
-// scatter cursors, per-bucket counts and list offsets (ascending bucket), and
-// the persistent strategy's work-item table (descending bucket).
-__global__ void __launch_bounds__(1024) k_batch_scan(int *ctl) {
-    __shared__ int part[1024];
-    __shared__ int s_total;
-    constexpr int PER = (NCELL + 1023) / 1024;
-    const int t = threadIdx.x;
-    const int lo = t * PER, hi = (t + 1) * PER < NCELL ? (t + 1) * PER : NCELL;
-    int sum = 0;
-    for (int i = lo; i < hi; i++) sum += ctl[CTL_HIST + i];
-    part[t] = sum;
-    __syncthreads();
-    if (t == 0) {
-        int run = 0;
-        for (int i = 0; i < 1024; i++) {
-            const int v = part[i];
-            part[i] = run;
-            run += v;
-        }
-        s_total = run;
-    }
-    __syncthreads();
-    int run = part[t];
-    for (int i = lo; i < hi; i++) {
-        const int v = ctl[CTL_HIST + i];
-        ctl[CTL_HIST + i] = run;  // cell offset
-        ctl[CTL_CUR + i] = run;   // scatter cursor
-        run += v;
-    }
-    __syncthreads();
-    if (t == 0) {
-        // segment k spans [offset of cell k NCB, offset of cell (k+1) NCB)
-        for (int k = 0; k < NSEG; k++) {
-            const int o0 = ctl[CTL_HIST + k * NCB];
-            const int o1 = k + 1 < NSEG ? ctl[CTL_HIST + (k + 1) * NCB] : s_total;
-            ctl[CTL_LISTOFF + k] = o0;
-            ctl[CTL_COUNT + k] = o1 - o0;
-        }
-        int item_run = 0;
-        for (int k = NBUCKET_PERSISTENT - 1; k >= 0; k--) {
-            const int c = ctl[CTL_COUNT + k];
-            const int bpw = bucket_bpw(k);
-            ctl[CTL_ITEMOFF + k] = item_run;
-            ctl[CTL_ITEMS + k] = (c + bpw - 1) / bpw;
-            item_run += (c + bpw - 1) / bpw;
-        }
-        ctl[CTL_TOTAL] = item_run;
-    }
+// Segment boundaries in the sorted keys: the first list position of
+// segment s goes to ctl[CTL_LISTOFF + s] and one past its last to
+// ctl[CTL_COUNT + s] (k_batch_table turns it into a count); both stay zero
+// for an empty segment.
+__global__ void k_batch_bounds(const unsigned *__restrict__ key, long long B, int *__restrict__ ctl) {
+    const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= B) return;
+    const unsigned s = key[i] / NCB;
+    if (s >= (unsigned)NSEG) return;
+    if (i == 0 || key[i - 1] / NCB != s) ctl[CTL_LISTOFF + s] = (int)i;
+    if (i == B - 1 || key[i + 1] / NCB != s) ctl[CTL_COUNT + s] = (int)(i + 1);
 }
 
-__global__ void k_batch_scatter(BatchArgs ba) {
-    const long long b = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (b >= ba.a.B) return;
-    const int cell = ba.cell[b];
-    if (cell < 0) return;
-    const int pos = atomicAdd(&ba.ctl[CTL_CUR + cell], 1);
-    ba.list[pos] = (int)b;
+// Per-segment counts, and the persistent strategy's work-item table
+// (descending bucket).  One thread.
+__global__ void k_batch_table(int *ctl) {
+    if (threadIdx.x != 0 || blockIdx.x != 0) return;
+    for (int k = 0; k < NSEG; k++) ctl[CTL_COUNT + k] -= ctl[CTL_LISTOFF + k];
+    int item_run = 0;
+    for (int k = NBUCKET_PERSISTENT - 1; k >= 0; k--) {
+        const int c = ctl[CTL_COUNT + k];
+        const int bpw = bucket_bpw(k);
+        ctl[CTL_ITEMOFF + k] = item_run;
+        ctl[CTL_ITEMS + k] = (c + bpw - 1) / bpw;
+        item_run += (c + bpw - 1) / bpw;
+    }
+    ctl[CTL_TOTAL] = item_run;
 }
 
 template <int N, class P>
@@ -322,10 +299,18 @@ struct BatchKernel<Mixed> {
     static void *fn() { return (void *)k_batch_ll_mixed; }
 };
 
+// Byte offset of the control block in a batch workspace of B binaries whose
+// packs take pack_bytes each: packs, then five int arrays of B (list, keys,
+// redo list, indices, sorted keys), each piece 256-byte aligned.
+inline size_t batch_ctl_offset(size_t pack_bytes, long long B) {
+    auto al = [](size_t n) { return (n + 255) & ~(size_t)255; };
+    return al(pack_bytes * (size_t)B) + 5 * al(sizeof(int) * (size_t)B);
+}
+
 template <class P>
 size_t batch_ws_bytes(long long B) {
     auto al = [](size_t n) { return (n + 255) & ~(size_t)255; };
-    return al(sizeof(Pack<P>) * (size_t)B) + 3 * al(sizeof(int) * (size_t)B) + al(sizeof(int) * CTL_WORDS);
+    return batch_ctl_offset(sizeof(Pack<P>), B) + al(sizeof(int) * CTL_WORDS) + lsd_sort_bytes<unsigned>((int)B);
 }
 
 template <class P>
@@ -442,18 +427,30 @@ static int launch_batch(BatchArgs ba, int precision, int strategy, void *workspa
         own = true;
     }
     auto al = [](size_t n) { return (n + 255) & ~(size_t)255; };
-    Pack<P> *pack = reinterpret_cast<Pack<P> *>(ws);
-    ba.list = reinterpret_cast<int *>(ws + al(sizeof(Pack<P>) * (size_t)B));
-    ba.cell = reinterpret_cast<int *>(ws + al(sizeof(Pack<P>) * (size_t)B) + al(sizeof(int) * (size_t)B));
-    ba.redo = reinterpret_cast<int *>(ws + al(sizeof(Pack<P>) * (size_t)B) + 2 * al(sizeof(int) * (size_t)B));
-    ba.ctl = reinterpret_cast<int *>(ws + al(sizeof(Pack<P>) * (size_t)B) + 3 * al(sizeof(int) * (size_t)B));
+    // layout (each piece 256-byte aligned): packs, list, keys, redo list,
+    // indices, sorted keys, control block, sort storage
+    const size_t ib = al(sizeof(int) * (size_t)B);
+    char *p = ws;
+    Pack<P> *pack = reinterpret_cast<Pack<P> *>(p);
+    p += al(sizeof(Pack<P>) * (size_t)B);
+    ba.list = reinterpret_cast<int *>(p);
+    ba.key = reinterpret_cast<unsigned *>(p + ib);
+    ba.redo = reinterpret_cast<int *>(p + 2 * ib);
+    ba.idx = reinterpret_cast<int *>(p + 3 * ib);
+    unsigned *key_sorted = reinterpret_cast<unsigned *>(p + 4 * ib);
+    ba.ctl = reinterpret_cast<int *>(ws + batch_ctl_offset(sizeof(Pack<P>), B));
+    void *sort_tmp = ws + batch_ctl_offset(sizeof(Pack<P>), B) + al(sizeof(int) * CTL_WORDS);
     ba.a.pack = pack;
     const unsigned gb = (unsigned)((B + 255) / 256);
     hipLaunchKernelGGL(k_batch_reset, dim3((CTL_WORDS + 255) / 256), dim3(256), 0, st, ba.ctl);
     hipLaunchKernelGGL((k_batch_setup<P>), dim3(gb), dim3(256), 0, st, ba, pack);
-    hipLaunchKernelGGL(k_batch_scan, dim3(1), dim3(1024), 0, st, ba.ctl);
-    hipLaunchKernelGGL(k_batch_scatter, dim3(gb), dim3(256), 0, st, ba);
-    hipError_t e = hipGetLastError();
+    // lsd_sort for every size: no memset, so the sequence stays capturable
+    hipError_t e = lsd_sort<unsigned>(sort_tmp, ba.key, key_sorted, ba.idx, ba.list, (int)B, BATCH_KEY_BITS, st);
+    if (e == hipSuccess) {
+        hipLaunchKernelGGL(k_batch_bounds, dim3(gb), dim3(256), 0, st, key_sorted, B, ba.ctl);
+        hipLaunchKernelGGL(k_batch_table, dim3(1), dim3(64), 0, st, ba.ctl);
+        e = hipGetLastError();
+    }
     int rc = e == hipSuccess ? GBGPU_OK : hip_fail(e, "batch set-up");
     if (!rc) rc = strategy == GBGPU_BATCH_PERSISTENT ? launch_persistent<P>(ba, st)
                                                      : launch_buckets<P>(ba, precision, st, variants, bucket_events);
@@ -699,10 +696,8 @@ extern "C" int gbgpu_batch_guard_counts(const void *workspace, int num_bin, int 
         set_error("gbgpu_batch_guard_counts: bad arguments");
         return GBGPU_E_ARG;
     }
-    auto al = [](size_t n) { return (n + 255) & ~(size_t)255; };
     const size_t pack = precision == GBGPU_MIXED ? sizeof(Pack<Mixed>) : sizeof(Pack<F64>);
-    const char *ctl = static_cast<const char *>(workspace) + al(pack * (size_t)num_bin) +
-                      3 * al(sizeof(int) * (size_t)num_bin);
+    const char *ctl = static_cast<const char *>(workspace) + batch_ctl_offset(pack, num_bin);
     hipStream_t st = static_cast<hipStream_t>(stream);
     // binaries evaluated in FP64: re-evaluated after the mixed pass, plus
     // those routed to FP64 at set-up
