@@ -1,14 +1,15 @@
-"""Batches above 2^18 binaries, where the allocating entry points sort by
-start bin with rocPRIM's onesweep radix sort (smaller batches merge-sort;
-below 8192 binaries nothing is sorted).
+"""Large batches: above 2^17 binaries the allocating entry points order by
+start bin with the library's LSD radix sort, above 2^21 with rocPRIM's
+onesweep (smaller batches merge-sort; below 8192 binaries nothing is
+sorted).
 
 * fused likelihood: each binary's arithmetic does not depend on the order
   the binaries run in, so one 300,000-binary call must equal, bit for bit,
   the same binaries scored in unsorted chunks of 4000;
-* fill: 300,000 synthetic rows summed into one 4-yr stream, against the
-  serial C restatement of the reference's fill_global
-  (gbgpu_utils.cu:104-122) to 1e-12 of the largest bin, and bit-identical
-  on a repeat.
+* fill: 300,000 (LSD) and 2,200,000 (onesweep) synthetic rows summed into
+  one 4-yr stream, against the serial C restatement of the reference's
+  fill_global (gbgpu_utils.cu:104-122) to 1e-12 of the largest bin, and
+  bit-identical on a repeat.
 """
 
 import ctypes
@@ -74,7 +75,8 @@ def test_large_sorted_batch_matches_unsorted_chunks(precision):
     assert np.median(ll[:64]) > np.median(ll[64:])
 
 
-def test_fill_large_batch_matches_oracle():
+@pytest.mark.parametrize("B,M", [(BIG, 128), (2_200_000, 32)])
+def test_fill_large_batch_matches_oracle(B, M):
     import torch
 
     from gbgpu_amd import _capi
@@ -82,22 +84,22 @@ def test_fill_large_batch_matches_oracle():
     lib = _capi.load()
     c = oracle_c()
     P = lambda a: a.ctypes.data_as(ctypes.c_void_p)  # noqa: E731
-    rng = np.random.default_rng(952)
-    M, L = 128, 4_207_681
-    start = rng.integers(-M // 2, L - M // 2, BIG).astype(np.int64)  # some rows cross either end
-    rows_A = (rng.standard_normal((BIG, M)) + 1j * rng.standard_normal((BIG, M))).astype(np.complex128)
-    rows_E = (rng.standard_normal((BIG, M)) + 1j * rng.standard_normal((BIG, M))).astype(np.complex128)
+    rng = np.random.default_rng(952 + B)
+    L = 4_207_681
+    start = rng.integers(-M // 2, L - M // 2, B).astype(np.int64)  # some rows cross either end
+    rows_A = (rng.standard_normal((B, M)) + 1j * rng.standard_normal((B, M))).astype(np.complex128)
+    rows_E = (rng.standard_normal((B, M)) + 1j * rng.standard_normal((B, M))).astype(np.complex128)
     tA_rows = torch.from_numpy(rows_A).to("cuda")
     tE_rows = torch.from_numpy(rows_E).to("cuda")
     st = torch.from_numpy(start).to("cuda")
-    g0 = torch.zeros(BIG, dtype=torch.int32, device="cuda")
+    g0 = torch.zeros(B, dtype=torch.int32, device="cuda")
     runs = []
     for rep in range(2):
-        print(f"fill large batch B={BIG} M={M} rep={rep}", flush=True)
+        print(f"fill large batch B={B} M={M} rep={rep}", flush=True)
         gA = torch.zeros(L, dtype=torch.complex128, device="cuda")
         gE = torch.zeros(L, dtype=torch.complex128, device="cuda")
         rc = lib.gbgpu_fill_global_rows(gA.data_ptr(), gE.data_ptr(), tA_rows.data_ptr(), tE_rows.data_ptr(), M,
-                                        st.data_ptr(), 0, M, BIG, g0.data_ptr(), L, 1, _capi.stream_handle())
+                                        st.data_ptr(), 0, M, B, g0.data_ptr(), L, 1, _capi.stream_handle())
         _capi.check(rc, "gbgpu_fill_global_rows")
         torch.cuda.synchronize()
         runs.append((gA.cpu().numpy(), gE.cpu().numpy()))
@@ -106,8 +108,8 @@ def test_fill_large_batch_matches_oracle():
     oE = np.zeros(L, np.complex128)
     At = np.ascontiguousarray(rows_A.T)
     Et = np.ascontiguousarray(rows_E.T)
-    c.oracle_fill_global(P(oA), P(oE), P(At), P(Et), P(start.astype(np.int32)), M, BIG,
-                         P(np.zeros(BIG, np.int32)), L)
+    c.oracle_fill_global(P(oA), P(oE), P(At), P(Et), P(start.astype(np.int32)), M, B,
+                         P(np.zeros(B, np.int32)), L)
     hA, hE = runs[0]
     for h, o in ((hA, oA), (hE, oE)):
         assert np.abs(h - o).max() <= 1e-12 * np.abs(o).max()
